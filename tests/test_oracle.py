@@ -1,0 +1,333 @@
+"""Pins the CPU oracle (oracle/al_oracle.c) before anything is checked against it.
+
+Each check is chosen so that a plausible transcription mistake -- a dropped term, a wrong
+index, a transposed operand, a reversed swap order -- fails it:
+  * golden vectors the crate's own tests hold (tests/golden/reference_fixtures.json);
+  * closed forms and hand-computed answers of the crate's tests;
+  * definitions computed independently (numpy on an explicitly transposed operand);
+  * scipy's LAPACK (OpenBLAS) on the src/parity.rs sample matrices;
+  * planted mistakes that the checkers must flag.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+import oracle as O
+from matrices import (SWEEP_ALPHA_BETA, SWEEP_SHAPES, cm, hilbert, lower3, mat, sample, sample4, singular3, spd,
+                      sweep_operands, transpose_flat, upper3)
+
+FIX = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_fixtures.json")))
+SEEDY = [1988, 1989, 1990, 1991]
+DIST = {"Uniform": O.UNIFORM, "Symmetric": O.SYMMETRIC, "Normal": O.NORMAL}
+
+
+# ------------------------------------------------------------------------------------------
+# generators: the LAPACK stream bit for bit (src/lapack/families.rs:1182-1258)
+# ------------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("case", FIX["larnv"]["cases"], ids=lambda c: c["dist"])
+def test_larnv_matches_the_reference_stream(case):
+    x, seed = O.larnv(DIST[case["dist"]], SEEDY, 200)
+    for i, w in enumerate(case["first12"]):
+        assert x[i] == w, (i, x[i], w)  # bitwise (float equality on finite values)
+    assert x[198] == case["last2"][0] and x[199] == case["last2"][1]
+    assert list(seed) == case["seed_after"]
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 129, 200])
+def test_larnv_is_continuous_across_batches(n):
+    # src/lapack/families.rs:1261-1278: a batch of n equals n one-at-a-time draws with the
+    # seed carried over, and both leave the same seed.
+    whole, s_all = O.larnv(O.UNIFORM, SEEDY, n)
+    seed = np.array(SEEDY, dtype=np.uint32)
+    one = []
+    for _ in range(n):
+        v, seed = O.larnv(O.UNIFORM, seed, 1)
+        one.append(v[0])
+    assert np.array_equal(whole, np.array(one))
+    assert list(s_all) == list(seed)
+
+
+@pytest.mark.parametrize("idx", range(12))
+def test_dlatms_matches_the_reference_matrices(idx):
+    c = FIX["dlatms"]["cases"][idx]
+    sym = {"NonSymmetric": O.NONSYMMETRIC, "PositiveSymmetric": O.POSITIVE_SYMMETRIC,
+           "Symmetric": O.SYMMETRIC_SYM}[c["sym"]]
+    pack = {"Full": O.FULL, "Upper": O.UPPER, "Lower": O.LOWER}[c["pack"]]
+    info, a, d, seed = O.dlatms(DIST[c["dist"]], sym, c["mode"], c["cond"], c["dmax"], pack, c["m"], c["n"],
+                                SEEDY, d=c["d"])
+    assert info == 0
+    assert list(seed) == c["seed_after"], "the draws' order and count"
+    for got, want in zip(d, c["d"]):
+        assert (got == 0.0) if want == 0.0 else abs(got - want) / abs(want) <= 1e-13
+    for got, want in zip(a, c["a"]):
+        assert (got == 0.0) if want == 0.0 else abs(got - want) / abs(want) <= 1e-13
+
+
+def test_dlatms_mode3_spectrum_is_geometric_and_matrix_has_it():
+    # config-3 generator: sigma_i = cond^(-i/(n-1)); the singular values of U D V are D.
+    n, cond = 24, 1e5
+    info, a, d, _ = O.dlatms(O.NORMAL, O.NONSYMMETRIC, 3, cond, 1.0, O.FULL, n, n, SEEDY)
+    assert info == 0
+    want = cond ** (-np.arange(n) / (n - 1.0))
+    assert np.allclose(d, want, rtol=1e-14)
+    sv = np.linalg.svd(mat(a, n, n), compute_uv=False)
+    assert np.allclose(np.sort(sv)[::-1], want, rtol=1e-12)
+
+
+# ------------------------------------------------------------------------------------------
+# dgemm: fixtures, the exact integer case, and the definition on every blocking branch
+# ------------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("name", ["dgemm_tn", "dgemm_nt", "dgemm_tt"])
+@pytest.mark.parametrize("avx", [1, 0])
+def test_dgemm_variants_match_cblas_fixtures(name, avx):
+    f = FIX[name]
+    O.set_avx(avx)
+    try:
+        fn = getattr(O, name)
+        for (alpha, beta), want in zip(f["alpha_beta"], (f["want"], f["want_ab"])):
+            got = fn(alpha, f["a"], f["b"], beta, f["c0"], f["m"], f["n"], f["k"])
+            scale = np.maximum(np.abs(want), 1.0)
+            assert np.all(np.abs(got - np.array(want)) / scale < 1e-13)
+    finally:
+        O.set_avx(-1)
+
+
+def test_dgemm_tn_exact_on_integers():
+    f = FIX["dgemm_tn_exact"]
+    got = O.dgemm_tn(1.0, f["a"], f["b"], 0.0, np.zeros(12), f["m"], f["n"], f["k"])
+    assert np.array_equal(got, np.array(f["want"]))
+
+
+def test_dgemm_hand_computation_and_beta():
+    # src/blas.rs:828-849
+    a = np.array([1.0, 4.0, 2.0, 5.0, 3.0, 6.0])
+    b = np.array([7.0, 9.0, 11.0, 8.0, 10.0, 12.0])
+    assert list(O.dgemm(1.0, a, b, 0.0, np.zeros(4), 2, 2, 3)) == [58.0, 139.0, 64.0, 154.0]
+    assert list(O.dgemm(1.0, a, b, 2.0, np.ones(4), 2, 2, 3)) == [60.0, 141.0, 66.0, 156.0]
+
+
+@pytest.mark.parametrize("m,n,k", SWEEP_SHAPES)
+@pytest.mark.parametrize("avx", [1, 0])
+def test_gemm_all_variants_match_the_definition(m, n, k, avx):
+    # src/blas.rs:747-794 / :2264-2319: the reference transposes explicitly and multiplies.
+    a_op, b_op, c0 = sweep_operands(m, n, k)
+    A, B, C0 = mat(a_op, m, k), mat(b_op, k, n), mat(c0, m, n)
+    a_t, b_t = transpose_flat(a_op, m, k), transpose_flat(b_op, k, n)
+    O.set_avx(avx)
+    try:
+        for ta in (False, True):
+            for tb in (False, True):
+                a_st = a_t if ta else a_op
+                b_st = b_t if tb else b_op
+                for alpha, beta in SWEEP_ALPHA_BETA:
+                    want = alpha * (A @ B) + (0.0 if beta == 0.0 else beta * C0)
+                    got = mat(O.gemm(ta, tb, alpha, a_st, b_st, beta, c0, m, n, k), m, n)
+                    assert np.all(np.abs(got - want) / np.maximum(np.abs(want), 1.0) < 1e-11), (ta, tb, alpha, beta)
+    finally:
+        O.set_avx(-1)
+
+
+def test_gemm_beta_zero_ignores_nan_in_c_and_alpha_zero_skips_products():
+    a = np.array([np.inf, 1.0, 1.0, 1.0])
+    c = np.array([np.nan] * 4)
+    got = O.dgemm(1.0, np.ones(4), np.ones(4), 0.0, c, 2, 2, 2)
+    assert np.array_equal(got, np.full(4, 2.0))
+    got = O.dgemm(0.0, a, np.ones(4), 2.0, np.ones(4), 2, 2, 2)  # inf never multiplied
+    assert np.array_equal(got, np.full(4, 2.0))
+
+
+# ------------------------------------------------------------------------------------------
+# dsyrk, triangular solves
+# ------------------------------------------------------------------------------------------
+
+def test_dsyrk_both_triangles_and_the_other_is_untouched():
+    # src/blas.rs:851-872
+    a = np.array([1.0, 3.0, 2.0, 4.0])
+    lo = O.dsyrk(1.0, a, 0.0, np.zeros(4), 2, 2, False)
+    assert list(lo) == [5.0, 11.0, 0.0, 25.0]
+    up = O.dsyrk(1.0, a, 0.0, np.zeros(4), 2, 2, True)
+    assert list(up) == [5.0, 0.0, 11.0, 25.0]
+
+
+@pytest.mark.parametrize("upper", [False, True])
+def test_dsyrk_and_rank_match_the_definition(upper):
+    n, k = 7, 5
+    rng = np.random.default_rng(3)
+    a = rng.uniform(-1, 1, n * k)
+    c0 = rng.uniform(-1, 1, n * n)
+    A, C0 = mat(a, n, k), mat(c0, n, n)
+    want = 2.5 * A @ A.T - 1.5 * C0
+    got = mat(O.dsyrk(2.5, a, -1.5, c0, n, k, upper), n, n)
+    tri = np.triu(np.ones((n, n), bool)) if upper else np.tril(np.ones((n, n), bool))
+    assert np.allclose(got[tri], want[tri], rtol=1e-13, atol=1e-13)
+    assert np.array_equal(got[~tri], C0[~tri])
+    # dsyrk_rank(transposed): A is k x n, C = alpha A^T A + beta C (src/blas.rs:1224-1251)
+    at = transpose_flat(a, n, k)
+    got2 = mat(O.dsyrk_rank(2.5, at, -1.5, c0, n, k, upper, True), n, n)
+    assert np.allclose(got2[tri], want[tri], rtol=1e-13, atol=1e-13)
+
+
+def test_dtrsv_and_dtrsm_solve_known_systems():
+    # src/blas.rs:2761-2821
+    x = np.array([1.0, -2.0, 0.5])
+    for a, lower in ((lower3(), True), (upper3(), False)):
+        b = mat(a, 3, 3) @ x
+        assert np.allclose(O.dtrsv(a, b, 3, lower, False), x, atol=1e-14)
+    unit = np.array([7.0, 1.0, 4.0, 0.0, 7.0, 5.0, 0.0, 0.0, 7.0])
+    b = np.array([[1, 0, 0], [1, 1, 0], [4, 5, 1.0]]) @ x
+    assert np.allclose(O.dtrsv(unit, b, 3, True, True), x, atol=1e-14)
+    b = mat(lower3(), 3, 3).T @ x
+    assert np.allclose(O.dtrsv_transpose(lower3(), b, 3, True, False), x, atol=1e-14)
+    X = np.array([1.0, 2.0, 3.0, -1.0, 0.5, 2.0])
+    B = cm(mat(lower3(), 3, 3) @ mat(X, 3, 2))
+    assert np.allclose(O.dtrsm(2.0, lower3(), B, 3, 2, True, True, False), 2.0 * X, atol=1e-14)
+    X = np.array([1.0, 4.0, 2.0, 5.0, 3.0, 6.0])
+    B = cm(mat(X, 2, 3) @ mat(lower3(), 3, 3))
+    assert np.allclose(O.dtrsm(2.0, lower3(), B, 2, 3, False, True, False), 2.0 * X, atol=1e-14)
+
+
+@pytest.mark.parametrize("left", [True, False])
+@pytest.mark.parametrize("lower", [True, False])
+@pytest.mark.parametrize("unit", [True, False])
+def test_dtrsm_matches_scipy(left, lower, unit):
+    m, n = 9, 6
+    rng = np.random.default_rng(11)
+    na = m if left else n
+    T = rng.uniform(-1, 1, (na, na)) + 4 * np.eye(na)
+    B = rng.uniform(-1, 1, (m, n))
+    want = sla.blas.dtrsm(1.5, T, B, side=0 if left else 1, lower=int(lower), diag=int(unit))
+    got = mat(O.dtrsm(1.5, cm(T), cm(B), m, n, left, lower, unit), m, n)
+    assert np.allclose(got, want, rtol=1e-13, atol=1e-13)
+
+
+# ------------------------------------------------------------------------------------------
+# LU / solve / Cholesky drivers
+# ------------------------------------------------------------------------------------------
+
+def test_dgetrf_determinant_by_cofactors():
+    # src/lapack/mod.rs:483-495
+    st, lu, ipiv, det = O.dgetrf(sample4(), 4)
+    assert st == 0 and len(ipiv) == 4
+    # the cofactor expansion over the first row of the reference test evaluates to 209
+    expect = (4.0 * (3.0 * (5.0 * 6.0 - 2.0 * 2.0) + 1.0 * (0.0 * 2.0 - 5.0 * 1.0))
+              - 1.0 * (1.0 * (5.0 * 6.0 - 2.0 * 2.0) + 1.0 * (2.0 * 2.0 - 5.0 * 1.0))
+              + 2.0 * (1.0 * (0.0 * 6.0 - 2.0 * 1.0) - 3.0 * (2.0 * 6.0 - 2.0 * 1.0) + 1.0 * (2.0 * 1.0))
+              - 1.0 * (1.0 * (0.0 * 2.0 - 5.0 * 1.0) - 3.0 * (2.0 * 2.0 - 5.0 * 1.0)))
+    assert expect == 209.0 and abs(det - expect) < 1e-12
+
+
+def test_dgetrf_singular_and_threshold():
+    # src/lapack/mod.rs:497-502 (needs unfused arithmetic), :670-683
+    assert O.dgetrf(singular3(), 3)[0] == 1
+    st, _, _, det = O.dgetrf(np.array([1e-16, 0.0, 0.0, 1.0]), 2)
+    assert st == 0 and abs(det) < 1e-15
+    assert O.dgetrf(np.array([1.0, 2.0, 2.0, 4.0]), 2)[0] == 1
+    st, _, ipiv, det = O.dgetrf(np.zeros(0), 0)
+    assert st == 0 and det == 1.0 and len(ipiv) == 0
+
+
+def test_dgetrf_rect_reconstructs_a_tall_panel():
+    # src/lapack/mod.rs:723-733
+    a0 = np.array([1.0, 5.0, 2.0, 3.0, 4.0, 2.0, -1.0, 6.0, 0.0, 1.0, 3.0, 2.0, 1.0, 7.0, -2.0])
+    st, a, ipiv = O.dgetrf_rect(a0, 5, 3)
+    assert st == 0 and ipiv[0] == 1
+    assert O.get01(a0, a, ipiv, 5, 3) < 30
+    assert O.dgetrf_rect(np.array([0.0, 0.0, 2.0, 3.0]), 2, 2)[0] == 1
+    assert O.dgetrf_rect(np.zeros(4), 1, 2)[0] == 2
+
+
+def test_dgesv_recovers_a_known_solution_and_reports_singular():
+    # src/lapack/mod.rs:537-558
+    x = np.array([1.0, -2.0, 3.0, 5.0])
+    b = mat(sample4(), 4, 4) @ x
+    st, _, got = O.dgesv(sample4(), b)
+    assert st == 0 and np.allclose(got, x, atol=1e-12)
+    assert O.dgesv(singular3(), np.ones(3))[0] == 1
+
+
+def test_dpotrf_known_factor_and_not_spd():
+    # src/lapack/mod.rs:560-607
+    st, a = O.dpotrf(np.array([4.0, 2.0, 2.0, 3.0]), 2)
+    assert st == 0 and abs(a[0] - 2) < 1e-14 and abs(a[1] - 1) < 1e-14 and abs(a[3] - math.sqrt(2)) < 1e-14
+    assert a[2] == 2.0, "the upper triangle is untouched"
+    assert O.dpotrf(np.array([1.0, 2.0, 2.0, 1.0]), 2)[0] == 1
+    assert O.dpotrf(np.array([1.0, 1.0, 1.0, 1.0]), 2)[0] == 1
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 7, 9, 33])
+def test_drivers_agree_with_scipy_lapack_on_parity_samples(n):
+    # src/parity.rs:152-283 at TOL = 1e-10 (src/parity.rs:29-35); scipy stands in for LAPACK.
+    a = sample(n)
+    st, lu, ipiv, det = O.dgetrf(a, n)
+    lu_s, piv_s = sla.lu_factor(mat(a, n, n))
+    assert st == 0
+    assert np.array_equal(ipiv, piv_s)
+    assert np.allclose(mat(lu, n, n), lu_s, rtol=1e-10, atol=1e-10)
+    assert abs(det - np.linalg.det(mat(a, n, n))) <= 1e-10 * abs(det)
+    x_true = 1.0 + 0.25 * np.arange(n)
+    b = mat(a, n, n) @ x_true
+    st, _, x = O.dgesv(a, b)
+    assert np.allclose(x, sla.solve(mat(a, n, n), b), rtol=1e-10) and np.allclose(x, x_true, rtol=1e-10)
+    s = spd(n)
+    st, l = O.dpotrf(s, n)
+    L = np.tril(mat(l, n, n))
+    assert st == 0 and np.allclose(L, sla.cholesky(mat(s, n, n), lower=True), rtol=1e-10, atol=1e-12)
+
+
+def test_lin_checkers_on_correct_solves_and_hilbert():
+    # src/lapack/checkers.rs:307-352
+    for n in (1, 2, 5, 8):
+        a0 = spd(n)
+        x = np.ones(n)
+        st, lu, ipiv, _ = O.dgetrf(a0, n)
+        assert O.get01(a0, lu, ipiv, n, n) <= 20
+        st, ainv = O.dgetri(lu, n, ipiv)
+        r3, rcond = O.get03(a0, ainv, n)
+        st, _, xs = O.dgesv(a0, mat(a0, n, n) @ x)
+        assert r3 <= 20 and O.get04(xs, x, n, 1, rcond) <= 20
+    n = 6
+    h = hilbert(n)
+    st, _, xs = O.dgesv(h, mat(h, n, n) @ np.ones(n))
+    st, lu, ipiv, _ = O.dgetrf(h, n)
+    _, ainv = O.dgetri(lu, n, ipiv)
+    _, rcond = O.get03(h, ainv, n)
+    assert rcond < 1e-6 and O.get04(xs, np.ones(n), n, 1, rcond) <= 20
+    assert O.get04(xs, np.ones(n), n, 1, 1.0) > 1.0
+    for n in (1, 2, 5, 9):
+        s = spd(n)
+        st, lu, ipiv, _ = O.dgetrf(s, n)
+        _, ainv = O.dgetri(lu, n, ipiv)
+        r, rc = O.pot03(s, ainv, n)
+        assert rc > 0 and r <= 20
+
+
+def test_checkers_flag_planted_mistakes():
+    """A test that cannot fail pins nothing: each planted error must push its ratio far above 30."""
+    n = 40
+    a = O.larnv(O.SYMMETRIC, SEEDY, n * n)[0]
+    st, lu, ipiv, _ = O.dgetrf(a, n)
+    assert O.get01(a, lu, ipiv, n, n) < 30
+    bad = lu.copy()
+    bad[5 + 2 * n] = -bad[5 + 2 * n]  # a multiplier with the wrong sign
+    assert O.get01(a, bad, ipiv, n, n) > 1e6
+    swapped = ipiv.copy()
+    swapped[[3, 7]] = swapped[[7, 3]]  # pivots applied in the wrong order
+    assert O.get01(a, lu, swapped, n, n) > 1e6
+    x = np.ones(n)
+    b = mat(a, n, n) @ x
+    assert O.backward_error(a, x, b, n) < 1e-15
+    assert O.backward_error(a, x + 1e-9, b, n) > 1e-11
+
+
+def test_backward_error_is_compensated():
+    # b = A x with cancellation: the compensated residual of the exact solution is 0.
+    a = cm([[1e16, -1e16], [1.0, 1.0]])
+    x = np.array([1.0, 1.0])
+    b = np.array([0.0, 2.0])
+    assert O.backward_error(a, x, b, 2) == 0.0

@@ -1,0 +1,70 @@
+"""Loads ``libalgebra_linear_hip.so`` (built in-tree for gfx950) and declares its C ABI.
+
+There is no CPU fallback: if the library is missing, importing the package fails with a
+message that says how to build it, and calls on a machine without a GPU return
+``AL_ERR_NO_DEVICE`` (raised as ``DeviceError``).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libalgebra_linear_hip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "algebra_linear_hip.h")
+
+AL_OK, AL_SINGULAR, AL_BAD_SHAPE, AL_CONVERGENCE = 0, 1, 2, 3
+AL_ERR_DEVICE, AL_ERR_NO_DEVICE, AL_ERR_OOM, AL_ERR_ARG = -1, -2, -3, -4
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_sp = ctypes.POINTER(ctypes.c_size_t)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_d, _sz, _i, _v = ctypes.c_double, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p
+
+SIGNATURES = {
+    "al_last_error": (ctypes.c_char_p, []),
+    "al_version": (ctypes.c_char_p, []),
+    "al_device_count": (_i, []),
+    "al_dgemm": (_i, [_d, _dp, _sz, _dp, _sz, _d, _dp, _sz, _sz, _sz, _sz]),
+    "al_dgemm_tn": (_i, [_d, _dp, _sz, _dp, _sz, _d, _dp, _sz, _sz, _sz, _sz]),
+    "al_dgemm_nt": (_i, [_d, _dp, _sz, _dp, _sz, _d, _dp, _sz, _sz, _sz, _sz]),
+    "al_dgemm_tt": (_i, [_d, _dp, _sz, _dp, _sz, _d, _dp, _sz, _sz, _sz, _sz]),
+    "al_dtrsm": (_i, [_d, _dp, _sz, _dp, _sz, _sz, _sz, _i, _i, _i]),
+    "al_dsyrk": (_i, [_d, _dp, _sz, _d, _dp, _sz, _sz, _sz, _i]),
+    "al_dgetrf": (_i, [_dp, _sz, _sz, _sp, _sz, _dp]),
+    "al_dgetrf_rect": (_i, [_dp, _sz, _sz, _sz, _sp, _sz]),
+    "al_dgetrs": (_i, [_dp, _sz, _sp, _sz, _dp, _sz]),
+    "al_dgesv": (_i, [_dp, _sz, _dp, _sz]),
+    "al_dpotrf": (_i, [_dp, _sz, _sz]),
+    "al_dgesv_nrhs": (_i, [_dp, _sz, _dp, _sz, _sz, _sz, _sp]),
+    "al_dgetrs_nrhs": (_i, [_dp, _sz, _sp, _sz, _dp, _sz, _sz, _sz]),
+    "al_dev_dgemm": (_i, [_i, _i, _sz, _sz, _sz, _d, _v, _sz, _v, _sz, _d, _v, _sz, _v]),
+    "al_dev_dsyrk": (_i, [_i, _sz, _sz, _d, _v, _sz, _d, _v, _sz, _v]),
+    "al_dev_dtrsm": (_i, [_i, _i, _i, _sz, _sz, _d, _v, _sz, _v, _sz, _v]),
+    "al_dev_dgetrf": (_i, [_sz, _sz, _v, _sz, _v, _v, _v]),
+    "al_dev_dgetrs": (_i, [_sz, _sz, _v, _sz, _v, _v, _sz, _v]),
+    "al_dev_dpotrf": (_i, [_sz, _v, _sz, _v, _v]),
+}
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """The loaded library (raises ImportError if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `make -C {HERE}` (hipcc, gfx950) or "
+                "`python -c 'import __graft_entry__ as g; g.build()'`")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def last_error() -> str:
+    return load().al_last_error().decode()

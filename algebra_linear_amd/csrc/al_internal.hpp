@@ -1,0 +1,101 @@
+// Shared internals of the MI355X (gfx950) dense solve library: status plumbing, the
+// device-side building blocks the kernels share, and the host-side launch helpers.
+// Matrices are column-major; element (i, j) of a matrix with leading dimension ld sits
+// at [i + j*ld], as in the algebra_linear crate (src/blas.rs:1-9).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/algebra_linear_hip.h"
+
+namespace al {
+
+// ---------------------------------------------------------------------------------------
+// host-side status handling
+// ---------------------------------------------------------------------------------------
+
+// Records the message a failed call leaves for al_last_error(); thread-local.
+void set_error(const std::string& msg);
+int fail_hip(hipError_t e, const char* what);
+
+#define AL_HIP(expr)                                         \
+    do {                                                     \
+        hipError_t _e = (expr);                              \
+        if (_e != hipSuccess) return ::al::fail_hip(_e, #expr); \
+    } while (0)
+
+#define AL_CHECK(expr)             \
+    do {                           \
+        int _s = (expr);           \
+        if (_s != AL_OK) return _s; \
+    } while (0)
+
+inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// ---------------------------------------------------------------------------------------
+// per-call device workspace: the cooperative panel kernels' exchange slots, counters and
+// the status word every kernel of a factorization checks before doing work.
+// ---------------------------------------------------------------------------------------
+struct Workspace {
+    int device = -1;
+    // status word (int32): 0 = fine, k > 0 = first zero pivot (1-based column), < 0 device
+    // error (a bounded spin timed out).
+    int* info = nullptr;
+    unsigned* counter = nullptr;   // grid-barrier arrival counter
+    unsigned* abort_flag = nullptr;
+    double* slots = nullptr;       // panel exchange area
+    size_t slots_bytes = 0;
+    int* ipiv32 = nullptr;         // int32 pivots on the device
+    size_t ipiv_len = 0;
+};
+
+// The calling thread's workspace on the current device (created on first use).
+int workspace(Workspace** out, size_t ipiv_len);
+
+// Per-thread HIP stream used by the host-slice API (default priority).
+int thread_stream(hipStream_t* out);
+
+// ---------------------------------------------------------------------------------------
+// device kernels' host launchers (all asynchronous on `stream`)
+// ---------------------------------------------------------------------------------------
+
+// C := alpha*op(A)*op(B) + beta*C; tri: 0 full, 1 lower triangle only, 2 upper only.
+int gemm_f64(hipStream_t s, bool ta, bool tb, size_t m, size_t n, size_t k, double alpha, const double* A,
+             size_t lda, const double* B, size_t ldb, double beta, double* C, size_t ldc, int tri = 0);
+
+// Triangular solve op(T) X = alpha B (left) or X op(T) = alpha B (right), in place in B.
+// trans selects op(T) = T^T (used by the Cholesky panel); `info` (nullable) turns the call
+// into a no-op once a factorization has failed.
+int trsm_f64(hipStream_t s, bool left, bool lower, bool trans, bool unit, size_t m, size_t n, double alpha,
+             const double* T, size_t ldt, double* B, size_t ldb, const int* info = nullptr);
+
+// Row interchanges ipiv[k0..k1) (int32, global row indices) on columns [0, ncols) of A.
+int laswp_f64(hipStream_t s, double* A, size_t lda, size_t ncols, const int* ipiv, size_t k0, size_t k1,
+              const int* info = nullptr);
+
+// Blocked LU with partial pivoting of the m x n matrix A (m >= n for the panel form).
+int getrf_f64(hipStream_t s, Workspace* ws, size_t m, size_t n, double* A, size_t lda, int* ipiv, int* info);
+
+// Solve with a getrf factorization: B (n x nrhs) := A^{-1} B.
+int getrs_f64(hipStream_t s, size_t n, size_t nrhs, const double* LU, size_t lda, const int* ipiv, double* B,
+              size_t ldb, const int* info = nullptr);
+
+// Lower Cholesky in place; only the lower triangle is read or written.
+int potrf_f64(hipStream_t s, Workspace* ws, size_t n, double* A, size_t lda, int* info);
+
+// Launch instrumentation (al_prof_*): when enabled, prof_begin records a start event on the
+// stream and returns a ticket; prof_end records the stop event with the launch's algorithmic
+// flops and bytes.  Both are no-ops (ticket -1) when disabled.
+int prof_begin(hipStream_t s);
+void prof_end(int ticket, hipStream_t s, const char* kernel_class, double flops, double bytes);
+
+// Fill helpers (kernels, never hipMemset, so the sequence stays graph-capturable).
+int fill_i32(hipStream_t s, int* p, size_t n, int v);
+int fill_u32(hipStream_t s, unsigned* p, size_t n, unsigned v);
+int iota_i32(hipStream_t s, int* p, size_t n);  // p[i] = i
+
+}  // namespace al

@@ -1,0 +1,470 @@
+// The C ABI of include/algebra_linear_hip.h: shape checks with the crate's own messages,
+// per-thread streams and device buffers, host<->HBM staging, and status mapping.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "al_internal.hpp"
+
+namespace al {
+
+size_t panel_slots_bytes();
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+struct ThreadCtx {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    DevBuf bufs[4];
+    std::unordered_map<hipStream_t, Workspace> ws;  // keyed by the stream the work runs on
+    ~ThreadCtx() {
+        for (auto& b : bufs)
+            if (b.p) (void)hipFree(b.p);
+        for (auto& kv : ws) {
+            Workspace& w = kv.second;
+            if (w.info) (void)hipFree(w.info);
+            if (w.slots) (void)hipFree(w.slots);
+            if (w.ipiv32) (void)hipFree(w.ipiv32);
+        }
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+thread_local ThreadCtx g_ctx;
+
+int ensure_device() {
+    if (g_ctx.device >= 0) return AL_OK;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) {
+        set_error("no HIP device visible (algebra_linear_hip needs an MI355X / gfx950)");
+        return AL_ERR_NO_DEVICE;
+    }
+    int dev = 0;
+    AL_HIP(hipGetDevice(&dev));
+    g_ctx.device = dev;
+    return AL_OK;
+}
+
+int buffer(int slot, size_t bytes, void** out) {
+    DevBuf& b = g_ctx.bufs[slot];
+    if (b.cap < bytes) {
+        if (b.p) AL_HIP(hipFree(b.p));
+        b.p = nullptr;
+        b.cap = 0;
+        size_t want = bytes < 256 ? 256 : bytes;
+        hipError_t e = hipMalloc(&b.p, want);
+        if (e != hipSuccess) {
+            b.p = nullptr;
+            set_error("device allocation failed");
+            return AL_ERR_OOM;
+        }
+        b.cap = want;
+    }
+    *out = b.p;
+    return AL_OK;
+}
+
+bool mul_ok(size_t a, size_t b, size_t* out) {
+    if (a != 0 && b > SIZE_MAX / a) return false;
+    *out = a * b;
+    return true;
+}
+
+bool len_is(size_t len, size_t a, size_t b) {
+    size_t p;
+    return mul_ok(a, b, &p) && p == len;
+}
+
+int bad_shape(const char* msg) {
+    set_error(msg);
+    return AL_BAD_SHAPE;
+}
+
+int upload(int slot, const double* h, size_t n, double** d, hipStream_t s) {
+    void* p;
+    AL_CHECK(buffer(slot, n * sizeof(double), &p));
+    *d = (double*)p;
+    if (n) AL_HIP(hipMemcpyAsync(p, h, n * sizeof(double), hipMemcpyHostToDevice, s));
+    return AL_OK;
+}
+
+int download(double* h, const double* d, size_t n, hipStream_t s) {
+    if (n) AL_HIP(hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    return AL_OK;
+}
+
+int finish(hipStream_t s) {
+    AL_HIP(hipStreamSynchronize(s));
+    return AL_OK;
+}
+
+int status_from_info(int info) {
+    if (info == 0) return AL_OK;
+    if (info > 0) {
+        set_error("matrix is singular (or not positive definite) to working precision");
+        return AL_SINGULAR;
+    }
+    set_error("device-side timeout in a cooperative panel barrier");
+    return AL_ERR_DEVICE;
+}
+
+int gemm_host(const char* name, bool ta, bool tb, double alpha, const double* a, size_t a_len, const double* b,
+              size_t b_len, double beta, double* c, size_t c_len, size_t m, size_t n, size_t k) {
+    // messages of src/blas.rs:202-204, :221-223, :238-240, :253-255
+    std::string nm(name);
+    if (!len_is(a_len, ta ? k : m, ta ? m : k)) return bad_shape((nm + (ta ? ": a must be k*m" : ": a must be m*k")).c_str());
+    if (!len_is(b_len, tb ? n : k, tb ? k : n)) return bad_shape((nm + (tb ? ": b must be n*k" : ": b must be k*n")).c_str());
+    if (!len_is(c_len, m, n)) return bad_shape((nm + ": c must be m*n").c_str());
+    if (m == 0 || n == 0) return AL_OK;
+    if ((!a && a_len) || (!b && b_len) || !c) {
+        set_error("null pointer");
+        return AL_ERR_ARG;
+    }
+    AL_CHECK(ensure_device());
+    hipStream_t s;
+    AL_CHECK(thread_stream(&s));
+    double *dA = nullptr, *dB = nullptr, *dC = nullptr;
+    const bool need_ab = !(k == 0 || alpha == 0.0);
+    if (need_ab) {
+        AL_CHECK(upload(0, a, a_len, &dA, s));
+        AL_CHECK(upload(1, b, b_len, &dB, s));
+    }
+    if (beta != 0.0) {
+        AL_CHECK(upload(2, c, c_len, &dC, s));
+    } else {
+        void* p;
+        AL_CHECK(buffer(2, c_len * sizeof(double), &p));
+        dC = (double*)p;
+    }
+    AL_CHECK(gemm_f64(s, ta, tb, m, n, k, alpha, dA, ta ? k : m, dB, tb ? n : k, beta, dC, m));
+    AL_CHECK(download(c, dC, c_len, s));
+    return finish(s);
+}
+
+int workspace_for(hipStream_t s, Workspace** out, size_t ipiv_len) {
+    Workspace& w = g_ctx.ws[s];
+    if (!w.info) {
+        void* p;
+        // info (int) + counter[2] (unsigned), 256-byte aligned block
+        AL_HIP(hipMalloc(&p, 256));
+        w.info = (int*)p;
+        w.counter = (unsigned*)((char*)p + 64);
+        w.abort_flag = w.counter + 1;
+        w.slots_bytes = panel_slots_bytes();
+        AL_HIP(hipMalloc((void**)&w.slots, w.slots_bytes));
+        w.device = g_ctx.device;
+    }
+    if (w.ipiv_len < ipiv_len) {
+        if (w.ipiv32) AL_HIP(hipFree(w.ipiv32));
+        w.ipiv32 = nullptr;
+        AL_HIP(hipMalloc((void**)&w.ipiv32, (ipiv_len < 64 ? 64 : ipiv_len) * sizeof(int)));
+        w.ipiv_len = ipiv_len < 64 ? 64 : ipiv_len;
+    }
+    *out = &w;
+    return AL_OK;
+}
+
+// Runs getrf on host data already validated; downloads the factors and pivots.
+int getrf_host(double* a, size_t M, size_t N, size_t* ipiv, hipStream_t s, Workspace* ws, double** dA_out) {
+    double* dA;
+    AL_CHECK(upload(0, a, M * N, &dA, s));
+    const size_t kmax = M < N ? M : N;
+    // identity pivots first, so that even a partial factorization never holds a wild index
+    AL_CHECK(iota_i32(s, ws->ipiv32, kmax));
+    AL_CHECK(getrf_f64(s, ws, M, N, dA, M, ws->ipiv32, ws->info));
+    int info = 0;
+    std::vector<int> piv(kmax);
+    AL_CHECK(download(a, dA, M * N, s));
+    if (kmax) AL_HIP(hipMemcpyAsync(piv.data(), ws->ipiv32, kmax * sizeof(int), hipMemcpyDeviceToHost, s));
+    AL_HIP(hipMemcpyAsync(&info, ws->info, sizeof(int), hipMemcpyDeviceToHost, s));
+    AL_CHECK(finish(s));
+    for (size_t i = 0; i < kmax; ++i) ipiv[i] = (size_t)piv[i];
+    if (dA_out) *dA_out = dA;
+    return status_from_info(info);
+}
+
+int check_pivots(const size_t* ipiv, size_t n) {
+    for (size_t k = 0; k < n; ++k)
+        if (ipiv[k] >= n) {
+            set_error("dgetrs: pivot index out of range");
+            return AL_ERR_ARG;
+        }
+    return AL_OK;
+}
+
+}  // namespace
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail_hip(hipError_t e, const char* what) {
+    g_last_error = std::string("HIP error: ") + hipGetErrorString(e) + " in " + what;
+    return AL_ERR_DEVICE;
+}
+
+int thread_stream(hipStream_t* out) {
+    AL_CHECK(ensure_device());
+    if (!g_ctx.stream) AL_HIP(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+    *out = g_ctx.stream;
+    return AL_OK;
+}
+
+int workspace(Workspace** out, size_t ipiv_len) {
+    hipStream_t s;
+    AL_CHECK(thread_stream(&s));
+    return workspace_for(s, out, ipiv_len);
+}
+
+}  // namespace al
+
+using namespace al;
+
+extern "C" {
+
+const char* al_last_error(void) { return g_last_error.c_str(); }
+
+const char* al_version(void) { return "algebra_linear_hip 0.1.0 (gfx950)"; }
+
+int al_device_count(void) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess) return 0;
+    return count;
+}
+
+int al_dgemm(double alpha, const double* a, size_t a_len, const double* b, size_t b_len, double beta, double* c,
+             size_t c_len, size_t m, size_t n, size_t k) {
+    return gemm_host("dgemm", false, false, alpha, a, a_len, b, b_len, beta, c, c_len, m, n, k);
+}
+int al_dgemm_tn(double alpha, const double* a, size_t a_len, const double* b, size_t b_len, double beta, double* c,
+                size_t c_len, size_t m, size_t n, size_t k) {
+    return gemm_host("dgemm_tn", true, false, alpha, a, a_len, b, b_len, beta, c, c_len, m, n, k);
+}
+int al_dgemm_nt(double alpha, const double* a, size_t a_len, const double* b, size_t b_len, double beta, double* c,
+                size_t c_len, size_t m, size_t n, size_t k) {
+    return gemm_host("dgemm_nt", false, true, alpha, a, a_len, b, b_len, beta, c, c_len, m, n, k);
+}
+int al_dgemm_tt(double alpha, const double* a, size_t a_len, const double* b, size_t b_len, double beta, double* c,
+                size_t c_len, size_t m, size_t n, size_t k) {
+    return gemm_host("dgemm_tt", true, true, alpha, a, a_len, b, b_len, beta, c, c_len, m, n, k);
+}
+
+int al_dtrsm(double alpha, const double* a, size_t a_len, double* b, size_t b_len, size_t m, size_t n, int left,
+             int lower, int unit) {
+    // src/blas.rs:1385-1397
+    if (!len_is(b_len, m, n)) return bad_shape("dtrsm: b must be m*n");
+    if (left && !len_is(a_len, m, m)) return bad_shape("dtrsm: a must be m*m for the left side");
+    if (!left && !len_is(a_len, n, n)) return bad_shape("dtrsm: a must be n*n for the right side");
+    if (m == 0 || n == 0) return AL_OK;
+    hipStream_t s;
+    AL_CHECK(thread_stream(&s));
+    double *dA, *dB;
+    AL_CHECK(upload(0, a, a_len, &dA, s));
+    AL_CHECK(upload(1, b, b_len, &dB, s));
+    AL_CHECK(trsm_f64(s, left != 0, lower != 0, false, unit != 0, m, n, alpha, dA, left ? m : n, dB, m));
+    AL_CHECK(download(b, dB, b_len, s));
+    return finish(s);
+}
+
+int al_dsyrk(double alpha, const double* a, size_t a_len, double beta, double* c, size_t c_len, size_t n, size_t k,
+             int upper) {
+    // src/blas.rs:610-611
+    if (!len_is(a_len, n, k)) return bad_shape("dsyrk: a must be n*k");
+    if (!len_is(c_len, n, n)) return bad_shape("dsyrk: c must be n*n");
+    if (n == 0) return AL_OK;
+    hipStream_t s;
+    AL_CHECK(thread_stream(&s));
+    double *dA = nullptr, *dC;
+    if (k) AL_CHECK(upload(0, a, a_len, &dA, s));
+    AL_CHECK(upload(2, c, c_len, &dC, s));  // the other triangle must come back unchanged
+    AL_CHECK(gemm_f64(s, false, true, n, n, k, alpha, dA, n, dA, n, beta, dC, n, upper ? 2 : 1));
+    AL_CHECK(download(c, dC, c_len, s));
+    return finish(s);
+}
+
+int al_dgetrf(double* a, size_t a_len, size_t n, size_t* ipiv, size_t ipiv_len, double* det) {
+    // src/lapack/mod.rs:112-118
+    if (!len_is(a_len, n, n) || ipiv_len != n) return bad_shape("dgetrf: a must be n*n and ipiv n");
+    if (n == 0) {
+        if (det) *det = 1.0;
+        return AL_OK;
+    }
+    hipStream_t s;
+    AL_CHECK(thread_stream(&s));
+    Workspace* ws;
+    AL_CHECK(workspace_for(s, &ws, n));
+    int st = getrf_host(a, n, n, ipiv, s, ws, nullptr);
+    if (st != AL_OK) return st;
+    if (det) {
+        // the crate's determinant: sign flip per interchange, then the running product, in
+        // pivot order (src/lapack/mod.rs:141-150)
+        double d = 1.0;
+        for (size_t k = 0; k < n; ++k) {
+            if (ipiv[k] != k) d = -d;
+            d *= a[k + k * n];
+        }
+        *det = d;
+    }
+    return AL_OK;
+}
+
+int al_dgetrf_rect(double* a, size_t a_len, size_t nrows, size_t ncols, size_t* ipiv, size_t ipiv_len) {
+    // src/lapack/mod.rs:203-205
+    if (nrows < ncols || !len_is(a_len, nrows, ncols) || ipiv_len != ncols)
+        return bad_shape("dgetrf_rect: need nrows >= ncols, a nrows*ncols and ipiv ncols");
+    if (ncols == 0) return AL_OK;
+    hipStream_t s;
+    AL_CHECK(thread_stream(&s));
+    Workspace* ws;
+    AL_CHECK(workspace_for(s, &ws, ncols));
+    return getrf_host(a, nrows, ncols, ipiv, s, ws, nullptr);
+}
+
+int al_dgetrs_nrhs(const double* lu, size_t lu_len, const size_t* ipiv, size_t ipiv_len, double* b, size_t b_len,
+                   size_t n, size_t nrhs) {
+    if (!len_is(lu_len, n, n) || ipiv_len != n || !len_is(b_len, n, nrhs))
+        return bad_shape("dgetrs: lu must be n*n, ipiv n and b n*nrhs");
+    if (n == 0 || nrhs == 0) return AL_OK;
+    AL_CHECK(check_pivots(ipiv, n));
+    hipStream_t s;
+    AL_CHECK(thread_stream(&s));
+    Workspace* ws;
+    AL_CHECK(workspace_for(s, &ws, n));
+    std::vector<int> piv(n);
+    for (size_t i = 0; i < n; ++i) piv[i] = (int)ipiv[i];
+    double *dLU, *dB;
+    AL_CHECK(upload(0, lu, lu_len, &dLU, s));
+    AL_CHECK(upload(1, b, b_len, &dB, s));
+    AL_HIP(hipMemcpyAsync(ws->ipiv32, piv.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
+    AL_CHECK(getrs_f64(s, n, nrhs, dLU, n, ws->ipiv32, dB, n));
+    AL_CHECK(download(b, dB, b_len, s));
+    return finish(s);
+}
+
+int al_dgetrs(const double* lu, size_t lu_len, const size_t* ipiv, size_t ipiv_len, double* b, size_t b_len) {
+    return al_dgetrs_nrhs(lu, lu_len, ipiv, ipiv_len, b, b_len, b_len, 1);
+}
+
+int al_dgesv_nrhs(double* a, size_t a_len, double* b, size_t b_len, size_t n, size_t nrhs, size_t* ipiv) {
+    if (!len_is(a_len, n, n) || !len_is(b_len, n, nrhs)) return bad_shape("dgesv: a must be n*n and b n*nrhs");
+    if (n == 0) return AL_OK;
+    hipStream_t s;
+    AL_CHECK(thread_stream(&s));
+    Workspace* ws;
+    AL_CHECK(workspace_for(s, &ws, n));
+    std::vector<size_t> piv(n);
+    double* dA;
+    int st = getrf_host(a, n, n, piv.data(), s, ws, &dA);
+    if (ipiv) memcpy(ipiv, piv.data(), n * sizeof(size_t));
+    if (st != AL_OK) return st;
+    if (nrhs == 0) return AL_OK;
+    double* dB;
+    AL_CHECK(upload(1, b, b_len, &dB, s));
+    AL_CHECK(getrs_f64(s, n, nrhs, dA, n, ws->ipiv32, dB, n));
+    AL_CHECK(download(b, dB, b_len, s));
+    return finish(s);
+}
+
+int al_dgesv(double* a, size_t a_len, double* b, size_t b_len) {
+    // src/lapack/mod.rs:309-323: n = b.len()
+    const size_t n = b_len;
+    if (!len_is(a_len, n, n)) return bad_shape("dgesv: a must be n*n with n = b.len()");
+    return al_dgesv_nrhs(a, a_len, b, b_len, n, 1, nullptr);
+}
+
+int al_dpotrf(double* a, size_t a_len, size_t n) {
+    if (!len_is(a_len, n, n)) return bad_shape("dpotrf: a must be n*n");
+    if (n == 0) return AL_OK;
+    hipStream_t s;
+    AL_CHECK(thread_stream(&s));
+    Workspace* ws;
+    AL_CHECK(workspace_for(s, &ws, 1));
+    double* dA;
+    AL_CHECK(upload(0, a, a_len, &dA, s));
+    AL_CHECK(potrf_f64(s, ws, n, dA, n, ws->info));
+    int info = 0;
+    AL_CHECK(download(a, dA, a_len, s));
+    AL_HIP(hipMemcpyAsync(&info, ws->info, sizeof(int), hipMemcpyDeviceToHost, s));
+    AL_CHECK(finish(s));
+    return status_from_info(info);
+}
+
+// ------------------------------------------------------------------------------------------
+// device-pointer API
+// ------------------------------------------------------------------------------------------
+
+static int dev_stream(void* stream, hipStream_t* s) {
+    if (stream) {
+        AL_CHECK(ensure_device());
+        *s = (hipStream_t)stream;
+        return AL_OK;
+    }
+    return thread_stream(s);
+}
+
+int al_dev_dgemm(int trans_a, int trans_b, size_t m, size_t n, size_t k, double alpha, const double* A, size_t lda,
+                 const double* B, size_t ldb, double beta, double* C, size_t ldc, void* stream) {
+    if (ldc < m || lda < (trans_a ? k : m) || ldb < (trans_b ? n : k)) return bad_shape("dgemm: leading dimension too small");
+    hipStream_t s;
+    AL_CHECK(dev_stream(stream, &s));
+    return gemm_f64(s, trans_a != 0, trans_b != 0, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+int al_dev_dsyrk(int upper, size_t n, size_t k, double alpha, const double* A, size_t lda, double beta, double* C,
+                 size_t ldc, void* stream) {
+    if (ldc < n || lda < n) return bad_shape("dsyrk: leading dimension too small");
+    hipStream_t s;
+    AL_CHECK(dev_stream(stream, &s));
+    return gemm_f64(s, false, true, n, n, k, alpha, A, lda, A, lda, beta, C, ldc, upper ? 2 : 1);
+}
+
+int al_dev_dtrsm(int left, int lower, int unit, size_t m, size_t n, double alpha, const double* A, size_t lda,
+                 double* B, size_t ldb, void* stream) {
+    if (ldb < m || lda < (left ? m : n)) return bad_shape("dtrsm: leading dimension too small");
+    hipStream_t s;
+    AL_CHECK(dev_stream(stream, &s));
+    return trsm_f64(s, left != 0, lower != 0, false, unit != 0, m, n, alpha, A, lda, B, ldb);
+}
+
+int al_dev_dgetrf(size_t m, size_t n, double* A, size_t lda, int32_t* d_ipiv, int32_t* d_info, void* stream) {
+    if (lda < m || m < n) return bad_shape("dgetrf: need lda >= m >= n");
+    hipStream_t s;
+    AL_CHECK(dev_stream(stream, &s));
+    Workspace* ws;
+    AL_CHECK(workspace_for(s, &ws, 1));
+    const size_t kmax = m < n ? m : n;
+    if (kmax == 0) return fill_i32(s, d_info, 1, 0);
+    // identity pivots first (a kernel, not a memset, so the sequence can be graph-captured)
+    AL_CHECK(iota_i32(s, d_ipiv, kmax));
+    return getrf_f64(s, ws, m, n, A, lda, d_ipiv, d_info);
+}
+
+int al_dev_dgetrs(size_t n, size_t nrhs, const double* LU, size_t lda, const int32_t* d_ipiv, double* B, size_t ldb,
+                  void* stream) {
+    if (lda < n || ldb < n) return bad_shape("dgetrs: leading dimension too small");
+    hipStream_t s;
+    AL_CHECK(dev_stream(stream, &s));
+    return getrs_f64(s, n, nrhs, LU, lda, d_ipiv, B, ldb);
+}
+
+int al_dev_dpotrf(size_t n, double* A, size_t lda, int32_t* d_info, void* stream) {
+    if (lda < n) return bad_shape("dpotrf: leading dimension too small");
+    hipStream_t s;
+    AL_CHECK(dev_stream(stream, &s));
+    Workspace* ws;
+    AL_CHECK(workspace_for(s, &ws, 1));
+    return potrf_f64(s, ws, n, A, lda, d_info);
+}
+
+}  // extern "C"

@@ -1,0 +1,251 @@
+// FP64 GEMM on CDNA4 matrix cores: C := alpha*op(A)*op(B) + beta*C, column-major, with
+// the four transposition variants of blas::dgemm{,_tn,_nt,_tt} (src/blas.rs:201-257) and a
+// triangle-restricted form for dsyrk (src/blas.rs:609-626).
+//
+// Design (MI355X-first, not a translation of the crate's Goto blocking):
+//   * one 256-thread workgroup (4 wave64s, 2x2) computes a 128x128 tile of C; each wave
+//     owns 64x64 = 4x4 tiles of v_mfma_f64_16x16x4_f64;
+//   * K advances in steps of 16 through double-buffered LDS; the next step's global loads
+//     are issued into registers before the current step's MFMAs, so HBM/L2 latency hides
+//     behind matrix-core time;
+//   * transposition is handled entirely in the global->LDS load pattern: both orientations
+//     land in the same k-major LDS layout, so the MFMA loop is shared by all variants;
+//   * the MFMA operands are fed as (B^T, A^T) so each lane's four accumulators are four
+//     consecutive *columns* of one C row group and 16 lanes cover 16 consecutive rows:
+//     stores and beta loads of the column-major C are contiguous 128-byte runs;
+//   * workgroup ids are remapped so each of the 8 XCDs walks a contiguous range of tiles,
+//     grouped 8 tile-rows high, keeping A row-panels and B column-panels in that XCD's L2.
+#include "al_internal.hpp"
+
+namespace al {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 128, BN = 128, BK = 16;
+constexpr int NT = 256;
+constexpr int LDS_ROW = BM + 1;  // one-double pad breaks the strided stores' bank conflicts
+constexpr int TILE_DOUBLES = BK * LDS_ROW;
+constexpr size_t LDS_BYTES = 4 * TILE_DOUBLES * sizeof(double);  // A and B, double-buffered
+
+// Loads the 128 x 16 operand tile whose element (r, p) is base[r + p*ld] (ROWCONTIG false:
+// the tile's columns are contiguous, e.g. a non-transposed A) or base[p + r*ld] (true: the
+// tile's rows are contiguous, e.g. a transposed A) into 8 registers per thread, zero-filling
+// outside [0, r_lim) x [0, kdim).
+template <bool ROWCONTIG>
+__device__ __forceinline__ void load_tile(const double* __restrict__ base, size_t ld, int r0, int r_lim, int k0,
+                                          int kdim, double (&reg)[8], int t) {
+    if (!ROWCONTIG) {
+        const int r = t & 127, p0 = t >> 7;
+        const bool rok = (r0 + r) < r_lim;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int p = k0 + p0 + 2 * j;
+            reg[j] = (rok && p < kdim) ? base[(size_t)(r0 + r) + (size_t)p * ld] : 0.0;
+        }
+    } else {
+        const int p = t & 15, rr = t >> 4;
+        const bool pok = (k0 + p) < kdim;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int r = r0 + rr + 16 * j;
+            reg[j] = (pok && r < r_lim) ? base[(size_t)(k0 + p) + (size_t)r * ld] : 0.0;
+        }
+    }
+}
+
+template <bool ROWCONTIG>
+__device__ __forceinline__ void store_tile(double* __restrict__ S, const double (&reg)[8], int t) {
+    if (!ROWCONTIG) {
+        const int r = t & 127, p0 = t >> 7;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) S[(p0 + 2 * j) * LDS_ROW + r] = reg[j];
+    } else {
+        const int p = t & 15, rr = t >> 4;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) S[p * LDS_ROW + rr + 16 * j] = reg[j];
+    }
+}
+
+__device__ __forceinline__ void tile_coords(int tiles_m, int tiles_n, int& tm, int& tn) {
+    const int nb = tiles_m * tiles_n;
+    const int bid = blockIdx.x;
+    // blocks are dispatched round-robin over the 8 XCDs; give XCD x a contiguous range
+    const int x = bid & 7, slot = bid >> 3;
+    const int base = nb >> 3, rem = nb & 7;
+    const int lin = x * base + (x < rem ? x : rem) + slot;
+    // grouped ordering: 8 tile-rows per group, column-major inside the group
+    constexpr int G = 8;
+    const int per_group = G * tiles_n;
+    const int g = lin / per_group;
+    const int first_m = g * G;
+    const int gsize = (tiles_m - first_m) < G ? (tiles_m - first_m) : G;
+    const int in = lin - g * per_group;
+    tm = first_m + in % gsize;
+    tn = in / gsize;
+}
+
+template <bool TA, bool TB, int TRI>
+__global__ __launch_bounds__(NT) void dgemm_mfma(int m, int n, int k, double alpha, const double* __restrict__ A,
+                                                 size_t lda, const double* __restrict__ B, size_t ldb, double beta,
+                                                 double* __restrict__ C, size_t ldc, int tiles_m, int tiles_n) {
+    extern __shared__ double lds[];
+    double* As0 = lds;
+    double* As1 = lds + TILE_DOUBLES;
+    double* Bs0 = lds + 2 * TILE_DOUBLES;
+    double* Bs1 = lds + 3 * TILE_DOUBLES;
+
+    int tm, tn;
+    tile_coords(tiles_m, tiles_n, tm, tn);
+    const int bm = tm * BM, bn = tn * BN;
+    if (TRI == 1 && bm + BM - 1 < bn) return;  // tile strictly above the diagonal
+    if (TRI == 2 && bm > bn + BN - 1) return;  // tile strictly below the diagonal
+
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+
+    // op(A) is m x k: non-transposed A has contiguous columns; transposed A contiguous rows.
+    // op(B) is k x n; its tile element (r = j, p) is B[p + j*ldb] (contiguous in p) when not
+    // transposed and B[j + p*ldb] (contiguous in j) when transposed.
+    constexpr bool A_RC = TA;
+    constexpr bool B_RC = !TB;
+
+    d4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+
+    double ra[8], rb[8];
+    const int ktiles = (k + BK - 1) / BK;
+    load_tile<A_RC>(A, lda, bm, m, 0, k, ra, t);
+    load_tile<B_RC>(B, ldb, bn, n, 0, k, rb, t);
+    store_tile<A_RC>(As0, ra, t);
+    store_tile<B_RC>(Bs0, rb, t);
+    __syncthreads();
+
+    const int col = lane & 15, kq = lane >> 4;
+    for (int kt = 0; kt < ktiles; ++kt) {
+        const bool more = (kt + 1) < ktiles;
+        if (more) {
+            load_tile<A_RC>(A, lda, bm, m, (kt + 1) * BK, k, ra, t);
+            load_tile<B_RC>(B, ldb, bn, n, (kt + 1) * BK, k, rb, t);
+        }
+        const double* As = (kt & 1) ? As1 : As0;
+        const double* Bs = (kt & 1) ? Bs1 : Bs0;
+#pragma unroll
+        for (int s = 0; s < BK / 4; ++s) {
+            const int kk = 4 * s + kq;
+            double af[4], bf[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) af[i] = As[kk * LDS_ROW + wm * 64 + i * 16 + col];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bf[j] = Bs[kk * LDS_ROW + wn * 64 + j * 16 + col];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
+        }
+        if (more) {
+            store_tile<A_RC>((kt & 1) ? As0 : As1, ra, t);
+            store_tile<B_RC>((kt & 1) ? Bs0 : Bs1, rb, t);
+        }
+        __syncthreads();
+    }
+
+    // lane holds C[row][col0 + r], row = .. + (lane & 15), col0 = .. + 4*(lane >> 4)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int row = bm + wm * 64 + i * 16 + col;
+        if (row >= m) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int cc = bn + wn * 64 + j * 16 + 4 * kq + r;
+                if (cc >= n) continue;
+                if (TRI == 1 && row < cc) continue;
+                if (TRI == 2 && row > cc) continue;
+                double* cp = C + (size_t)row + (size_t)cc * ldc;
+                double v = alpha * acc[i][j][r];
+                if (beta != 0.0) v += beta * *cp;
+                *cp = v;
+            }
+        }
+    }
+}
+
+// C := beta*C (0 when beta == 0) -- the crate's k == 0 / alpha == 0 path (src/blas.rs:297-308).
+__global__ void scale_c(int m, int n, double beta, double* C, size_t ldc, int tri) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)m * n) return;
+    const int i = (int)(idx % m), j = (int)(idx / m);
+    if (tri == 1 && i < j) return;
+    if (tri == 2 && i > j) return;
+    double* cp = C + (size_t)i + (size_t)j * ldc;
+    *cp = (beta == 0.0) ? 0.0 : beta * *cp;
+}
+
+template <bool TA, bool TB, int TRI>
+int launch(hipStream_t s, int m, int n, int k, double alpha, const double* A, size_t lda, const double* B,
+           size_t ldb, double beta, double* C, size_t ldc) {
+    static bool attr_set = false;  // benign race: every thread sets the same value
+    auto fn = dgemm_mfma<TA, TB, TRI>;
+    if (!attr_set) {
+        AL_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_BYTES));
+        attr_set = true;
+    }
+    const int tiles_m = (m + BM - 1) / BM, tiles_n = (n + BN - 1) / BN;
+    const int ticket = prof_begin(s);
+    hipLaunchKernelGGL(fn, dim3(tiles_m * tiles_n), dim3(NT), LDS_BYTES, s, m, n, k, alpha, A, lda, B, ldb, beta,
+                       C, ldc, tiles_m, tiles_n);
+    AL_HIP(hipGetLastError());
+    if (ticket >= 0) {
+        // algorithmic work: 2mnk flops (half, plus the diagonal, for a triangle); bytes: A and B
+        // once, C written (and read when beta != 0)
+        const double dm = m, dn = n, dk = k;
+        const double cfrac = TRI ? (dn + 1.0) / (2.0 * dn) : 1.0;
+        const double fl = 2.0 * dm * dn * dk * cfrac;
+        const double by = 8.0 * (dm * dk + dk * dn + dm * dn * cfrac * (beta != 0.0 ? 2.0 : 1.0));
+        static const char* names[3][4] = {{"dgemm_nn", "dgemm_tn", "dgemm_nt", "dgemm_tt"},
+                                          {"dsyrk_lower", "dsyrk_lower", "dsyrk_lower", "dsyrk_lower"},
+                                          {"dsyrk_upper", "dsyrk_upper", "dsyrk_upper", "dsyrk_upper"}};
+        prof_end(ticket, s, names[TRI][(TA ? 1 : 0) + (TB ? 2 : 0)], fl, by);
+    }
+    return AL_OK;
+}
+
+template <int TRI>
+int dispatch(hipStream_t s, bool ta, bool tb, int m, int n, int k, double alpha, const double* A, size_t lda,
+             const double* B, size_t ldb, double beta, double* C, size_t ldc) {
+    if (!ta && !tb) return launch<false, false, TRI>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    if (ta && !tb) return launch<true, false, TRI>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    if (!ta && tb) return launch<false, true, TRI>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    return launch<true, true, TRI>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+}  // namespace
+
+int gemm_f64(hipStream_t s, bool ta, bool tb, size_t m, size_t n, size_t k, double alpha, const double* A,
+             size_t lda, const double* B, size_t ldb, double beta, double* C, size_t ldc, int tri) {
+    if (m == 0 || n == 0) return AL_OK;
+    if (m > INT32_MAX || n > INT32_MAX || k > INT32_MAX) {
+        set_error("gemm: dimensions beyond 2^31 are not supported");
+        return AL_ERR_ARG;
+    }
+    if (k == 0 || alpha == 0.0) {
+        if (beta == 1.0) return AL_OK;
+        const size_t total = m * n;
+        hipLaunchKernelGGL(scale_c, dim3(ceil_div(total, 256)), dim3(256), 0, s, (int)m, (int)n, beta, C, ldc, tri);
+        AL_HIP(hipGetLastError());
+        return AL_OK;
+    }
+    switch (tri) {
+        case 1: return dispatch<1>(s, ta, tb, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc);
+        case 2: return dispatch<2>(s, ta, tb, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc);
+        default: return dispatch<0>(s, ta, tb, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc);
+    }
+}
+
+}  // namespace al

@@ -1,0 +1,93 @@
+// Lower Cholesky, lapack::dpotrf (src/lapack/mod.rs:368-399).
+//
+// The crate runs a left-looking dot-product loop over the whole matrix.  Here it is a
+// right-looking blocked factorization: a 64x64 diagonal block is factored in LDS by one
+// workgroup (`potf2_lds`, the crate's own dot-product order per element, unfused, so a matrix
+// of order <= 64 factors bit-for-bit as the crate factors it), the panel below it is a
+// triangular solve against the block's transpose, and the trailing matrix gets a
+// lower-triangle-only FP64 MFMA rank-k update.  Two levels (64 inside 256) keep the trailing
+// update at depth 256.  Only the lower triangle is read or written anywhere.
+#include "al_internal.hpp"
+
+namespace al {
+namespace {
+
+constexpr int PB = 64;
+constexpr int PBO = 256;
+
+#pragma clang fp contract(off)
+__global__ __launch_bounds__(256) void potf2_lds(double* A, size_t lda, int nb, int col0, int* info) {
+    if (*info != 0) return;
+    __shared__ double L[PB][PB + 1];
+    __shared__ int bad;
+    const int t = threadIdx.x;
+    for (int idx = t; idx < nb * nb; idx += 256) {
+        const int i = idx % nb, j = idx / nb;
+        if (i >= j) L[i][j] = A[(size_t)i + (size_t)j * lda];
+    }
+    if (t == 0) bad = 0;
+    __syncthreads();
+    for (int j = 0; j < nb; ++j) {
+        if (t == 0) {
+            double sum = L[j][j];
+            for (int k = 0; k < j; ++k) {
+                const double ljk = L[j][k];
+                sum -= ljk * ljk;
+            }
+            if (sum <= 0.0) {
+                bad = 1;
+                *info = col0 + j + 1;
+            } else {
+                L[j][j] = sqrt(sum);
+            }
+        }
+        __syncthreads();
+        if (bad) break;
+        const double ljj = L[j][j];
+        for (int i = j + 1 + t; i < nb; i += 256) {
+            double s = L[i][j];
+            for (int k = 0; k < j; ++k) s -= L[i][k] * L[j][k];
+            L[i][j] = s / ljj;
+        }
+        __syncthreads();
+    }
+    // write back the lower triangle (partially factored on failure, like the crate)
+    for (int idx = t; idx < nb * nb; idx += 256) {
+        const int i = idx % nb, j = idx / nb;
+        if (i >= j) A[(size_t)i + (size_t)j * lda] = L[i][j];
+    }
+}
+#pragma clang fp contract(on)
+
+int potrf_level(hipStream_t s, size_t n, double* A, size_t lda, int* info, size_t col0, size_t nb) {
+    for (size_t k = 0; k < n; k += nb) {
+        const size_t jb = (n - k) < nb ? (n - k) : nb;
+        double* Akk = A + k + k * lda;
+        if (jb <= (size_t)PB) {
+            hipLaunchKernelGGL(potf2_lds, dim3(1), dim3(256), 0, s, Akk, lda, (int)jb, (int)(col0 + k), info);
+            AL_HIP(hipGetLastError());
+        } else {
+            AL_CHECK(potrf_level(s, jb, Akk, lda, info, col0 + k, PB));
+        }
+        if (k + jb < n) {
+            double* A21 = Akk + jb;
+            const size_t mr = n - k - jb;
+            // A21 := A21 * L11^{-T}
+            AL_CHECK(trsm_f64(s, false, true, true, false, mr, jb, 1.0, Akk, lda, A21, lda, info));
+            // A22 := A22 - A21 * A21^T, lower triangle only
+            AL_CHECK(gemm_f64(s, false, true, mr, mr, jb, -1.0, A21, lda, A21, lda, 1.0, A21 + jb * lda, lda, 1));
+        }
+    }
+    return AL_OK;
+}
+
+}  // namespace
+
+int potrf_f64(hipStream_t s, Workspace* ws, size_t n, double* A, size_t lda, int* info) {
+    (void)ws;
+    AL_CHECK(fill_i32(s, info, 1, 0));
+    if (n == 0) return AL_OK;
+    return potrf_level(s, n, A, lda, info, 0, n > (size_t)PB ? (size_t)PBO : (size_t)PB);
+}
+
+}  // namespace al

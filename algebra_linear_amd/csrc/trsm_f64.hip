@@ -1,0 +1,211 @@
+// Triangular solves and row interchanges.
+//
+// blas::dtrsm (src/blas.rs:1384-1438) solves one vector at a time with a scalar
+// substitution.  Here a triangular solve is blocked in 64-wide diagonal blocks: each
+// diagonal block is solved by `trsm_diag` (the 64x64 triangle staged in LDS, one thread per
+// right-hand-side vector, the vector held in registers), and the off-diagonal coupling is
+// an FP64 MFMA GEMM, so almost all flops of a large solve run on the matrix cores.
+// Row interchanges (`laswp`) apply a block of pivots to a column range, one thread per
+// column, in pivot order -- the crate's full-row swap of src/lapack/mod.rs:141-146.
+#include "al_internal.hpp"
+
+namespace al {
+namespace {
+
+constexpr int DB = 64;   // diagonal block order
+constexpr int DT = 256;  // threads (= vectors) per workgroup
+
+// Solves M x = x in place for nvec vectors, M the nb x nb (nb <= 64) triangle whose element
+// (i, k) is T[i*t_rs + k*t_cs]; vector v's element i is X[i*x_es + v*x_vs].  `forward`
+// selects the lower sweep.  Only the triangle (and the diagonal unless `unit`) is read.
+__global__ __launch_bounds__(DT) void trsm_diag(int nb, int nvec, const double* __restrict__ T, size_t t_rs,
+                                                size_t t_cs, double* __restrict__ X, size_t x_es, size_t x_vs,
+                                                int forward, int unit, const int* __restrict__ info) {
+    if (info && *info != 0) return;
+    __shared__ double Ts[DB][DB + 1];
+    const int t = threadIdx.x;
+    // stage the triangle; pad to 64 with the identity so the solve below is branch-free
+    for (int idx = t; idx < DB * DB; idx += DT) {
+        const int i = idx % DB, k = idx / DB;
+        double v = 0.0;
+        const bool in_tri = forward ? (k < i) : (k > i);
+        if (i < nb && k < nb) {
+            if (in_tri) v = T[(size_t)i * t_rs + (size_t)k * t_cs];
+            else if (i == k) v = unit ? 1.0 : T[(size_t)i * t_rs + (size_t)i * t_cs];
+        } else if (i == k) {
+            v = 1.0;
+        }
+        Ts[i][k] = v;
+    }
+    __syncthreads();
+    const int vec = blockIdx.x * DT + t;
+    if (vec >= nvec) return;
+    double x[DB];
+    double* xp = X + (size_t)vec * x_vs;
+#pragma unroll
+    for (int i = 0; i < DB; ++i) x[i] = (i < nb) ? xp[(size_t)i * x_es] : 0.0;
+    if (forward) {
+#pragma unroll
+        for (int i = 0; i < DB; ++i) {
+            double s = x[i];
+#pragma unroll
+            for (int k = 0; k < i; ++k) s -= Ts[i][k] * x[k];
+            x[i] = unit ? s : s / Ts[i][i];
+        }
+    } else {
+#pragma unroll
+        for (int i = DB - 1; i >= 0; --i) {
+            double s = x[i];
+#pragma unroll
+            for (int k = i + 1; k < DB; ++k) s -= Ts[i][k] * x[k];
+            x[i] = unit ? s : s / Ts[i][i];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < DB; ++i)
+        if (i < nb) xp[(size_t)i * x_es] = x[i];
+}
+
+__global__ void scale_block(int m, int n, double alpha, double* B, size_t ldb, const int* info) {
+    if (info && *info != 0) return;
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)m * n) return;
+    const int i = (int)(idx % m), j = (int)(idx / m);
+    B[(size_t)i + (size_t)j * ldb] *= alpha;
+}
+
+__global__ void laswp_cols(double* A, size_t lda, int ncols, const int* __restrict__ ipiv, int k0, int k1,
+                           const int* info) {
+    if (info && *info != 0) return;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncols) return;
+    double* col = A + (size_t)c * lda;
+    for (int s = k0; s < k1; ++s) {
+        const int p = ipiv[s];
+        if (p != s) {
+            const double a = col[s];
+            col[s] = col[p];
+            col[p] = a;
+        }
+    }
+}
+
+__global__ void fill_i32_k(int* p, size_t n, int v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void iota_i32_k(int* p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (int)i;
+}
+
+__global__ void fill_u32_k(unsigned* p, size_t n, unsigned v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Element (r, c) of op(T) for a sub-block of op(T) at rows r0.., cols c0..: pointer and
+// transposition flag for the GEMM that consumes it.
+inline const double* opT_block(const double* T, size_t ldt, bool trans, size_t r0, size_t c0) {
+    return trans ? T + c0 + r0 * ldt : T + r0 + c0 * ldt;
+}
+
+int diag_solve(hipStream_t s, int nb, int nvec, const double* T, size_t t_rs, size_t t_cs, double* X, size_t x_es,
+               size_t x_vs, bool forward, bool unit, const int* info) {
+    if (nvec <= 0 || nb <= 0) return AL_OK;
+    hipLaunchKernelGGL(trsm_diag, dim3(ceil_div((size_t)nvec, DT)), dim3(DT), 0, s, nb, nvec, T, t_rs, t_cs, X, x_es,
+                       x_vs, forward ? 1 : 0, unit ? 1 : 0, info);
+    AL_HIP(hipGetLastError());
+    return AL_OK;
+}
+
+}  // namespace
+
+int fill_i32(hipStream_t s, int* p, size_t n, int v) {
+    if (n == 0) return AL_OK;
+    hipLaunchKernelGGL(fill_i32_k, dim3(ceil_div(n, 256)), dim3(256), 0, s, p, n, v);
+    AL_HIP(hipGetLastError());
+    return AL_OK;
+}
+
+int iota_i32(hipStream_t s, int* p, size_t n) {
+    if (n == 0) return AL_OK;
+    hipLaunchKernelGGL(iota_i32_k, dim3(ceil_div(n, 256)), dim3(256), 0, s, p, n);
+    AL_HIP(hipGetLastError());
+    return AL_OK;
+}
+
+int fill_u32(hipStream_t s, unsigned* p, size_t n, unsigned v) {
+    if (n == 0) return AL_OK;
+    hipLaunchKernelGGL(fill_u32_k, dim3(ceil_div(n, 256)), dim3(256), 0, s, p, n, v);
+    AL_HIP(hipGetLastError());
+    return AL_OK;
+}
+
+int laswp_f64(hipStream_t s, double* A, size_t lda, size_t ncols, const int* ipiv, size_t k0, size_t k1,
+              const int* info) {
+    if (ncols == 0 || k1 <= k0) return AL_OK;
+    hipLaunchKernelGGL(laswp_cols, dim3(ceil_div(ncols, 256)), dim3(256), 0, s, A, lda, (int)ncols, ipiv, (int)k0,
+                       (int)k1, info);
+    AL_HIP(hipGetLastError());
+    return AL_OK;
+}
+
+int trsm_f64(hipStream_t s, bool left, bool lower, bool trans, bool unit, size_t m, size_t n, double alpha,
+             const double* T, size_t ldt, double* B, size_t ldb, const int* info) {
+    if (m == 0 || n == 0) return AL_OK;
+    if (alpha != 1.0) {
+        // the crate scales the right-hand side first, then solves (src/blas.rs:1388-1403)
+        hipLaunchKernelGGL(scale_block, dim3(ceil_div(m * n, 256)), dim3(256), 0, s, (int)m, (int)n, alpha, B, ldb,
+                           info);
+        AL_HIP(hipGetLastError());
+    }
+    // op(T) is lower triangular iff lower != trans
+    const bool op_lower = (lower != trans);
+    // element (i, k) of op(T)'s diagonal block at (d, d): op(T)[i][k] = trans ? T[k][i] : T[i][k]
+    const size_t opt_rs = trans ? ldt : 1, opt_cs = trans ? 1 : ldt;
+    if (left) {
+        // op(T) X = B, T is m x m; vectors are the columns of B
+        const size_t nblk = (m + DB - 1) / DB;
+        for (size_t bi = 0; bi < nblk; ++bi) {
+            const size_t b = op_lower ? bi : (nblk - 1 - bi);
+            const size_t d = b * DB, nb = (m - d) < (size_t)DB ? (m - d) : (size_t)DB;
+            const double* Tdd = T + d + d * ldt;
+            AL_CHECK(diag_solve(s, (int)nb, (int)n, Tdd, opt_rs, opt_cs, B + d, 1, ldb, op_lower, unit, info));
+            if (op_lower && d + nb < m) {
+                // B[d+nb:, :] -= op(T)[d+nb:, d:d+nb] * X_b
+                AL_CHECK(gemm_f64(s, trans, false, m - d - nb, n, nb, -1.0, opT_block(T, ldt, trans, d + nb, d), ldt,
+                                  B + d, ldb, 1.0, B + d + nb, ldb));
+            } else if (!op_lower && d > 0) {
+                // B[:d, :] -= op(T)[:d, d:d+nb] * X_b
+                AL_CHECK(gemm_f64(s, trans, false, d, n, nb, -1.0, opT_block(T, ldt, trans, 0, d), ldt, B + d, ldb,
+                                  1.0, B, ldb));
+            }
+        }
+    } else {
+        // X op(T) = B, T is n x n; vectors are the rows of B.  Row x^T of a column block solves
+        // op(T)_bb^T x = b, whose matrix element (i, k) is op(T)[k][i].
+        const size_t mt_rs = opt_cs, mt_cs = opt_rs;
+        const size_t nblk = (n + DB - 1) / DB;
+        for (size_t bi = 0; bi < nblk; ++bi) {
+            // op(T) upper -> forward over column blocks; lower -> backward
+            const size_t b = op_lower ? (nblk - 1 - bi) : bi;
+            const size_t d = b * DB, nb = (n - d) < (size_t)DB ? (n - d) : (size_t)DB;
+            const double* Tdd = T + d + d * ldt;
+            AL_CHECK(diag_solve(s, (int)nb, (int)m, Tdd, mt_rs, mt_cs, B + d * ldb, ldb, 1, !op_lower, unit, info));
+            if (!op_lower && d + nb < n) {
+                // B[:, d+nb:] -= X_b * op(T)[d:d+nb, d+nb:]
+                AL_CHECK(gemm_f64(s, false, trans, m, n - d - nb, nb, -1.0, B + d * ldb, ldb,
+                                  opT_block(T, ldt, trans, d, d + nb), ldt, 1.0, B + (d + nb) * ldb, ldb));
+            } else if (op_lower && d > 0) {
+                // B[:, :d] -= X_b * op(T)[d:d+nb, :d]
+                AL_CHECK(gemm_f64(s, false, trans, m, d, nb, -1.0, B + d * ldb, ldb, opT_block(T, ldt, trans, d, 0),
+                                  ldt, 1.0, B, ldb));
+            }
+        }
+    }
+    return AL_OK;
+}
+
+}  // namespace al

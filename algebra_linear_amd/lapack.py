@@ -1,0 +1,69 @@
+"""``algebra_linear::lapack`` on the GPU: LU, solve and Cholesky (src/lapack/mod.rs).
+
+Same names and argument meaning as the crate; where the crate returns
+``Err(LapackError::..)`` these raise ``LapackError`` with that ``kind``.  Matrices are
+1-D float64 column-major numpy arrays overwritten in place; pivots are 0-based.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .blas import _in, _out, _p
+from .errors import BAD_SHAPE, LapackError, check_lapack
+
+_sp = ctypes.POINTER(ctypes.c_size_t)
+
+
+def _piv_out(ipiv) -> np.ndarray:
+    if not (isinstance(ipiv, np.ndarray) and ipiv.dtype == np.uintp and ipiv.flags.c_contiguous):
+        raise TypeError("ipiv must be a contiguous numpy array of dtype uintp (usize)")
+    return ipiv.reshape(-1)
+
+
+def dgetrf(a: np.ndarray, n: int):
+    """In-place P*A = L*U; returns ``(ipiv, det)`` (src/lapack/mod.rs:181)."""
+    aa = _out(a)
+    ipiv = np.zeros(n, dtype=np.uintp)
+    det = ctypes.c_double(0.0)
+    check_lapack(_lib.load().al_dgetrf(_p(aa), aa.size, n, ipiv.ctypes.data_as(_sp), ipiv.size, ctypes.byref(det)))
+    return [int(v) for v in ipiv], det.value
+
+
+def dgetrf_rect(a: np.ndarray, nrows: int, ncols: int, ipiv: np.ndarray) -> None:
+    """Panel LU of nrows x ncols, nrows >= ncols; ``ipiv`` (len ncols) filled (src/lapack/mod.rs:202)."""
+    aa, pv = _out(a), _piv_out(ipiv)
+    check_lapack(_lib.load().al_dgetrf_rect(_p(aa), aa.size, nrows, ncols, pv.ctypes.data_as(_sp), pv.size))
+
+
+def dgetrs(lu, ipiv, b: np.ndarray) -> None:
+    """Solve with a ``dgetrf`` factorization; ``n = len(b)`` (src/lapack/mod.rs:330)."""
+    lu = _in(lu)
+    bb = _out(b)
+    pv = np.ascontiguousarray(np.asarray(ipiv, dtype=np.uintp))
+    check_lapack(_lib.load().al_dgetrs(_p(lu), lu.size, pv.ctypes.data_as(_sp), pv.size, _p(bb), bb.size))
+
+
+def dgesv(a: np.ndarray, b: np.ndarray) -> None:
+    """Solve A x = b for one right-hand side, ``n = len(b)``; ``a`` is destroyed (src/lapack/mod.rs:309)."""
+    aa, bb = _out(a), _out(b)
+    check_lapack(_lib.load().al_dgesv(_p(aa), aa.size, _p(bb), bb.size))
+
+
+def dgesv_nrhs(a: np.ndarray, b: np.ndarray, n: int, nrhs: int) -> list:
+    """Extension: A X = B with B n x nrhs column-major; returns the pivots."""
+    aa, bb = _out(a), _out(b)
+    ipiv = np.zeros(max(n, 1), dtype=np.uintp)
+    check_lapack(_lib.load().al_dgesv_nrhs(_p(aa), aa.size, _p(bb), bb.size, n, nrhs, ipiv.ctypes.data_as(_sp)))
+    return [int(v) for v in ipiv[:n]]
+
+
+def dpotrf(a: np.ndarray, n: int) -> None:
+    """Lower Cholesky in place; the upper triangle is neither read nor written (src/lapack/mod.rs:368)."""
+    aa = _out(a)
+    check_lapack(_lib.load().al_dpotrf(_p(aa), aa.size, n))
+
+
+__all__ = ["dgetrf", "dgetrf_rect", "dgetrs", "dgesv", "dgesv_nrhs", "dpotrf", "LapackError", "BAD_SHAPE"]

@@ -1,0 +1,136 @@
+/*
+ * algebra_linear_hip: the dense BLAS-3 / LAPACK solve path of the algebra_linear crate
+ * on one MI355X (gfx950), behind a C ABI whose entry points mirror the crate's Rust
+ * signatures one for one.
+ *
+ * Conventions kept from the crate (README.md:25-38, AGENTS.md:21-52):
+ *   - column-major, element (i, j) at [i + j*nrow], leading dimension = row count;
+ *   - slices cross as (pointer, length); lengths are checked exactly as the crate's
+ *     assert_eq!/BadShape checks do;
+ *   - pivots are 0-based size_t: ipiv[k] = the row interchanged with row k at step k;
+ *   - no workspace arguments: the library allocates device memory internally;
+ *   - a routine that writes one triangle leaves the other untouched (and does not read it).
+ *
+ * Status codes: AL_OK, AL_SINGULAR, AL_BAD_SHAPE, AL_CONVERGENCE map to the crate's
+ * Ok / LapackError::{Singular, BadShape, Convergence} (src/lapack/mod.rs:77-96).  The
+ * crate's BLAS routines panic on a shape mismatch; here they return AL_BAD_SHAPE and
+ * al_last_error() returns the crate's panic message (e.g. "dgemm: a must be m*k"), so a
+ * binding can re-raise the same panic.  Negative codes are device/runtime failures.
+ *
+ * Thread safety: every entry point may be called from many threads at once; each thread
+ * gets its own HIP stream and device buffers (the crate's functions are reentrant too).
+ */
+#ifndef ALGEBRA_LINEAR_HIP_H
+#define ALGEBRA_LINEAR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    AL_OK = 0,
+    AL_SINGULAR = 1,     /* LapackError::Singular */
+    AL_BAD_SHAPE = 2,    /* LapackError::BadShape, or a BLAS assert_eq! failure */
+    AL_CONVERGENCE = 3,  /* LapackError::Convergence (mixed-precision refinement stalled) */
+    AL_ERR_DEVICE = -1,  /* HIP runtime error or a device-side timeout */
+    AL_ERR_NO_DEVICE = -2,
+    AL_ERR_OOM = -3,
+    AL_ERR_ARG = -4      /* invalid argument outside the crate's contract (e.g. null pointer) */
+};
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* al_last_error(void);
+/* Library version string. */
+const char* al_version(void);
+/* Number of visible HIP devices (0 if none); does not create a context. */
+int al_device_count(void);
+
+/* ------------------------------------------------------------------------------------
+ * Host-slice API: drop-in replacements of the crate's functions.  Inputs are copied to
+ * HBM, computed on the GPU, and results copied back before return.
+ * ------------------------------------------------------------------------------------ */
+
+/* blas::dgemm, src/blas.rs:201-206: C := alpha*A*B + beta*C; a m*k, b k*n, c m*n. */
+int al_dgemm(double alpha, const double* a, size_t a_len, const double* b, size_t b_len, double beta, double* c,
+             size_t c_len, size_t m, size_t n, size_t k);
+/* blas::dgemm_tn, src/blas.rs:220-225: C := alpha*A^T*B + beta*C; a k*m, b k*n. */
+int al_dgemm_tn(double alpha, const double* a, size_t a_len, const double* b, size_t b_len, double beta, double* c,
+                size_t c_len, size_t m, size_t n, size_t k);
+/* blas::dgemm_nt, src/blas.rs:237-242: C := alpha*A*B^T + beta*C; a m*k, b n*k. */
+int al_dgemm_nt(double alpha, const double* a, size_t a_len, const double* b, size_t b_len, double beta, double* c,
+                size_t c_len, size_t m, size_t n, size_t k);
+/* blas::dgemm_tt, src/blas.rs:252-257: C := alpha*A^T*B^T + beta*C; a k*m, b n*k. */
+int al_dgemm_tt(double alpha, const double* a, size_t a_len, const double* b, size_t b_len, double beta, double* c,
+                size_t c_len, size_t m, size_t n, size_t k);
+
+/* blas::dtrsm, src/blas.rs:1384-1405: B := alpha*A^-1*B (left, a m*m) or alpha*B*A^-1
+ * (right, a n*n); b m*n; lower/unit as in the crate (unit: the diagonal is not read). */
+int al_dtrsm(double alpha, const double* a, size_t a_len, double* b, size_t b_len, size_t m, size_t n, int left,
+             int lower, int unit);
+
+/* blas::dsyrk, src/blas.rs:609-626: one triangle of C := alpha*A*A^T + beta*C; a n*k, c n*n. */
+int al_dsyrk(double alpha, const double* a, size_t a_len, double beta, double* c, size_t c_len, size_t n, size_t k,
+             int upper);
+
+/* lapack::dgetrf, src/lapack/mod.rs:181-185 (dgetrf_with_pivots :112-163): in-place P*A = L*U;
+ * writes ipiv (len n) and det (sign-flipped product of U's diagonal, in the crate's order). */
+int al_dgetrf(double* a, size_t a_len, size_t n, size_t* ipiv, size_t ipiv_len, double* det);
+
+/* lapack::dgetrf_rect, src/lapack/mod.rs:202-238: panel LU of nrows x ncols, nrows >= ncols;
+ * swaps touch only the panel's columns. */
+int al_dgetrf_rect(double* a, size_t a_len, size_t nrows, size_t ncols, size_t* ipiv, size_t ipiv_len);
+
+/* lapack::dgetrs (crate-private), src/lapack/mod.rs:330-354: n = b_len, one right-hand side. */
+int al_dgetrs(const double* lu, size_t lu_len, const size_t* ipiv, size_t ipiv_len, double* b, size_t b_len);
+
+/* lapack::dgesv, src/lapack/mod.rs:309-323: n = b_len, one right-hand side; a is destroyed. */
+int al_dgesv(double* a, size_t a_len, double* b, size_t b_len);
+
+/* lapack::dpotrf, src/lapack/mod.rs:368-399: lower Cholesky in place, upper triangle untouched. */
+int al_dpotrf(double* a, size_t a_len, size_t n);
+
+/* Extension (no counterpart in the crate, whose dgesv takes one right-hand side): B is
+ * n x nrhs column-major; ipiv (nullable, len n) receives the pivots. */
+int al_dgesv_nrhs(double* a, size_t a_len, double* b, size_t b_len, size_t n, size_t nrhs, size_t* ipiv);
+
+/* Extension: multi-RHS dgetrs with the same conventions as al_dgesv_nrhs. */
+int al_dgetrs_nrhs(const double* lu, size_t lu_len, const size_t* ipiv, size_t ipiv_len, double* b, size_t b_len,
+                   size_t n, size_t nrhs);
+
+/* ------------------------------------------------------------------------------------
+ * Device-pointer API (extension): the same operations on HBM-resident operands with
+ * explicit leading dimensions, enqueued on `stream` (a hipStream_t; NULL = the calling
+ * thread's library stream).  Results are final once the stream is synchronized; the
+ * status of a factorization is written to the device int `*d_info` (0 ok, k > 0: first
+ * zero / non-positive pivot at 1-based column k, < 0 device error).  Pivots are int32
+ * 0-based global row indices on the device.
+ * ------------------------------------------------------------------------------------ */
+int al_dev_dgemm(int trans_a, int trans_b, size_t m, size_t n, size_t k, double alpha, const double* A, size_t lda,
+                 const double* B, size_t ldb, double beta, double* C, size_t ldc, void* stream);
+int al_dev_dsyrk(int upper, size_t n, size_t k, double alpha, const double* A, size_t lda, double beta, double* C,
+                 size_t ldc, void* stream);
+int al_dev_dtrsm(int left, int lower, int unit, size_t m, size_t n, double alpha, const double* A, size_t lda,
+                 double* B, size_t ldb, void* stream);
+int al_dev_dgetrf(size_t m, size_t n, double* A, size_t lda, int32_t* d_ipiv, int32_t* d_info, void* stream);
+int al_dev_dgetrs(size_t n, size_t nrhs, const double* LU, size_t lda, const int32_t* d_ipiv, double* B,
+                  size_t ldb, void* stream);
+int al_dev_dpotrf(size_t n, double* A, size_t lda, int32_t* d_info, void* stream);
+
+/* ------------------------------------------------------------------------------------
+ * Kernel instrumentation (extension): while enabled, every launch of an instrumented
+ * kernel class is bracketed by HIP events on the stream it runs on, with its algorithmic
+ * flops and bytes.  Classes: "dgemm_nn", "dgemm_tn", "dgemm_nt", "dgemm_tt",
+ * "dsyrk_lower", "dsyrk_upper", "panel_getf2".  al_prof_query synchronizes the recorded
+ * events and returns the totals for one class.
+ * ------------------------------------------------------------------------------------ */
+void al_prof_enable(int on);
+void al_prof_reset(void);
+int al_prof_query(const char* kernel_class, int* launches, double* total_ms, double* flops, double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ALGEBRA_LINEAR_HIP_H */

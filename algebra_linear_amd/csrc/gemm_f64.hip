@@ -10,9 +10,9 @@
 //     behind matrix-core time;
 //   * transposition is handled entirely in the global->LDS load pattern: both orientations
 //     land in the same k-major LDS layout, so the MFMA loop is shared by all variants;
-//   * the MFMA operands are fed as (B^T, A^T) so each lane's four accumulators are four
-//     consecutive *columns* of one C row group and 16 lanes cover 16 consecutive rows:
-//     stores and beta loads of the column-major C are contiguous 128-byte runs;
+//   * the MFMA operands are fed as (B^T, A^T) so each lane's four accumulators lie in one
+//     C row and 16 lanes cover 16 consecutive rows of a column: stores and beta loads of the
+//     column-major C are contiguous 128-byte runs;
 //   * workgroup ids are remapped so each of the 8 XCDs walks a contiguous range of tiles,
 //     grouped 8 tile-rows high, keeping A row-panels and B column-panels in that XCD's L2.
 #include "al_internal.hpp"
@@ -154,7 +154,9 @@ __global__ __launch_bounds__(NT) void dgemm_mfma(int m, int n, int k, double alp
         __syncthreads();
     }
 
-    // lane holds C[row][col0 + r], row = .. + (lane & 15), col0 = .. + 4*(lane >> 4)
+    // v_mfma_f64_16x16x4_f64 leaves D[(lane >> 4) + 4*r][lane & 15] in accumulator r; with the
+    // operands fed as (B^T, A^T), D = C^T, so lane holds C[row][col], row = .. + (lane & 15),
+    // col = .. + (lane >> 4) + 4*r
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int row = bm + wm * 64 + i * 16 + col;
@@ -163,7 +165,7 @@ __global__ __launch_bounds__(NT) void dgemm_mfma(int m, int n, int k, double alp
         for (int j = 0; j < 4; ++j) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int cc = bn + wn * 64 + j * 16 + 4 * kq + r;
+                const int cc = bn + wn * 64 + j * 16 + kq + 4 * r;
                 if (cc >= n) continue;
                 if (TRI == 1 && row < cc) continue;
                 if (TRI == 2 && row > cc) continue;

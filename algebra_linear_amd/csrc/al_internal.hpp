@@ -51,7 +51,15 @@ struct Workspace {
     size_t slots_bytes = 0;
     int* ipiv32 = nullptr;         // int32 pivots on the device
     size_t ipiv_len = 0;
+    int* pairs = nullptr;          // composed row permutation of one pivot block
+    double* perm = nullptr;        // row origins of a whole factorization (exact integers)
+    size_t perm_len = 0;
+    double* wbuf = nullptr;        // right-hand-side staging for the solves
+    size_t wbuf_len = 0;
 };
+
+// Grows the workspace's perm (n entries) and staging buffer (nw doubles).
+int ws_reserve(Workspace* ws, size_t n, size_t nw);
 
 // The calling thread's workspace on the current device (created on first use).
 int workspace(Workspace** out, size_t ipiv_len);
@@ -73,16 +81,25 @@ int gemm_f64(hipStream_t s, bool ta, bool tb, size_t m, size_t n, size_t k, doub
 int trsm_f64(hipStream_t s, bool left, bool lower, bool trans, bool unit, size_t m, size_t n, double alpha,
              const double* T, size_t ldt, double* B, size_t ldb, const int* info = nullptr);
 
-// Row interchanges ipiv[k0..k1) (int32, global row indices) on columns [0, ncols) of A.
-int laswp_f64(hipStream_t s, double* A, size_t lda, size_t ncols, const int* ipiv, size_t k0, size_t k1,
-              const int* info = nullptr);
+// Row interchanges ipiv[k0..k0+jb) (int32, global row indices, jb <= 256) applied to the n0
+// columns at A0 and the n1 columns at A1 (leading dimension lda) and, if perm is not null, to
+// the row-origin vector perm -- with the result of the sequential swaps, as one gather.
+int swap_block_f64(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, size_t jb, double* A0, size_t n0,
+                   double* A1, size_t n1, size_t lda, double* perm, const int* info);
+int perm_init(hipStream_t s, double* perm, size_t n);                          // perm[i] = i
+int perm_from_ipiv(hipStream_t s, const int* ipiv, size_t n, double* perm);    // compose all pivots
+int gather_rows(hipStream_t s, const double* B, size_t ldb, size_t n, size_t nrhs, const double* perm, double* W);
+int copy_block(hipStream_t s, const double* W, size_t n, size_t nrhs, double* B, size_t ldb);
 
-// Blocked LU with partial pivoting of the m x n matrix A (m >= n for the panel form).
-int getrf_f64(hipStream_t s, Workspace* ws, size_t m, size_t n, double* A, size_t lda, int* ipiv, int* info);
+// Blocked LU with partial pivoting of the m x n matrix A (m >= n for the panel form).  If perm
+// is not null it receives the row origins (perm[i] = original row now at row i, m entries).
+int getrf_f64(hipStream_t s, Workspace* ws, size_t m, size_t n, double* A, size_t lda, int* ipiv, int* info,
+              double* perm = nullptr);
 
-// Solve with a getrf factorization: B (n x nrhs) := A^{-1} B.
-int getrs_f64(hipStream_t s, size_t n, size_t nrhs, const double* LU, size_t lda, const int* ipiv, double* B,
-              size_t ldb, const int* info = nullptr);
+// Solve with a getrf factorization: B (n x nrhs) := A^{-1} B.  perm (nullable) is getrf's row
+// origin vector; without it the pivots are composed first.
+int getrs_f64(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const double* LU, size_t lda, const int* ipiv,
+              const double* perm, double* B, size_t ldb, const int* info = nullptr);
 
 // Lower Cholesky in place; only the lower triangle is read or written.
 int potrf_f64(hipStream_t s, Workspace* ws, size_t n, double* A, size_t lda, int* info);

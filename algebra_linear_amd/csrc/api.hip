@@ -37,6 +37,9 @@ struct ThreadCtx {
             if (w.info) (void)hipFree(w.info);
             if (w.slots) (void)hipFree(w.slots);
             if (w.ipiv32) (void)hipFree(w.ipiv32);
+            if (w.pairs) (void)hipFree(w.pairs);
+            if (w.perm) (void)hipFree(w.perm);
+            if (w.wbuf) (void)hipFree(w.wbuf);
         }
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -165,6 +168,7 @@ int workspace_for(hipStream_t s, Workspace** out, size_t ipiv_len) {
         w.abort_flag = w.counter + 1;
         w.slots_bytes = panel_slots_bytes();
         AL_HIP(hipMalloc((void**)&w.slots, w.slots_bytes));
+        AL_HIP(hipMalloc((void**)&w.pairs, (1 + 2 * 512) * sizeof(int)));
         w.device = g_ctx.device;
     }
     if (w.ipiv_len < ipiv_len) {
@@ -184,7 +188,8 @@ int getrf_host(double* a, size_t M, size_t N, size_t* ipiv, hipStream_t s, Works
     const size_t kmax = M < N ? M : N;
     // identity pivots first, so that even a partial factorization never holds a wild index
     AL_CHECK(iota_i32(s, ws->ipiv32, kmax));
-    AL_CHECK(getrf_f64(s, ws, M, N, dA, M, ws->ipiv32, ws->info));
+    AL_CHECK(ws_reserve(ws, M, 0));
+    AL_CHECK(getrf_f64(s, ws, M, N, dA, M, ws->ipiv32, ws->info, ws->perm));
     int info = 0;
     std::vector<int> piv(kmax);
     AL_CHECK(download(a, dA, M * N, s));
@@ -218,6 +223,24 @@ int thread_stream(hipStream_t* out) {
     AL_CHECK(ensure_device());
     if (!g_ctx.stream) AL_HIP(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
     *out = g_ctx.stream;
+    return AL_OK;
+}
+
+int ws_reserve(Workspace* ws, size_t n, size_t nw) {
+    if (ws->perm_len < n) {
+        if (ws->perm) AL_HIP(hipFree(ws->perm));
+        ws->perm = nullptr;
+        ws->perm_len = 0;
+        AL_HIP(hipMalloc((void**)&ws->perm, n * sizeof(double)));
+        ws->perm_len = n;
+    }
+    if (ws->wbuf_len < nw) {
+        if (ws->wbuf) AL_HIP(hipFree(ws->wbuf));
+        ws->wbuf = nullptr;
+        ws->wbuf_len = 0;
+        AL_HIP(hipMalloc((void**)&ws->wbuf, nw * sizeof(double)));
+        ws->wbuf_len = nw;
+    }
     return AL_OK;
 }
 
@@ -341,13 +364,20 @@ int al_dgetrs_nrhs(const double* lu, size_t lu_len, const size_t* ipiv, size_t i
     AL_CHECK(thread_stream(&s));
     Workspace* ws;
     AL_CHECK(workspace_for(s, &ws, n));
-    std::vector<int> piv(n);
-    for (size_t i = 0; i < n; ++i) piv[i] = (int)ipiv[i];
+    // compose the pivots into row origins on the host (O(n)), as the crate applies them
+    std::vector<double> perm(n);
+    {
+        std::vector<size_t> p(n);
+        for (size_t i = 0; i < n; ++i) p[i] = i;
+        for (size_t k = 0; k < n; ++k) std::swap(p[k], p[ipiv[k]]);
+        for (size_t i = 0; i < n; ++i) perm[i] = (double)p[i];
+    }
+    AL_CHECK(ws_reserve(ws, n, n * nrhs));
     double *dLU, *dB;
     AL_CHECK(upload(0, lu, lu_len, &dLU, s));
     AL_CHECK(upload(1, b, b_len, &dB, s));
-    AL_HIP(hipMemcpyAsync(ws->ipiv32, piv.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
-    AL_CHECK(getrs_f64(s, n, nrhs, dLU, n, ws->ipiv32, dB, n));
+    AL_HIP(hipMemcpyAsync(ws->perm, perm.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+    AL_CHECK(getrs_f64(s, ws, n, nrhs, dLU, n, nullptr, ws->perm, dB, n));
     AL_CHECK(download(b, dB, b_len, s));
     return finish(s);
 }
@@ -371,7 +401,8 @@ int al_dgesv_nrhs(double* a, size_t a_len, double* b, size_t b_len, size_t n, si
     if (nrhs == 0) return AL_OK;
     double* dB;
     AL_CHECK(upload(1, b, b_len, &dB, s));
-    AL_CHECK(getrs_f64(s, n, nrhs, dA, n, ws->ipiv32, dB, n));
+    AL_CHECK(ws_reserve(ws, n, n * nrhs));
+    AL_CHECK(getrs_f64(s, ws, n, nrhs, dA, n, ws->ipiv32, ws->perm, dB, n));
     AL_CHECK(download(b, dB, b_len, s));
     return finish(s);
 }
@@ -404,13 +435,11 @@ int al_dpotrf(double* a, size_t a_len, size_t n) {
 // device-pointer API
 // ------------------------------------------------------------------------------------------
 
+// The device API runs on the caller's stream; NULL is HIP's null (default) stream.
 static int dev_stream(void* stream, hipStream_t* s) {
-    if (stream) {
-        AL_CHECK(ensure_device());
-        *s = (hipStream_t)stream;
-        return AL_OK;
-    }
-    return thread_stream(s);
+    AL_CHECK(ensure_device());
+    *s = (hipStream_t)stream;
+    return AL_OK;
 }
 
 int al_dev_dgemm(int trans_a, int trans_b, size_t m, size_t n, size_t k, double alpha, const double* A, size_t lda,
@@ -455,7 +484,9 @@ int al_dev_dgetrs(size_t n, size_t nrhs, const double* LU, size_t lda, const int
     if (lda < n || ldb < n) return bad_shape("dgetrs: leading dimension too small");
     hipStream_t s;
     AL_CHECK(dev_stream(stream, &s));
-    return getrs_f64(s, n, nrhs, LU, lda, d_ipiv, B, ldb);
+    Workspace* ws;
+    AL_CHECK(workspace_for(s, &ws, 1));
+    return getrs_f64(s, ws, n, nrhs, LU, lda, d_ipiv, nullptr, B, ldb);
 }
 
 int al_dev_dpotrf(size_t n, double* A, size_t lda, int32_t* d_info, void* stream) {

@@ -261,11 +261,11 @@ int panel_blocked(hipStream_t s, Workspace* ws, double* A, size_t lda, size_t M,
             prof_end(ticket, s, "panel_getf2", dm * db * db - db * db * db / 3.0, 16.0 * dm * db);
         }
         // swaps on this outer panel's other columns, then U12 and the in-panel trailing update
-        AL_CHECK(laswp_f64(s, A + k0 * lda, lda, kk - k0, ipiv, kk, kk + ib, info));
         const size_t rest = k0 + n - kk - ib;
+        AL_CHECK(swap_block_f64(s, ws, ipiv, kk, ib, A + k0 * lda, kk - k0, A + (kk + ib) * lda, rest, lda, nullptr,
+                                info));
         if (rest > 0) {
             double* A12 = A + kk + (kk + ib) * lda;
-            AL_CHECK(laswp_f64(s, A + (kk + ib) * lda, lda, rest, ipiv, kk, kk + ib, info));
             AL_CHECK(trsm_f64(s, true, true, false, true, ib, rest, 1.0, A + kk + kk * lda, lda, A12, lda, info));
             if (M > kk + ib)
                 AL_CHECK(gemm_f64(s, false, false, M - kk - ib, rest, ib, -1.0, A + (kk + ib) + kk * lda, lda, A12,
@@ -277,35 +277,44 @@ int panel_blocked(hipStream_t s, Workspace* ws, double* A, size_t lda, size_t M,
 
 }  // namespace
 
-int getrf_f64(hipStream_t s, Workspace* ws, size_t M, size_t N, double* A, size_t lda, int* ipiv, int* info) {
+int getrf_f64(hipStream_t s, Workspace* ws, size_t M, size_t N, double* A, size_t lda, int* ipiv, int* info,
+              double* perm) {
     const size_t kmax = M < N ? M : N;
     if (M > (size_t)CHUNK_MAX * MAX_WG || N > INT32_MAX) {
         set_error("getrf: more rows than the cooperative panel holds (32768)");
         return AL_ERR_ARG;
     }
     AL_CHECK(fill_i32(s, info, 1, 0));
+    if (perm) AL_CHECK(perm_init(s, perm, M));
     for (size_t k = 0; k < kmax; k += NBO) {
         const size_t jb = (kmax - k) < (size_t)NBO ? (kmax - k) : (size_t)NBO;
         AL_CHECK(panel_blocked(s, ws, A, lda, M, k, jb, ipiv, info));
-        AL_CHECK(laswp_f64(s, A, lda, k, ipiv, k, k + jb, info));
-        if (k + jb < N) {
+        const size_t right = (k + jb < N) ? N - k - jb : 0;
+        AL_CHECK(swap_block_f64(s, ws, ipiv, k, jb, A, k, A + (k + jb) * lda, right, lda, perm, info));
+        if (right > 0) {
             double* A12 = A + k + (k + jb) * lda;
-            AL_CHECK(laswp_f64(s, A + (k + jb) * lda, lda, N - k - jb, ipiv, k, k + jb, info));
-            AL_CHECK(trsm_f64(s, true, true, false, true, jb, N - k - jb, 1.0, A + k + k * lda, lda, A12, lda, info));
+            AL_CHECK(trsm_f64(s, true, true, false, true, jb, right, 1.0, A + k + k * lda, lda, A12, lda, info));
             if (M > k + jb)
-                AL_CHECK(gemm_f64(s, false, false, M - k - jb, N - k - jb, jb, -1.0, A + (k + jb) + k * lda, lda, A12,
-                                  lda, 1.0, A + (k + jb) + (k + jb) * lda, lda));
+                AL_CHECK(gemm_f64(s, false, false, M - k - jb, right, jb, -1.0, A + (k + jb) + k * lda, lda, A12, lda,
+                                  1.0, A + (k + jb) + (k + jb) * lda, lda));
         }
     }
     return AL_OK;
 }
 
-int getrs_f64(hipStream_t s, size_t n, size_t nrhs, const double* LU, size_t lda, const int* ipiv, double* B,
-              size_t ldb, const int* info) {
+int getrs_f64(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const double* LU, size_t lda, const int* ipiv,
+              const double* perm, double* B, size_t ldb, const int* info) {
     if (n == 0 || nrhs == 0) return AL_OK;
-    AL_CHECK(laswp_f64(s, B, ldb, nrhs, ipiv, 0, n, info));
-    AL_CHECK(trsm_f64(s, true, true, false, true, n, nrhs, 1.0, LU, lda, B, ldb, info));
-    AL_CHECK(trsm_f64(s, true, false, false, false, n, nrhs, 1.0, LU, lda, B, ldb, info));
+    AL_CHECK(ws_reserve(ws, n, n * nrhs));
+    if (!perm) {
+        AL_CHECK(perm_from_ipiv(s, ipiv, n, ws->perm));
+        perm = ws->perm;
+    }
+    // apply the interchanges as one gather into the staging block, solve there, copy back
+    AL_CHECK(gather_rows(s, B, ldb, n, nrhs, perm, ws->wbuf));
+    AL_CHECK(trsm_f64(s, true, true, false, true, n, nrhs, 1.0, LU, lda, ws->wbuf, n, info));
+    AL_CHECK(trsm_f64(s, true, false, false, false, n, nrhs, 1.0, LU, lda, ws->wbuf, n, info));
+    AL_CHECK(copy_block(s, ws->wbuf, n, nrhs, B, ldb));
     return AL_OK;
 }
 

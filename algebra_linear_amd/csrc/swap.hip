@@ -1,0 +1,209 @@
+// Row interchanges without per-column dependency chains.
+//
+// The crate applies a pivot block as jb sequential full-row swaps (src/lapack/mod.rs:141-146).
+// Done literally on the GPU, one thread per column walks jb dependent load/store pairs with a
+// column stride of lda: latency-bound.  Here the block's swaps are first composed into one
+// permutation of the touched rows (`piv_pairs`: at most 2*jb (destination, source) pairs,
+// computed once by one workgroup), and then every column applies that permutation as an
+// independent gather (`apply_pairs`: read all sources, then write all destinations).  The
+// result is identical to the sequential swaps: each touched row ends up holding exactly the
+// row the swap sequence would have moved there.
+#include "al_internal.hpp"
+
+namespace al {
+namespace {
+
+constexpr int JB_MAX = 256;
+constexpr int PAIRS_MAX = 2 * JB_MAX;
+
+// out[0] = number of pairs; out[1 + 2p] = destination row, out[2 + 2p] = source row.
+__global__ __launch_bounds__(256) void piv_pairs(const int* __restrict__ ipiv, int k0, int jb, int* __restrict__ out,
+                                                 const int* __restrict__ info) {
+    __shared__ int piv[JB_MAX];
+    __shared__ int first[JB_MAX];
+    __shared__ int seg[JB_MAX];   // seg[i]: original row now held by row k0 + i
+    __shared__ int osrc[JB_MAX];  // osrc[s]: original row now held by row piv[s] (first occurrence s)
+    const int t = threadIdx.x;
+    if (info && *info != 0) {
+        if (t == 0) out[0] = 0;
+        return;
+    }
+    const int hi = k0 + jb;
+    if (t < jb) piv[t] = ipiv[k0 + t];
+    __syncthreads();
+    if (t < jb) {
+        const int p = piv[t];
+        int f = t;
+        if (p >= hi)
+            for (int s = 0; s < t; ++s)
+                if (piv[s] == p) {
+                    f = s;
+                    break;
+                }
+        first[t] = f;
+        seg[t] = k0 + t;
+        osrc[t] = p;
+    }
+    __syncthreads();
+    if (t == 0) {
+        for (int s = 0; s < jb; ++s) {
+            const int p = piv[s];
+            if (p < hi) {
+                const int q = p - k0;
+                const int a = seg[s];
+                seg[s] = seg[q];
+                seg[q] = a;
+            } else {
+                const int sl = first[s];
+                const int a = seg[s];
+                seg[s] = osrc[sl];
+                osrc[sl] = a;
+            }
+        }
+        int cnt = 0;
+        for (int i = 0; i < jb; ++i)
+            if (seg[i] != k0 + i) {
+                out[1 + 2 * cnt] = k0 + i;
+                out[2 + 2 * cnt] = seg[i];
+                ++cnt;
+            }
+        for (int s = 0; s < jb; ++s)
+            if (piv[s] >= hi && first[s] == s && osrc[s] != piv[s]) {
+                out[1 + 2 * cnt] = piv[s];
+                out[2 + 2 * cnt] = osrc[s];
+                ++cnt;
+            }
+        out[0] = cnt;
+    }
+}
+
+// One wave per column: columns [0, n0) of A0, then [0, n1) of A1, then (if perm) the vector
+// perm as one more column.
+__global__ __launch_bounds__(256) void apply_pairs(double* __restrict__ A0, int n0, double* __restrict__ A1, int n1,
+                                                   size_t lda, double* __restrict__ perm,
+                                                   const int* __restrict__ pairs, const int* __restrict__ info) {
+    __shared__ int pr[1 + PAIRS_MAX * 2];
+    if (info && *info != 0) return;
+    const int t = threadIdx.x;
+    const int cnt = pairs[0];
+    for (int i = t; i < 2 * cnt; i += 256) pr[1 + i] = pairs[1 + i];
+    __syncthreads();
+    const int lane = t & 63;
+    const int c = blockIdx.x * 4 + (t >> 6);
+    const int ntot = n0 + n1 + (perm ? 1 : 0);
+    double* col = nullptr;
+    if (c < n0)
+        col = A0 + (size_t)c * lda;
+    else if (c < n0 + n1)
+        col = A1 + (size_t)(c - n0) * lda;
+    else if (c < ntot)
+        col = perm;
+    double v[PAIRS_MAX / 64];
+#pragma unroll
+    for (int q = 0; q < PAIRS_MAX / 64; ++q) {
+        const int i = lane + 64 * q;
+        v[q] = (col && i < cnt) ? col[pr[2 + 2 * i]] : 0.0;
+    }
+    __syncthreads();  // every source read before any destination is written
+#pragma unroll
+    for (int q = 0; q < PAIRS_MAX / 64; ++q) {
+        const int i = lane + 64 * q;
+        if (col && i < cnt) col[pr[1 + 2 * i]] = v[q];
+    }
+}
+
+__global__ void perm_iota(double* perm, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) perm[i] = (double)i;
+}
+
+// Composes all n pivots into the row-origin vector with the swaps staged in LDS.
+__global__ __launch_bounds__(256) void perm_from_pivots(const int* __restrict__ ipiv, int n, double* __restrict__ perm) {
+    extern __shared__ int p[];
+    for (int i = threadIdx.x; i < n; i += 256) p[i] = i;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int s = 0; s < n; ++s) {
+            const int q = ipiv[s];
+            const int a = p[s];
+            p[s] = p[q];
+            p[q] = a;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += 256) perm[i] = (double)p[i];
+}
+
+__global__ void gather_rows_k(const double* __restrict__ B, size_t ldb, int n, int nrhs, const double* __restrict__ perm,
+                              double* __restrict__ W) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)n * nrhs) return;
+    const int i = (int)(idx % n), c = (int)(idx / n);
+    W[idx] = B[(size_t)perm[i] + (size_t)c * ldb];
+}
+
+__global__ void copy_block_k(const double* __restrict__ W, int n, int nrhs, double* __restrict__ B, size_t ldb) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)n * nrhs) return;
+    const int i = (int)(idx % n), c = (int)(idx / n);
+    B[(size_t)i + (size_t)c * ldb] = W[idx];
+}
+
+}  // namespace
+
+int swap_block_f64(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, size_t jb, double* A0, size_t n0,
+                   double* A1, size_t n1, size_t lda, double* perm, const int* info) {
+    if (jb == 0 || (n0 + n1 == 0 && !perm)) return AL_OK;
+    if (jb > (size_t)JB_MAX) {
+        set_error("swap_block: pivot block wider than 256");
+        return AL_ERR_ARG;
+    }
+    hipLaunchKernelGGL(piv_pairs, dim3(1), dim3(256), 0, s, ipiv, (int)k0, (int)jb, ws->pairs, info);
+    AL_HIP(hipGetLastError());
+    const size_t ncols = n0 + n1 + (perm ? 1 : 0);
+    hipLaunchKernelGGL(apply_pairs, dim3(ceil_div(ncols, 4)), dim3(256), 0, s, A0, (int)n0, A1, (int)n1, lda, perm,
+                       ws->pairs, info);
+    AL_HIP(hipGetLastError());
+    return AL_OK;
+}
+
+int perm_init(hipStream_t s, double* perm, size_t n) {
+    if (n == 0) return AL_OK;
+    hipLaunchKernelGGL(perm_iota, dim3(ceil_div(n, 256)), dim3(256), 0, s, perm, n);
+    AL_HIP(hipGetLastError());
+    return AL_OK;
+}
+
+int perm_from_ipiv(hipStream_t s, const int* ipiv, size_t n, double* perm) {
+    if (n == 0) return AL_OK;
+    if (n > 40000) {
+        set_error("getrs: pivot vector longer than the LDS composition holds");
+        return AL_ERR_ARG;
+    }
+    static bool attr = false;
+    if (!attr) {
+        AL_HIP(hipFuncSetAttribute((const void*)perm_from_pivots, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   40000 * (int)sizeof(int)));
+        attr = true;
+    }
+    hipLaunchKernelGGL(perm_from_pivots, dim3(1), dim3(256), n * sizeof(int), s, ipiv, (int)n, perm);
+    AL_HIP(hipGetLastError());
+    return AL_OK;
+}
+
+int gather_rows(hipStream_t s, const double* B, size_t ldb, size_t n, size_t nrhs, const double* perm, double* W) {
+    if (n == 0 || nrhs == 0) return AL_OK;
+    hipLaunchKernelGGL(gather_rows_k, dim3(ceil_div(n * nrhs, 256)), dim3(256), 0, s, B, ldb, (int)n, (int)nrhs, perm,
+                       W);
+    AL_HIP(hipGetLastError());
+    return AL_OK;
+}
+
+int copy_block(hipStream_t s, const double* W, size_t n, size_t nrhs, double* B, size_t ldb) {
+    if (n == 0 || nrhs == 0) return AL_OK;
+    hipLaunchKernelGGL(copy_block_k, dim3(ceil_div(n * nrhs, 256)), dim3(256), 0, s, W, (int)n, (int)nrhs, B, ldb);
+    AL_HIP(hipGetLastError());
+    return AL_OK;
+}
+
+}  // namespace al

@@ -5,8 +5,6 @@
 // diagonal block is solved by `trsm_diag` (the 64x64 triangle staged in LDS, one thread per
 // right-hand-side vector, the vector held in registers), and the off-diagonal coupling is
 // an FP64 MFMA GEMM, so almost all flops of a large solve run on the matrix cores.
-// Row interchanges (`laswp`) apply a block of pivots to a column range, one thread per
-// column, in pivot order -- the crate's full-row swap of src/lapack/mod.rs:141-146.
 #include "al_internal.hpp"
 
 namespace al {
@@ -74,22 +72,6 @@ __global__ void scale_block(int m, int n, double alpha, double* B, size_t ldb, c
     B[(size_t)i + (size_t)j * ldb] *= alpha;
 }
 
-__global__ void laswp_cols(double* A, size_t lda, int ncols, const int* __restrict__ ipiv, int k0, int k1,
-                           const int* info) {
-    if (info && *info != 0) return;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncols) return;
-    double* col = A + (size_t)c * lda;
-    for (int s = k0; s < k1; ++s) {
-        const int p = ipiv[s];
-        if (p != s) {
-            const double a = col[s];
-            col[s] = col[p];
-            col[p] = a;
-        }
-    }
-}
-
 __global__ void fill_i32_k(int* p, size_t n, int v) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -139,15 +121,6 @@ int iota_i32(hipStream_t s, int* p, size_t n) {
 int fill_u32(hipStream_t s, unsigned* p, size_t n, unsigned v) {
     if (n == 0) return AL_OK;
     hipLaunchKernelGGL(fill_u32_k, dim3(ceil_div(n, 256)), dim3(256), 0, s, p, n, v);
-    AL_HIP(hipGetLastError());
-    return AL_OK;
-}
-
-int laswp_f64(hipStream_t s, double* A, size_t lda, size_t ncols, const int* ipiv, size_t k0, size_t k1,
-              const int* info) {
-    if (ncols == 0 || k1 <= k0) return AL_OK;
-    hipLaunchKernelGGL(laswp_cols, dim3(ceil_div(ncols, 256)), dim3(256), 0, s, A, lda, (int)ncols, ipiv, (int)k0,
-                       (int)k1, info);
     AL_HIP(hipGetLastError());
     return AL_OK;
 }

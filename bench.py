@@ -81,7 +81,8 @@ def main() -> int:
     B = torch.empty_like(B0)
     ipiv = torch.empty(n, dtype=torch.int32, device=dev)
     info = torch.zeros(1, dtype=torch.int32, device=dev)
-    stream = torch.cuda.current_stream(dev)
+    stream = torch.cuda.Stream(device=dev)  # default priority; everything below is ordered on it
+    torch.cuda.set_stream(stream)
     sp = ctypes.c_void_p(stream.cuda_stream)
 
     def step():

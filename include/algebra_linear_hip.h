@@ -102,8 +102,8 @@ int al_dgetrs_nrhs(const double* lu, size_t lu_len, const size_t* ipiv, size_t i
 
 /* ------------------------------------------------------------------------------------
  * Device-pointer API (extension): the same operations on HBM-resident operands with
- * explicit leading dimensions, enqueued on `stream` (a hipStream_t; NULL = the calling
- * thread's library stream).  Results are final once the stream is synchronized; the
+ * explicit leading dimensions, enqueued on `stream` (a hipStream_t; NULL = HIP's null
+ * stream).  Results are final once the stream is synchronized; the
  * status of a factorization is written to the device int `*d_info` (0 ok, k > 0: first
  * zero / non-positive pivot at 1-based column k, < 0 device error).  Pivots are int32
  * 0-based global row indices on the device.

@@ -44,15 +44,30 @@ SIGNATURES = {
     "al_dev_dgetrf": (_i, [_sz, _sz, _v, _sz, _v, _v, _v]),
     "al_dev_dgetrs": (_i, [_sz, _sz, _v, _sz, _v, _v, _sz, _v]),
     "al_dev_dpotrf": (_i, [_sz, _v, _sz, _v, _v]),
+    "al_dev_dlaswp": (_i, [_sz, _v, _sz, _sz, _sz, _v, _v]),
+    "al_prof_enable": (None, [_i]),
+    "al_prof_reset": (None, []),
+    "al_prof_query": (_i, [ctypes.c_char_p, ctypes.POINTER(_i), ctypes.POINTER(_d), ctypes.POINTER(_d),
+                           ctypes.POINTER(_d)]),
 }
 
 _lib = None
 
 
 def load() -> ctypes.CDLL:
-    """The loaded library (raises ImportError if it has not been built)."""
+    """The loaded library (raises ImportError if it has not been built).
+
+    If PyTorch is installed it is imported first: torch bundles its own HIP runtime, and
+    loading it after this library would put two HIP runtimes in one process (torch's would
+    then see no device, and streams/pointers could not be shared).  Loaded first, torch's
+    runtime satisfies this library's libamdhip64.so.7 dependency as well.
+    """
     global _lib
     if _lib is None:
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise ImportError(
                 f"{LIB_PATH} is missing: build it with `make -C {HERE}` (hipcc, gfx950) or "

@@ -45,8 +45,9 @@ struct Workspace {
     // status word (int32): 0 = fine, k > 0 = first zero pivot (1-based column), < 0 device
     // error (a bounded spin timed out).
     int* info = nullptr;
-    unsigned* counter = nullptr;   // grid-barrier arrival counter
-    unsigned* abort_flag = nullptr;
+    unsigned* abort_flag = nullptr;  // raised by a timed-out spin; stops every cooperating workgroup
+    unsigned* flags = nullptr;     // per-workgroup publication flags of the cooperative panels
+    unsigned gen = 0;              // last generation used on `flags` (host side, monotonic)
     double* slots = nullptr;       // panel exchange area
     size_t slots_bytes = 0;
     int* ipiv32 = nullptr;         // int32 pivots on the device
@@ -56,7 +57,24 @@ struct Workspace {
     size_t perm_len = 0;
     double* wbuf = nullptr;        // right-hand-side staging for the solves
     size_t wbuf_len = 0;
+    unsigned* trsv_flags = nullptr;  // per-block completion flags of the one-launch solves
+    size_t trsv_flags_len = 0;
+    unsigned trsv_gen = 0;         // last generation used on trsv_flags (host side)
 };
+
+// The workspace of `stream` for the calling thread (created on first use).
+int workspace_on(hipStream_t stream, Workspace** out);
+
+
+// Grows the one-launch triangular solve's flag array (entries, zero-filled when new).
+int ws_reserve_flags(Workspace* ws, size_t entries);
+
+// One-launch blocked triangular solve of op = lower/upper, unit or not, with nrhs <= 16
+// right-hand sides in X (n x nrhs, FP64); the matrix may be FP64 or FP32.
+bool trsv_fits(size_t nrhs);
+template <typename TA>
+int trsv_solve(hipStream_t s, Workspace* ws, bool lower, bool unit, size_t n, size_t nrhs, const TA* A, size_t lda,
+               double* X, size_t ldx);
 
 // Grows the workspace's perm (n entries) and staging buffer (nw doubles).
 int ws_reserve(Workspace* ws, size_t n, size_t nw);

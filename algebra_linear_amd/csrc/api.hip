@@ -14,6 +14,7 @@
 namespace al {
 
 size_t panel_slots_bytes();
+size_t panel_flags_count();
 
 namespace {
 
@@ -36,10 +37,12 @@ struct ThreadCtx {
             Workspace& w = kv.second;
             if (w.info) (void)hipFree(w.info);
             if (w.slots) (void)hipFree(w.slots);
+            if (w.flags) (void)hipFree(w.flags);
             if (w.ipiv32) (void)hipFree(w.ipiv32);
             if (w.pairs) (void)hipFree(w.pairs);
             if (w.perm) (void)hipFree(w.perm);
             if (w.wbuf) (void)hipFree(w.wbuf);
+            if (w.trsv_flags) (void)hipFree(w.trsv_flags);
         }
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -161,11 +164,14 @@ int workspace_for(hipStream_t s, Workspace** out, size_t ipiv_len) {
     Workspace& w = g_ctx.ws[s];
     if (!w.info) {
         void* p;
-        // info (int) + counter[2] (unsigned), 256-byte aligned block
+        // info (int) and the abort flag, each on its own 64-byte line
         AL_HIP(hipMalloc(&p, 256));
+        AL_HIP(hipMemset(p, 0, 256));
         w.info = (int*)p;
-        w.counter = (unsigned*)((char*)p + 64);
-        w.abort_flag = w.counter + 1;
+        w.abort_flag = (unsigned*)((char*)p + 64);
+        AL_HIP(hipMalloc((void**)&w.flags, panel_flags_count() * sizeof(unsigned)));
+        AL_HIP(hipMemset(w.flags, 0, panel_flags_count() * sizeof(unsigned)));
+        w.gen = 0;
         w.slots_bytes = panel_slots_bytes();
         AL_HIP(hipMalloc((void**)&w.slots, w.slots_bytes));
         AL_HIP(hipMalloc((void**)&w.pairs, (1 + 2 * 512) * sizeof(int)));
@@ -240,6 +246,20 @@ int ws_reserve(Workspace* ws, size_t n, size_t nw) {
         ws->wbuf_len = 0;
         AL_HIP(hipMalloc((void**)&ws->wbuf, nw * sizeof(double)));
         ws->wbuf_len = nw;
+    }
+    return AL_OK;
+}
+
+int workspace_on(hipStream_t s, Workspace** out) { return workspace_for(s, out, 1); }
+
+int ws_reserve_flags(Workspace* ws, size_t entries) {
+    if (ws->trsv_flags_len < entries) {
+        if (ws->trsv_flags) AL_HIP(hipFree(ws->trsv_flags));
+        ws->trsv_flags = nullptr;
+        ws->trsv_flags_len = 0;
+        AL_HIP(hipMalloc((void**)&ws->trsv_flags, entries * sizeof(unsigned)));
+        AL_HIP(hipMemset(ws->trsv_flags, 0, entries * sizeof(unsigned)));
+        ws->trsv_flags_len = entries;
     }
     return AL_OK;
 }
@@ -496,6 +516,20 @@ int al_dev_dpotrf(size_t n, double* A, size_t lda, int32_t* d_info, void* stream
     Workspace* ws;
     AL_CHECK(workspace_for(s, &ws, 1));
     return potrf_f64(s, ws, n, A, lda, d_info);
+}
+
+int al_dev_dlaswp(size_t ncols, double* A, size_t lda, size_t k0, size_t k1, const int32_t* d_ipiv, void* stream) {
+    if (k1 < k0) return bad_shape("dlaswp: k1 < k0");
+    if (k1 > k0 && lda < k1) return bad_shape("dlaswp: leading dimension too small");
+    hipStream_t s;
+    AL_CHECK(dev_stream(stream, &s));
+    Workspace* ws;
+    AL_CHECK(workspace_for(s, &ws, 1));
+    for (size_t k = k0; k < k1; k += 256) {
+        const size_t jb = (k1 - k) < 256 ? (k1 - k) : 256;
+        AL_CHECK(swap_block_f64(s, ws, d_ipiv, k, jb, A, ncols, nullptr, 0, lda, nullptr, nullptr));
+    }
+    return AL_OK;
 }
 
 }  // extern "C"

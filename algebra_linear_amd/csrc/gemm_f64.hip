@@ -5,7 +5,9 @@
 // Design (MI355X-first, not a translation of the crate's Goto blocking):
 //   * one 256-thread workgroup (4 wave64s, 2x2) computes a 128x128 tile of C; each wave
 //     owns 64x64 = 4x4 tiles of v_mfma_f64_16x16x4_f64;
-//   * K advances in steps of 16 through double-buffered LDS; the next step's global loads
+//   * K advances in steps of 16 through double-buffered LDS (66 KiB) with at most 256
+//     registers per lane, so two workgroups share a CU and one's loads and epilogue overlap
+//     the other's MFMAs; the next step's global loads
 //     are issued into registers before the current step's MFMAs, so HBM/L2 latency hides
 //     behind matrix-core time;
 //   * transposition is handled entirely in the global->LDS load pattern: both orientations
@@ -28,42 +30,44 @@ constexpr int LDS_ROW = BM + 1;  // one-double pad breaks the strided stores' ba
 constexpr int TILE_DOUBLES = BK * LDS_ROW;
 constexpr size_t LDS_BYTES = 4 * TILE_DOUBLES * sizeof(double);  // A and B, double-buffered
 
-// Loads the 128 x 16 operand tile whose element (r, p) is base[r + p*ld] (ROWCONTIG false:
+// Loads the 128 x BK operand tile whose element (r, p) is base[r + p*ld] (ROWCONTIG false:
 // the tile's columns are contiguous, e.g. a non-transposed A) or base[p + r*ld] (true: the
-// tile's rows are contiguous, e.g. a transposed A) into 8 registers per thread, zero-filling
+// tile's rows are contiguous, e.g. a transposed A) into LPT registers per thread, zero-filling
 // outside [0, r_lim) x [0, kdim).
+constexpr int LPT = BM * BK / NT;  // operand elements per thread per tile
+
 template <bool ROWCONTIG>
 __device__ __forceinline__ void load_tile(const double* __restrict__ base, size_t ld, int r0, int r_lim, int k0,
-                                          int kdim, double (&reg)[8], int t) {
+                                          int kdim, double (&reg)[LPT], int t) {
     if (!ROWCONTIG) {
         const int r = t & 127, p0 = t >> 7;
         const bool rok = (r0 + r) < r_lim;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
+        for (int j = 0; j < LPT; ++j) {
             const int p = k0 + p0 + 2 * j;
             reg[j] = (rok && p < kdim) ? base[(size_t)(r0 + r) + (size_t)p * ld] : 0.0;
         }
     } else {
-        const int p = t & 15, rr = t >> 4;
+        const int p = t % BK, rr = t / BK;
         const bool pok = (k0 + p) < kdim;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int r = r0 + rr + 16 * j;
+        for (int j = 0; j < LPT; ++j) {
+            const int r = r0 + rr + (NT / BK) * j;
             reg[j] = (pok && r < r_lim) ? base[(size_t)(k0 + p) + (size_t)r * ld] : 0.0;
         }
     }
 }
 
 template <bool ROWCONTIG>
-__device__ __forceinline__ void store_tile(double* __restrict__ S, const double (&reg)[8], int t) {
+__device__ __forceinline__ void store_tile(double* __restrict__ S, const double (&reg)[LPT], int t) {
     if (!ROWCONTIG) {
         const int r = t & 127, p0 = t >> 7;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) S[(p0 + 2 * j) * LDS_ROW + r] = reg[j];
+        for (int j = 0; j < LPT; ++j) S[(p0 + 2 * j) * LDS_ROW + r] = reg[j];
     } else {
-        const int p = t & 15, rr = t >> 4;
+        const int p = t % BK, rr = t / BK;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) S[p * LDS_ROW + rr + 16 * j] = reg[j];
+        for (int j = 0; j < LPT; ++j) S[p * LDS_ROW + rr + (NT / BK) * j] = reg[j];
     }
 }
 
@@ -86,7 +90,7 @@ __device__ __forceinline__ void tile_coords(int tiles_m, int tiles_n, int& tm, i
 }
 
 template <bool TA, bool TB, int TRI>
-__global__ __launch_bounds__(NT) void dgemm_mfma(int m, int n, int k, double alpha, const double* __restrict__ A,
+__global__ __launch_bounds__(NT, 2) void dgemm_mfma(int m, int n, int k, double alpha, const double* __restrict__ A,
                                                  size_t lda, const double* __restrict__ B, size_t ldb, double beta,
                                                  double* __restrict__ C, size_t ldc, int tiles_m, int tiles_n) {
     extern __shared__ double lds[];
@@ -117,7 +121,7 @@ __global__ __launch_bounds__(NT) void dgemm_mfma(int m, int n, int k, double alp
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
 
-    double ra[8], rb[8];
+    double ra[LPT], rb[LPT];
     const int ktiles = (k + BK - 1) / BK;
     load_tile<A_RC>(A, lda, bm, m, 0, k, ra, t);
     load_tile<B_RC>(B, ldb, bn, n, 0, k, rb, t);

@@ -24,29 +24,32 @@ namespace {
 
 constexpr int PNB = 64;        // panel width
 constexpr int PT = 256;        // threads per panel workgroup
-constexpr int CHUNK_MAX = 256; // rows per workgroup (LDS: 256 x 64 doubles = 128 KiB)
+constexpr int CHUNK_MAX = 256; // rows per workgroup, one per thread (LDS: 256 x 64 doubles = 128 KiB)
 constexpr int MAX_WG = 128;    // cooperative workgroups per panel (one per CU at most)
-constexpr int SLOT = PNB + 2;  // key, index, row values
+constexpr int SLOT2 = 2 + 2 * PNB;  // key, index, candidate row, row j
 constexpr int NBO = 256;       // outer block (trailing-update depth)
 
 // Candidate ordering: larger key wins, ties go to the lower row (first maximum).
-__device__ __forceinline__ bool better(double k1, int i1, double k2, int i2) {
+template <typename T>
+__device__ __forceinline__ bool better(T k1, int i1, T k2, int i2) {
     return (k1 > k2) || (k1 == k2 && i1 < i2);
 }
 
 // Pivot key of a candidate: |v|, except that NaN never wins unless it sits on the diagonal,
 // where the crate's scan starts (src/lapack/mod.rs:126-134): a NaN start blocks every later
 // candidate, a later NaN is skipped.
-__device__ __forceinline__ double pivot_key(double v, int g, int j) {
-    if (v != v) return (g == j) ? INFINITY : -1.0;
-    return fabs(v);
+template <typename T>
+__device__ __forceinline__ T pivot_key(T v, int g, int j) {
+    if (v != v) return (g == j) ? (T)INFINITY : (T)-1;
+    return v < (T)0 ? -v : v;
 }
 
-__device__ __forceinline__ void block_argmax(double& key, int& idx, double* red_k, int* red_i) {
+template <typename T>
+__device__ __forceinline__ void block_argmax(T& key, int& idx, T* red_k, int* red_i) {
     // wave64 butterfly, then across the 4 waves through LDS
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
-        const double ok = __shfl_xor(key, off, 64);
+        const T ok = __shfl_xor(key, off, 64);
         const int oi = __shfl_xor(idx, off, 64);
         if (better(ok, oi, key, idx)) {
             key = ok;
@@ -70,136 +73,178 @@ __device__ __forceinline__ void block_argmax(double& key, int& idx, double* red_
     __syncthreads();
 }
 
-// Grid barrier over `nwg` co-resident workgroups: arrival counter with release/acquire at
-// agent scope.  The spin is bounded: past `timeout` ticks of the 100 MHz wall clock it
-// raises the abort flag, so a barrier can never hang the device.  Returns false on abort.
-__device__ bool grid_barrier(unsigned* counter, unsigned* abort_flag, unsigned target, long long timeout,
-                             int* ok_lds) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int ok = 1;
-        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        const long long t0 = wall_clock64();
-        while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
-                ok = 0;
-                break;
-            }
-            if (wall_clock64() - t0 > timeout) {
-                __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                ok = 0;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-        *ok_lds = ok;
-    }
-    __syncthreads();
-    return *ok_lds != 0;
-}
-
 #pragma clang fp contract(off)
 // Factors the m x ncols panel at P (leading dimension lda), whose top-left element is the
-// global diagonal element (row0, row0).  Workgroup w owns panel rows [w*chunk, (w+1)*chunk).
-__global__ __launch_bounds__(PT) void panel_getf2(double* __restrict__ P, size_t lda, int row0, int m, int ncols,
+// global diagonal element (row0, row0).  Workgroup w owns panel rows [w*chunk, (w+1)*chunk)
+// (chunk <= 256, one row per thread) and keeps them in LDS for the whole panel.
+//
+// Column exchange (COOP, several workgroups): wave 0 writes the workgroup's slot -- its best
+// candidate (key, row index) with that row, and row j if it owns it -- and raises the
+// workgroup's flag to the column's generation with one release store.  Wave 0 polls every
+// flag with relaxed loads (no contended counter), takes one acquire fence, and copies the
+// winner's row and row j into LDS.  Slots alternate by column parity: a workgroup overwrites
+// parity p only after every workgroup has published the next column, which each does only
+// after it has finished reading parity p.  Every spin is bounded by `timeout` ticks of the
+// 100 MHz wall clock; on expiry the abort flag stops every workgroup and info is set to -1.
+// A single workgroup exchanges through LDS only.
+//
+// Pipelining: after the swap, column j+1 is eliminated first, the next candidate is chosen,
+// its row (and row j+1) are brought up to date, and the slot is published before the rest of
+// the elimination, which then overlaps the other workgroups' publication latency.
+template <typename T, bool COOP>
+__global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda, int row0, int m, int ncols,
                                                   int chunk, int* __restrict__ ipiv, int* __restrict__ info,
-                                                  unsigned* counter, unsigned* abort_flag,
-                                                  double* __restrict__ slots, long long timeout) {
-    extern __shared__ double S[];  // chunk x PNB, column-major with leading dimension `chunk`
-    __shared__ double prow[PNB];
-    __shared__ double rj[PNB];
-    __shared__ double red_k[PT / 64];
+                                                  unsigned* __restrict__ flags, unsigned* __restrict__ abort_flag,
+                                                  T* __restrict__ slots, unsigned gen0, long long timeout) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    T* S = (T*)smem;  // chunk x PNB, column-major with leading dimension `chunk`
+    __shared__ T lslot[2][SLOT2];  // single-workgroup exchange
+    __shared__ T prow_c[PNB];
+    __shared__ T rj_c[PNB];
+    __shared__ T red_k[PT / 64];
     __shared__ int red_i[PT / 64];
+    __shared__ int win_lds;
+    __shared__ T wkey_lds;
     __shared__ int ok_lds;
 
-    const int nwg = gridDim.x;
-    const int w = blockIdx.x, t = threadIdx.x;
-    if (*info != 0) return;  // an earlier panel hit a zero pivot
+    const int nwg = COOP ? (int)gridDim.x : 1;
+    const int w = COOP ? (int)blockIdx.x : 0;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    if (*info != 0) return;  // an earlier panel hit a zero pivot or failed
     const int rb = w * chunk;
-    const int nr = (m - rb) < chunk ? (m - rb) : chunk;
-    double* rowj_buf = slots + (size_t)2 * nwg * SLOT;  // [2][PNB]
+    const int nr = (m - rb) < chunk ? ((m - rb) > 0 ? (m - rb) : 0) : chunk;
+    const bool has = t < nr;  // this thread's row: panel row rb + t
+    const int g = rb + t;
 
-    for (int idx = t; idx < nr * ncols; idx += PT) {
-        const int r = idx % nr, c = idx / nr;
-        S[c * chunk + r] = P[(size_t)(rb + r) + (size_t)c * lda];
-    }
+    for (int c = 0; c < ncols; ++c)
+        if (has) S[c * chunk + t] = P[(size_t)g + (size_t)c * lda];
     __syncthreads();
 
-    unsigned bar = 0;
-    int stop = 0;
-    for (int j = 0; j < ncols; ++j) {
-        // 1. this workgroup's candidate for pivot j
-        double key = -2.0;
-        int idx = 0x7fffffff;
-        for (int r = t; r < nr; r += PT) {
-            const int g = rb + r;
-            if (g < j) continue;
-            const double kv = pivot_key(S[j * chunk + r], g, j);
-            if (better(kv, g, key, idx)) {
-                key = kv;
-                idx = g;
+    auto slot_ptr = [&](int par, int q) -> T* {
+        return COOP ? slots + ((size_t)par * nwg + q) * SLOT2 : &lslot[par][0];
+    };
+    // wave 0 writes this workgroup's slot for column jj and raises its flag
+    auto publish = [&](int jj, T key, int idx) {
+        if (wave == 0) {
+            T* my = slot_ptr(jj & 1, w);
+            if (key > (T)-2) {
+                const int lr = idx - rb;
+                for (int c = lane; c < ncols; c += 64) my[2 + c] = S[c * chunk + lr];
+            }
+            if (jj >= rb && jj < rb + nr)
+                for (int c = lane; c < ncols; c += 64) my[2 + PNB + c] = S[c * chunk + (jj - rb)];
+            if (lane == 0) {
+                my[0] = key;
+                my[1] = (T)idx;
+                if (COOP)
+                    __hip_atomic_store(&flags[w * 16], gen0 + (unsigned)jj + 1u, __ATOMIC_RELEASE,
+                                       __HIP_MEMORY_SCOPE_AGENT);
             }
         }
-        block_argmax(key, idx, red_k, red_i);
-
-        // 2. publish it (with its row) and, if owned here, row j
-        const int par = j & 1;
-        double* my = slots + ((size_t)par * nwg + w) * SLOT;
-        if (key > -2.0) {
-            const int lr = idx - rb;
-            for (int c = t; c < ncols; c += PT) my[2 + c] = S[c * chunk + lr];
-        }
-        if (t == 0) {
-            my[0] = key;
-            my[1] = (double)idx;
-        }
-        if (j >= rb && j < rb + nr)
-            for (int c = t; c < ncols; c += PT) rowj_buf[par * PNB + c] = S[c * chunk + (j - rb)];
-        __threadfence();
-
-        // 3. one grid barrier per column
-        if (nwg > 1) {
-            ++bar;
-            if (!grid_barrier(counter, abort_flag, bar * (unsigned)nwg, timeout, &ok_lds)) {
-                if (w == 0 && t == 0) *info = -1;
-                stop = 1;
-                break;
+    };
+    // everyone: the winner of column jj with its row in prow and row jj in rj; false on abort
+    auto exchange = [&](int jj, const T*& prow, const T*& rj) -> bool {
+        const int par = jj & 1;
+        if (COOP) {
+            if (wave == 0) {
+                int ok = 1;
+                const unsigned want = gen0 + (unsigned)jj + 1u;
+                const long long t0 = wall_clock64();
+                for (;;) {
+                    bool mine = true;
+                    for (int q = lane; q < nwg; q += 64)
+                        mine = mine && (__hip_atomic_load(&flags[q * 16], __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT) >= want);
+                    if (__all(mine)) break;
+                    int stop = 0;
+                    if (lane == 0) {
+                        if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+                            stop = 1;
+                        } else if (wall_clock64() - t0 > timeout) {
+                            __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            stop = 1;
+                        }
+                    }
+                    if (__shfl(stop, 0, 64)) {
+                        ok = 0;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                if (ok) {
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                    T bk = (T)-3;
+                    int bi = 0x7fffffff;
+                    for (int q = lane; q < nwg; q += 64) {
+                        const T* sq = slot_ptr(par, q);
+                        const T kq = sq[0];
+                        const int iq = (int)sq[1];
+                        if (better(kq, iq, bk, bi)) {
+                            bk = kq;
+                            bi = iq;
+                        }
+                    }
+#pragma unroll
+                    for (int off = 32; off >= 1; off >>= 1) {
+                        const T ok2 = __shfl_xor(bk, off, 64);
+                        const int oi = __shfl_xor(bi, off, 64);
+                        if (better(ok2, oi, bk, bi)) {
+                            bk = ok2;
+                            bi = oi;
+                        }
+                    }
+                    const T* win = slot_ptr(par, bi / chunk);
+                    const T* own = slot_ptr(par, jj / chunk);
+                    for (int c = lane; c < ncols; c += 64) {
+                        prow_c[c] = win[2 + c];
+                        rj_c[c] = own[2 + PNB + c];
+                    }
+                    if (lane == 0) {
+                        win_lds = bi;
+                        wkey_lds = bk;
+                    }
+                }
+                if (lane == 0) ok_lds = ok;
             }
+            __syncthreads();
+            prow = prow_c;
+            rj = rj_c;
+            return ok_lds != 0;
         } else {
             __syncthreads();
-        }
-
-        // 4. the same winner everywhere: the best of all published candidates
-        double bk = -2.0;
-        int bi = 0x7fffffff, bw = 0;
-        for (int q = t; q < nwg; q += PT) {
-            const double* sq = slots + ((size_t)par * nwg + q) * SLOT;
-            const double kq = sq[0];
-            const int iq = (int)sq[1];
-            if (better(kq, iq, bk, bi)) {
-                bk = kq;
-                bi = iq;
-                bw = q;
+            const T* sl = &lslot[par][0];
+            prow = sl + 2;
+            rj = sl + 2 + PNB;
+            if (t == 0) {
+                win_lds = (int)sl[1];
+                wkey_lds = sl[0];
             }
+            __syncthreads();
+            return true;
         }
-        // reduce (key, idx) and recover the owner workgroup from the row index
-        block_argmax(bk, bi, red_k, red_i);
-        bw = bi / chunk;
-        if (!(bk > 0.0)) {
-            // |pivot| <= 0: singular (src/lapack/mod.rs:136-138); NaN diagonals carry +inf
-            if (w == 0 && t == 0) *info = row0 + j + 1;
-            stop = 1;
+    };
+
+    {
+        T key = has ? pivot_key(S[t], g, 0) : (T)-2;
+        int idx = has ? g : 0x7fffffff;
+        block_argmax(key, idx, red_k, red_i);
+        publish(0, key, idx);
+    }
+
+    for (int j = 0; j < ncols; ++j) {
+        const T* prow;
+        const T* rj;
+        if (!exchange(j, prow, rj)) {
+            if (w == 0 && t == 0) *info = -1;
             break;
         }
-        const double* win = slots + ((size_t)par * nwg + bw) * SLOT;
-        for (int c = t; c < ncols; c += PT) {
-            prow[c] = win[2 + c];
-            rj[c] = rowj_buf[par * PNB + c];
+        const int p = win_lds;
+        if (!(wkey_lds > (T)0)) {
+            // |pivot| <= 0: singular (src/lapack/mod.rs:136-138); a NaN diagonal carries +inf
+            if (w == 0 && t == 0) *info = row0 + j + 1;
+            break;
         }
-        __syncthreads();
-
-        // 5. swap rows j and bi (full panel width), record the pivot
-        const int p = bi;
+        // swap rows j and p across the panel, record the pivot
         if (p != j) {
             if (p >= rb && p < rb + nr)
                 for (int c = t; c < ncols; c += PT) S[c * chunk + (p - rb)] = rj[c];
@@ -209,57 +254,104 @@ __global__ __launch_bounds__(PT) void panel_getf2(double* __restrict__ P, size_t
         if (w == 0 && t == 0) ipiv[row0 + j] = row0 + p;
         __syncthreads();
 
-        // 6. eliminate below the pivot: factor = a/d, then a -= factor*u, unfused
-        const double d = prow[j];
-        for (int r = t; r < nr; r += PT) {
-            const int g = rb + r;
-            if (g <= j) continue;
-            const double f = S[j * chunk + r] / d;
-            S[j * chunk + r] = f;
-            for (int c = j + 1; c < ncols; ++c) S[c * chunk + r] -= f * prow[c];
+        const T d = prow[j];
+        const bool mine = has && g > j;
+        if (j + 1 < ncols) {
+            // multipliers of column j and column j+1's update, then the next candidate
+            T key = (T)-2;
+            int idx = 0x7fffffff;
+            if (mine) {
+                const T f = S[j * chunk + t] / d;
+                S[j * chunk + t] = f;
+                const T v = S[(j + 1) * chunk + t] - f * prow[j + 1];
+                S[(j + 1) * chunk + t] = v;
+                key = pivot_key(v, g, j + 1);
+                idx = g;
+            }
+            block_argmax(key, idx, red_k, red_i);
+            // bring the candidate's row and row j+1 up to date, publish, then finish the rest
+            const int lc = (key > (T)-2) ? idx - rb : -1;
+            const int ln = (j + 1 >= rb && j + 1 < rb + nr) ? j + 1 - rb : -1;
+            for (int c = j + 2 + t; c < ncols; c += PT) {
+                if (lc >= 0) S[c * chunk + lc] -= S[j * chunk + lc] * prow[c];
+                if (ln >= 0 && ln != lc) S[c * chunk + ln] -= S[j * chunk + ln] * prow[c];
+            }
+            __syncthreads();
+            publish(j + 1, key, idx);
+            if (mine && t != lc && t != ln) {
+                const T f = S[j * chunk + t];
+                int c = j + 2;
+                for (; c + 8 <= ncols; c += 8) {
+                    T v[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) v[q] = S[(c + q) * chunk + t];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) S[(c + q) * chunk + t] = v[q] - f * prow[c + q];
+                }
+                for (; c < ncols; ++c) S[c * chunk + t] -= f * prow[c];
+            }
+            __syncthreads();
+        } else {
+            if (mine) S[j * chunk + t] = S[j * chunk + t] / d;
+            __syncthreads();
         }
-        __syncthreads();
     }
-    (void)stop;
-    for (int idx = t; idx < nr * ncols; idx += PT) {
-        const int r = idx % nr, c = idx / nr;
-        P[(size_t)(rb + r) + (size_t)c * lda] = S[c * chunk + r];
-    }
+    for (int c = 0; c < ncols; ++c)
+        if (has) P[(size_t)g + (size_t)c * lda] = S[c * chunk + t];
 }
 #pragma clang fp contract(on)
+
+template <typename T>
+int launch_panel(hipStream_t s, Workspace* ws, T* Ppanel, size_t lda, size_t row0, size_t m, size_t ncols,
+                 int* ipiv, int* info) {
+    static bool attr_set = false;
+    if (!attr_set) {
+        AL_HIP(hipFuncSetAttribute((const void*)panel_getf2<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(CHUNK_MAX * PNB * sizeof(T))));
+        AL_HIP(hipFuncSetAttribute((const void*)panel_getf2<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(CHUNK_MAX * PNB * sizeof(T))));
+        attr_set = true;
+    }
+    size_t nwg = (m + CHUNK_MAX - 1) / CHUNK_MAX;
+    if (nwg > (size_t)MAX_WG) nwg = MAX_WG;
+    const size_t chunk = (m + nwg - 1) / nwg;
+    if (chunk > (size_t)CHUNK_MAX) {
+        set_error("getrf: panel taller than the cooperative kernel holds");
+        return AL_ERR_ARG;
+    }
+    if (ws->gen > 0xF0000000u) {
+        AL_CHECK(fill_u32(s, ws->flags, (size_t)MAX_WG * 16, 0u));
+        ws->gen = 0;
+    }
+    const long long timeout = 200000000LL;  // 2 s at 100 MHz
+    const unsigned gen0 = ws->gen;
+    ws->gen += (unsigned)ncols;
+    const int ticket = prof_begin(s);
+    if (nwg > 1) {
+        hipLaunchKernelGGL((panel_getf2<T, true>), dim3((unsigned)nwg), dim3(PT), chunk * PNB * sizeof(T), s, Ppanel,
+                           lda, (int)row0, (int)m, (int)ncols, (int)chunk, ipiv, info, ws->flags, ws->abort_flag,
+                           (T*)ws->slots, gen0, timeout);
+    } else {
+        hipLaunchKernelGGL((panel_getf2<T, false>), dim3(1), dim3(PT), chunk * PNB * sizeof(T), s, Ppanel, lda,
+                           (int)row0, (int)m, (int)ncols, (int)chunk, ipiv, info, ws->flags, ws->abort_flag,
+                           (T*)ws->slots, gen0, timeout);
+    }
+    AL_HIP(hipGetLastError());
+    if (ticket >= 0) {
+        // unblocked LU of an m x ncols panel: m*ncols^2 - ncols^3/3 flops; the panel read and written once
+        const double dm = (double)m, db = (double)ncols;
+        prof_end(ticket, s, "panel_getf2", dm * db * db - db * db * db / 3.0, 2.0 * sizeof(T) * dm * db);
+    }
+    return AL_OK;
+}
 
 // Factors the m x n block at (k0, k0) of A with 64-wide cooperative panels; swaps are applied
 // inside columns [k0, k0 + n) only -- the caller applies them outside.
 int panel_blocked(hipStream_t s, Workspace* ws, double* A, size_t lda, size_t M, size_t k0, size_t n, int* ipiv,
                   int* info) {
-    static bool attr_set = false;
-    if (!attr_set) {
-        AL_HIP(hipFuncSetAttribute((const void*)panel_getf2, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)(CHUNK_MAX * PNB * sizeof(double))));
-        attr_set = true;
-    }
     for (size_t kk = k0; kk < k0 + n; kk += PNB) {
         const size_t ib = (k0 + n - kk) < (size_t)PNB ? (k0 + n - kk) : (size_t)PNB;
-        const size_t m = M - kk;
-        size_t nwg = (m + CHUNK_MAX - 1) / CHUNK_MAX;
-        if (nwg > (size_t)MAX_WG) nwg = MAX_WG;
-        const size_t chunk = (m + nwg - 1) / nwg;
-        if (chunk > (size_t)CHUNK_MAX) {
-            set_error("getrf: panel taller than the cooperative kernel holds");
-            return AL_ERR_ARG;
-        }
-        AL_CHECK(fill_u32(s, ws->counter, 2, 0u));  // counter and abort flag
-        const long long timeout = 200000000LL;     // 2 s at 100 MHz
-        const int ticket = prof_begin(s);
-        hipLaunchKernelGGL(panel_getf2, dim3((unsigned)nwg), dim3(PT), chunk * PNB * sizeof(double), s,
-                           A + kk + kk * lda, lda, (int)kk, (int)m, (int)ib, (int)chunk, ipiv, info, ws->counter,
-                           ws->counter + 1, ws->slots, timeout);
-        AL_HIP(hipGetLastError());
-        if (ticket >= 0) {
-            // unblocked LU of an m x ib panel: m*ib^2 - ib^3/3 flops; the panel read and written once
-            const double dm = (double)m, db = (double)ib;
-            prof_end(ticket, s, "panel_getf2", dm * db * db - db * db * db / 3.0, 16.0 * dm * db);
-        }
+        AL_CHECK(launch_panel<double>(s, ws, A + kk + kk * lda, lda, kk, M - kk, ib, ipiv, info));
         // swaps on this outer panel's other columns, then U12 and the in-panel trailing update
         const size_t rest = k0 + n - kk - ib;
         AL_CHECK(swap_block_f64(s, ws, ipiv, kk, ib, A + k0 * lda, kk - k0, A + (kk + ib) * lda, rest, lda, nullptr,
@@ -285,6 +377,7 @@ int getrf_f64(hipStream_t s, Workspace* ws, size_t M, size_t N, double* A, size_
         return AL_ERR_ARG;
     }
     AL_CHECK(fill_i32(s, info, 1, 0));
+    AL_CHECK(fill_u32(s, ws->abort_flag, 1, 0u));
     if (perm) AL_CHECK(perm_init(s, perm, M));
     for (size_t k = 0; k < kmax; k += NBO) {
         const size_t jb = (kmax - k) < (size_t)NBO ? (kmax - k) : (size_t)NBO;
@@ -312,12 +405,18 @@ int getrs_f64(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const double*
     }
     // apply the interchanges as one gather into the staging block, solve there, copy back
     AL_CHECK(gather_rows(s, B, ldb, n, nrhs, perm, ws->wbuf));
-    AL_CHECK(trsm_f64(s, true, true, false, true, n, nrhs, 1.0, LU, lda, ws->wbuf, n, info));
-    AL_CHECK(trsm_f64(s, true, false, false, false, n, nrhs, 1.0, LU, lda, ws->wbuf, n, info));
+    if (trsv_fits(nrhs)) {
+        AL_CHECK(trsv_solve<double>(s, ws, true, true, n, nrhs, LU, lda, ws->wbuf, n));
+        AL_CHECK(trsv_solve<double>(s, ws, false, false, n, nrhs, LU, lda, ws->wbuf, n));
+    } else {
+        AL_CHECK(trsm_f64(s, true, true, false, true, n, nrhs, 1.0, LU, lda, ws->wbuf, n, info));
+        AL_CHECK(trsm_f64(s, true, false, false, false, n, nrhs, 1.0, LU, lda, ws->wbuf, n, info));
+    }
     AL_CHECK(copy_block(s, ws->wbuf, n, nrhs, B, ldb));
     return AL_OK;
 }
 
-size_t panel_slots_bytes() { return ((size_t)2 * MAX_WG * SLOT + 2 * PNB) * sizeof(double); }
+size_t panel_slots_bytes() { return (size_t)2 * MAX_WG * SLOT2 * sizeof(double); }
+size_t panel_flags_count() { return (size_t)MAX_WG * 16; }
 
 }  // namespace al

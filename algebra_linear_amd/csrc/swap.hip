@@ -4,7 +4,7 @@
 // Done literally on the GPU, one thread per column walks jb dependent load/store pairs with a
 // column stride of lda: latency-bound.  Here the block's swaps are first composed into one
 // permutation of the touched rows (`piv_pairs`: at most 2*jb (destination, source) pairs,
-// computed once by one workgroup), and then every column applies that permutation as an
+// computed once, each touched row traced back through the swaps in parallel), and then every column applies that permutation as an
 // independent gather (`apply_pairs`: read all sources, then write all destinations).  The
 // result is identical to the sequential swaps: each touched row ends up holding exactly the
 // row the swap sequence would have moved there.
@@ -16,65 +16,53 @@ namespace {
 constexpr int JB_MAX = 256;
 constexpr int PAIRS_MAX = 2 * JB_MAX;
 
-// out[0] = number of pairs; out[1 + 2p] = destination row, out[2 + 2p] = source row.
-__global__ __launch_bounds__(256) void piv_pairs(const int* __restrict__ ipiv, int k0, int jb, int* __restrict__ out,
+// Composes the pivots ipiv[k0..k0+jb) into (destination, source) row pairs: out[0] = count,
+// out[1 + 2p] = destination row, out[2 + 2p] = source row.  Every row the block touches --
+// the rows k0..k0+jb-1 and the pivot rows below them -- is traced backwards through the swaps
+// independently (one thread per row, the swaps broadcast from LDS): walking the sequence in
+// reverse from a destination row yields the original row whose value ends there.
+__global__ __launch_bounds__(512) void piv_pairs(const int* __restrict__ ipiv, int k0, int jb, int* __restrict__ out,
                                                  const int* __restrict__ info) {
     __shared__ int piv[JB_MAX];
-    __shared__ int first[JB_MAX];
-    __shared__ int seg[JB_MAX];   // seg[i]: original row now held by row k0 + i
-    __shared__ int osrc[JB_MAX];  // osrc[s]: original row now held by row piv[s] (first occurrence s)
+    __shared__ int cnt;
     const int t = threadIdx.x;
     if (info && *info != 0) {
         if (t == 0) out[0] = 0;
         return;
     }
-    const int hi = k0 + jb;
     if (t < jb) piv[t] = ipiv[k0 + t];
+    if (t == 0) cnt = 0;
     __syncthreads();
+    const int hi = k0 + jb;
+    int dst = -1;
     if (t < jb) {
-        const int p = piv[t];
-        int f = t;
-        if (p >= hi)
-            for (int s = 0; s < t; ++s)
-                if (piv[s] == p) {
-                    f = s;
+        dst = k0 + t;
+    } else if (t < 2 * jb) {
+        const int s0 = t - jb;
+        const int p = piv[s0];
+        if (p >= hi) {
+            dst = p;
+            for (int s = 0; s < s0; ++s)
+                if (piv[s] == p) {  // not the first step that touched this row
+                    dst = -1;
                     break;
                 }
-        first[t] = f;
-        seg[t] = k0 + t;
-        osrc[t] = p;
+        }
+    }
+    int x = dst;
+    if (dst >= 0) {
+        for (int s = jb - 1; s >= 0; --s) {
+            const int a = k0 + s, b = piv[s];
+            x = (x == a) ? b : (x == b) ? a : x;
+        }
+    }
+    if (dst >= 0 && x != dst) {
+        const int at = atomicAdd(&cnt, 1);
+        out[1 + 2 * at] = dst;
+        out[2 + 2 * at] = x;
     }
     __syncthreads();
-    if (t == 0) {
-        for (int s = 0; s < jb; ++s) {
-            const int p = piv[s];
-            if (p < hi) {
-                const int q = p - k0;
-                const int a = seg[s];
-                seg[s] = seg[q];
-                seg[q] = a;
-            } else {
-                const int sl = first[s];
-                const int a = seg[s];
-                seg[s] = osrc[sl];
-                osrc[sl] = a;
-            }
-        }
-        int cnt = 0;
-        for (int i = 0; i < jb; ++i)
-            if (seg[i] != k0 + i) {
-                out[1 + 2 * cnt] = k0 + i;
-                out[2 + 2 * cnt] = seg[i];
-                ++cnt;
-            }
-        for (int s = 0; s < jb; ++s)
-            if (piv[s] >= hi && first[s] == s && osrc[s] != piv[s]) {
-                out[1 + 2 * cnt] = piv[s];
-                out[2 + 2 * cnt] = osrc[s];
-                ++cnt;
-            }
-        out[0] = cnt;
-    }
+    if (t == 0) out[0] = cnt;
 }
 
 // One wave per column: columns [0, n0) of A0, then [0, n1) of A1, then (if perm) the vector
@@ -117,21 +105,25 @@ __global__ void perm_iota(double* perm, size_t n) {
     if (i < n) perm[i] = (double)i;
 }
 
-// Composes all n pivots into the row-origin vector with the swaps staged in LDS.
+// Row origins of a whole pivot sequence: thread i traces row i backwards through all n swaps
+// (the pivots staged through LDS in chunks), ending at the original row that lands at i.
 __global__ __launch_bounds__(256) void perm_from_pivots(const int* __restrict__ ipiv, int n, double* __restrict__ perm) {
-    extern __shared__ int p[];
-    for (int i = threadIdx.x; i < n; i += 256) p[i] = i;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int s = 0; s < n; ++s) {
-            const int q = ipiv[s];
-            const int a = p[s];
-            p[s] = p[q];
-            p[q] = a;
+    __shared__ int piv[1024];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    int x = i;
+    const int nchunks = (n + 1023) / 1024;
+    for (int ch = nchunks - 1; ch >= 0; --ch) {
+        const int base = ch * 1024;
+        const int len = (n - base) < 1024 ? (n - base) : 1024;
+        __syncthreads();
+        for (int q = threadIdx.x; q < len; q += 256) piv[q] = ipiv[base + q];
+        __syncthreads();
+        for (int s = len - 1; s >= 0; --s) {
+            const int a = base + s, b = piv[s];
+            x = (x == a) ? b : (x == b) ? a : x;
         }
     }
-    __syncthreads();
-    for (int i = threadIdx.x; i < n; i += 256) perm[i] = (double)p[i];
+    if (i < n) perm[i] = (double)x;
 }
 
 __global__ void gather_rows_k(const double* __restrict__ B, size_t ldb, int n, int nrhs, const double* __restrict__ perm,
@@ -158,7 +150,7 @@ int swap_block_f64(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, siz
         set_error("swap_block: pivot block wider than 256");
         return AL_ERR_ARG;
     }
-    hipLaunchKernelGGL(piv_pairs, dim3(1), dim3(256), 0, s, ipiv, (int)k0, (int)jb, ws->pairs, info);
+    hipLaunchKernelGGL(piv_pairs, dim3(1), dim3(512), 0, s, ipiv, (int)k0, (int)jb, ws->pairs, info);
     AL_HIP(hipGetLastError());
     const size_t ncols = n0 + n1 + (perm ? 1 : 0);
     hipLaunchKernelGGL(apply_pairs, dim3(ceil_div(ncols, 4)), dim3(256), 0, s, A0, (int)n0, A1, (int)n1, lda, perm,
@@ -176,17 +168,7 @@ int perm_init(hipStream_t s, double* perm, size_t n) {
 
 int perm_from_ipiv(hipStream_t s, const int* ipiv, size_t n, double* perm) {
     if (n == 0) return AL_OK;
-    if (n > 40000) {
-        set_error("getrs: pivot vector longer than the LDS composition holds");
-        return AL_ERR_ARG;
-    }
-    static bool attr = false;
-    if (!attr) {
-        AL_HIP(hipFuncSetAttribute((const void*)perm_from_pivots, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   40000 * (int)sizeof(int)));
-        attr = true;
-    }
-    hipLaunchKernelGGL(perm_from_pivots, dim3(1), dim3(256), n * sizeof(int), s, ipiv, (int)n, perm);
+    hipLaunchKernelGGL(perm_from_pivots, dim3(ceil_div(n, 256)), dim3(256), 0, s, ipiv, (int)n, perm);
     AL_HIP(hipGetLastError());
     return AL_OK;
 }

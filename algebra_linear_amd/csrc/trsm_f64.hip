@@ -2,28 +2,28 @@
 //
 // blas::dtrsm (src/blas.rs:1384-1438) solves one vector at a time with a scalar
 // substitution.  Here a triangular solve is blocked in 64-wide diagonal blocks: each
-// diagonal block is solved by `trsm_diag` (the 64x64 triangle staged in LDS, one thread per
-// right-hand-side vector, the vector held in registers), and the off-diagonal coupling is
-// an FP64 MFMA GEMM, so almost all flops of a large solve run on the matrix cores.
+// diagonal block is solved by `trsm_diag` (substitution, one thread per right-hand-side
+// vector), and the off-diagonal coupling is an FP64 MFMA GEMM, so almost all flops of a
+// large solve run on the matrix cores.
 #include "al_internal.hpp"
 
 namespace al {
 namespace {
 
 constexpr int DB = 64;   // diagonal block order
-constexpr int DT = 256;  // threads (= vectors) per workgroup
 
 // Solves M x = x in place for nvec vectors, M the nb x nb (nb <= 64) triangle whose element
-// (i, k) is T[i*t_rs + k*t_cs]; vector v's element i is X[i*x_es + v*x_vs].  `forward`
-// selects the lower sweep.  Only the triangle (and the diagonal unless `unit`) is read.
-__global__ __launch_bounds__(DT) void trsm_diag(int nb, int nvec, const double* __restrict__ T, size_t t_rs,
-                                                size_t t_cs, double* __restrict__ X, size_t x_es, size_t x_vs,
-                                                int forward, int unit, const int* __restrict__ info) {
+// (i, k) is T[i*t_rs + k*t_cs]; vector v's element i is X[i*x_es + v*x_vs].  One thread per
+// vector, the vector in registers, the triangle staged in LDS (padded to 64 with the
+// identity).  Each x_i is the crate's row sweep, its terms subtracted in the crate's order
+// (src/blas.rs:1410-1438).  Only the triangle (and the diagonal unless `unit`) is read.
+__global__ __launch_bounds__(256) void trsm_diag(int nb, int nvec, const double* __restrict__ T, size_t t_rs,
+                                                 size_t t_cs, double* __restrict__ X, size_t x_es, size_t x_vs,
+                                                 int forward, int unit, const int* __restrict__ info) {
     if (info && *info != 0) return;
     __shared__ double Ts[DB][DB + 1];
     const int t = threadIdx.x;
-    // stage the triangle; pad to 64 with the identity so the solve below is branch-free
-    for (int idx = t; idx < DB * DB; idx += DT) {
+    for (int idx = t; idx < DB * DB; idx += 256) {
         const int i = idx % DB, k = idx / DB;
         double v = 0.0;
         const bool in_tri = forward ? (k < i) : (k > i);
@@ -36,7 +36,7 @@ __global__ __launch_bounds__(DT) void trsm_diag(int nb, int nvec, const double* 
         Ts[i][k] = v;
     }
     __syncthreads();
-    const int vec = blockIdx.x * DT + t;
+    const int vec = blockIdx.x * 256 + t;
     if (vec >= nvec) return;
     double x[DB];
     double* xp = X + (size_t)vec * x_vs;
@@ -45,18 +45,18 @@ __global__ __launch_bounds__(DT) void trsm_diag(int nb, int nvec, const double* 
     if (forward) {
 #pragma unroll
         for (int i = 0; i < DB; ++i) {
-            double s = x[i];
+            double sacc = x[i];
 #pragma unroll
-            for (int k = 0; k < i; ++k) s -= Ts[i][k] * x[k];
-            x[i] = unit ? s : s / Ts[i][i];
+            for (int k = 0; k < i; ++k) sacc -= Ts[i][k] * x[k];
+            x[i] = unit ? sacc : sacc / Ts[i][i];
         }
     } else {
 #pragma unroll
         for (int i = DB - 1; i >= 0; --i) {
-            double s = x[i];
+            double sacc = x[i];
 #pragma unroll
-            for (int k = i + 1; k < DB; ++k) s -= Ts[i][k] * x[k];
-            x[i] = unit ? s : s / Ts[i][i];
+            for (int k = i + 1; k < DB; ++k) sacc -= Ts[i][k] * x[k];
+            x[i] = unit ? sacc : sacc / Ts[i][i];
         }
     }
 #pragma unroll
@@ -96,8 +96,8 @@ inline const double* opT_block(const double* T, size_t ldt, bool trans, size_t r
 int diag_solve(hipStream_t s, int nb, int nvec, const double* T, size_t t_rs, size_t t_cs, double* X, size_t x_es,
                size_t x_vs, bool forward, bool unit, const int* info) {
     if (nvec <= 0 || nb <= 0) return AL_OK;
-    hipLaunchKernelGGL(trsm_diag, dim3(ceil_div((size_t)nvec, DT)), dim3(DT), 0, s, nb, nvec, T, t_rs, t_cs, X, x_es,
-                       x_vs, forward ? 1 : 0, unit ? 1 : 0, info);
+    hipLaunchKernelGGL(trsm_diag, dim3(ceil_div((size_t)nvec, 256)), dim3(256), 0, s, nb, nvec, T, t_rs, t_cs, X,
+                       x_es, x_vs, forward ? 1 : 0, unit ? 1 : 0, info);
     AL_HIP(hipGetLastError());
     return AL_OK;
 }

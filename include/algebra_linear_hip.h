@@ -118,6 +118,9 @@ int al_dev_dgetrf(size_t m, size_t n, double* A, size_t lda, int32_t* d_ipiv, in
 int al_dev_dgetrs(size_t n, size_t nrhs, const double* LU, size_t lda, const int32_t* d_ipiv, double* B,
                   size_t ldb, void* stream);
 int al_dev_dpotrf(size_t n, double* A, size_t lda, int32_t* d_info, void* stream);
+/* Row interchanges d_ipiv[k0..k1) applied in order to the ncols columns of A (the crate's
+ * full-row swap, src/lapack/mod.rs:141-146; LAPACK's dlaswp with incx = 1). */
+int al_dev_dlaswp(size_t ncols, double* A, size_t lda, size_t k0, size_t k1, const int32_t* d_ipiv, void* stream);
 
 /* ------------------------------------------------------------------------------------
  * Kernel instrumentation (extension): while enabled, every launch of an instrumented

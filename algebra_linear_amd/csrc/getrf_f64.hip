@@ -44,33 +44,50 @@ __device__ __forceinline__ T pivot_key(T v, int g, int j) {
     return v < (T)0 ? -v : v;
 }
 
+// Order-preserving integer image of a pivot key: non-negative keys (|v| or +inf) keep their
+// IEEE order as unsigned bit patterns; the sentinels -1 (a skipped NaN) and -2 (no candidate)
+// sort below every real key.
+__device__ __forceinline__ unsigned long long key_code(double k) {
+    return k >= 0.0 ? (unsigned long long)__double_as_longlong(k) + 2ull : (k == -1.0 ? 1ull : 0ull);
+}
+__device__ __forceinline__ unsigned long long key_code(float k) {
+    return k >= 0.0f ? (unsigned long long)__float_as_uint(k) + 2ull : (k == -1.0f ? 1ull : 0ull);
+}
 template <typename T>
-__device__ __forceinline__ void block_argmax(T& key, int& idx, T* red_k, int* red_i) {
-    // wave64 butterfly, then across the 4 waves through LDS
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        const T ok = __shfl_xor(key, off, 64);
-        const int oi = __shfl_xor(idx, off, 64);
-        if (better(ok, oi, key, idx)) {
-            key = ok;
-            idx = oi;
-        }
-    }
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        red_k[w] = key;
-        red_i[w] = idx;
-    }
+__device__ __forceinline__ T key_value(unsigned long long c);
+template <>
+__device__ __forceinline__ double key_value<double>(unsigned long long c) {
+    return c >= 2ull ? __longlong_as_double((long long)(c - 2ull)) : (c == 1ull ? -1.0 : -2.0);
+}
+template <>
+__device__ __forceinline__ float key_value<float>(unsigned long long c) {
+    return c >= 2ull ? __uint_as_float((unsigned)(c - 2ull)) : (c == 1ull ? -1.0f : -2.0f);
+}
+
+// Workgroup-wide first maximum: the largest key, ties to the lowest row.  Two LDS atomics
+// (max of the key's integer image, then min of the row among the maximal keys) instead of a
+// shuffle tree.  `red` alternates between two slots per call so the next call's reset needs
+// no extra barrier (callers always pass a workgroup barrier between two calls); `call`
+// counts the calls.
+struct ArgmaxLds {
+    unsigned long long key[2];
+    int idx[2];
+};
+template <typename T>
+__device__ __forceinline__ void block_argmax(T& key, int& idx, ArgmaxLds* red, int& call) {
+    const int s = call & 1;
+    const unsigned long long code = key_code(key);
+    atomicMax(&red->key[s], code);
     __syncthreads();
-    key = red_k[0];
-    idx = red_i[0];
-#pragma unroll
-    for (int q = 1; q < PT / 64; ++q)
-        if (better(red_k[q], red_i[q], key, idx)) {
-            key = red_k[q];
-            idx = red_i[q];
-        }
+    if (code == red->key[s]) atomicMin(&red->idx[s], idx);
     __syncthreads();
+    key = key_value<T>(red->key[s]);
+    idx = red->idx[s];
+    if (threadIdx.x == 0) {  // the other slot is no longer read by anyone: reset it
+        red->key[s ^ 1] = 0ull;
+        red->idx[s ^ 1] = 0x7fffffff;
+    }
+    ++call;
 }
 
 #pragma clang fp contract(off)
@@ -101,8 +118,7 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
     __shared__ T lslot[2][SLOT2];  // single-workgroup exchange
     __shared__ T prow_c[PNB];
     __shared__ T rj_c[PNB];
-    __shared__ T red_k[PT / 64];
-    __shared__ int red_i[PT / 64];
+    __shared__ ArgmaxLds red;
     __shared__ int win_lds;
     __shared__ T wkey_lds;
     __shared__ int ok_lds;
@@ -115,6 +131,11 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
     const int nr = (m - rb) < chunk ? ((m - rb) > 0 ? (m - rb) : 0) : chunk;
     const bool has = t < nr;  // this thread's row: panel row rb + t
     const int g = rb + t;
+    int calls = 0;
+    if (t == 0) {
+        red.key[0] = red.key[1] = 0ull;
+        red.idx[0] = red.idx[1] = 0x7fffffff;
+    }
 
     for (int c = 0; c < ncols; ++c)
         if (has) S[c * chunk + t] = P[(size_t)g + (size_t)c * lda];
@@ -227,7 +248,7 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
     {
         T key = has ? pivot_key(S[t], g, 0) : (T)-2;
         int idx = has ? g : 0x7fffffff;
-        block_argmax(key, idx, red_k, red_i);
+        block_argmax(key, idx, &red, calls);
         publish(0, key, idx);
     }
 
@@ -268,7 +289,7 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
                 key = pivot_key(v, g, j + 1);
                 idx = g;
             }
-            block_argmax(key, idx, red_k, red_i);
+            block_argmax(key, idx, &red, calls);
             // bring the candidate's row and row j+1 up to date, publish, then finish the rest
             const int lc = (key > (T)-2) ? idx - rb : -1;
             const int ln = (j + 1 >= rb && j + 1 < rb + nr) ? j + 1 - rb : -1;

@@ -17,6 +17,22 @@ AL_OK, AL_SINGULAR, AL_BAD_SHAPE, AL_CONVERGENCE = 0, 1, 2, 3
 AL_ERR_DEVICE, AL_ERR_NO_DEVICE, AL_ERR_OOM, AL_ERR_ARG = -1, -2, -3, -4
 
 _dp = ctypes.POINTER(ctypes.c_double)
+
+
+class IrOptions(ctypes.Structure):
+    """``al_ir_options``: refinement controls of the mixed-precision solve."""
+    _fields_ = [("max_iter", ctypes.c_int), ("allow_fallback", ctypes.c_int), ("tol", ctypes.c_double),
+                ("stop_tol", ctypes.c_double)]
+
+
+class IrStats(ctypes.Structure):
+    """``al_ir_stats``: what the mixed-precision solve did."""
+    _fields_ = [("iterations", ctypes.c_int), ("fell_back", ctypes.c_int), ("factor_breakdown", ctypes.c_int),
+                ("converged", ctypes.c_int), ("backward_error", ctypes.c_double),
+                ("backward_error_low", ctypes.c_double)]
+
+
+AL_FACTOR_FP64, AL_FACTOR_FP32, AL_FACTOR_BF16 = 0, 1, 2
 _sp = ctypes.POINTER(ctypes.c_size_t)
 _ip = ctypes.POINTER(ctypes.c_int32)
 _d, _sz, _i, _v = ctypes.c_double, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p
@@ -45,6 +61,12 @@ SIGNATURES = {
     "al_dev_dgetrs": (_i, [_sz, _sz, _v, _sz, _v, _v, _sz, _v]),
     "al_dev_dpotrf": (_i, [_sz, _v, _sz, _v, _v]),
     "al_dev_dlaswp": (_i, [_sz, _v, _sz, _sz, _sz, _v, _v]),
+    "al_dev_sgemm": (_i, [_i, _i, _sz, _sz, _sz, ctypes.c_float, _v, _sz, _v, _sz, ctypes.c_float, _v, _sz, _v]),
+    "al_dev_gemm_bf16": (_i, [_sz, _sz, _sz, ctypes.c_float, _v, _sz, _v, _sz, ctypes.c_float, _v, _sz, _v]),
+    "al_ir_default_options": (None, [ctypes.POINTER(IrOptions)]),
+    "al_dgesv_mixed": (_i, [_dp, _sz, _dp, _sz, _sz, _sz, _i, ctypes.POINTER(IrOptions), ctypes.POINTER(IrStats)]),
+    "al_dev_dgesv_mixed": (_i, [_sz, _sz, _v, _sz, _v, _sz, _v, _sz, _i, ctypes.POINTER(IrOptions),
+                                ctypes.POINTER(IrStats), _v]),
     "al_prof_enable": (None, [_i]),
     "al_prof_reset": (None, []),
     "al_prof_query": (_i, [ctypes.c_char_p, ctypes.POINTER(_i), ctypes.POINTER(_d), ctypes.POINTER(_d),

@@ -37,6 +37,32 @@ int fail_hip(hipError_t e, const char* what);
 inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
 // ---------------------------------------------------------------------------------------
+// device helpers shared by the tiled kernels
+// ---------------------------------------------------------------------------------------
+
+// Tile (tm, tn) of this workgroup in a tiles_m x tiles_n grid of output tiles.  Workgroups
+// are dispatched round-robin over the 8 XCDs; each XCD is given a contiguous range of tiles,
+// walked in groups of 8 tile-rows (column-major inside a group), so the operand panels a
+// group shares stay in that XCD's L2.
+__device__ inline void tile_coords(int tiles_m, int tiles_n, int& tm, int& tn) {
+    const int nb = tiles_m * tiles_n;
+    const int bid = blockIdx.x;
+    // blocks are dispatched round-robin over the 8 XCDs; give XCD x a contiguous range
+    const int x = bid & 7, slot = bid >> 3;
+    const int base = nb >> 3, rem = nb & 7;
+    const int lin = x * base + (x < rem ? x : rem) + slot;
+    // grouped ordering: 8 tile-rows per group, column-major inside the group
+    constexpr int G = 8;
+    const int per_group = G * tiles_n;
+    const int g = lin / per_group;
+    const int first_m = g * G;
+    const int gsize = (tiles_m - first_m) < G ? (tiles_m - first_m) : G;
+    const int in = lin - g * per_group;
+    tm = first_m + in % gsize;
+    tn = in / gsize;
+}
+
+// ---------------------------------------------------------------------------------------
 // per-call device workspace: the cooperative panel kernels' exchange slots, counters and
 // the status word every kernel of a factorization checks before doing work.
 // ---------------------------------------------------------------------------------------
@@ -60,6 +86,14 @@ struct Workspace {
     unsigned* trsv_flags = nullptr;  // per-block completion flags of the one-launch solves
     size_t trsv_flags_len = 0;
     unsigned trsv_gen = 0;         // last generation used on trsv_flags (host side)
+    void* mbuf = nullptr;          // mixed-precision solver arena (factor, vectors, partial sums)
+    size_t mbuf_bytes = 0;
+    void* fbuf = nullptr;          // its FP64 fallback: a copy of A and the pivots
+    size_t fbuf_bytes = 0;
+    void* hnorms = nullptr;        // pinned host words the refinement loop reads back
+    unsigned mgen = 0;             // last generation of the no-pivot panel flag (host side)
+    void* gbuf = nullptr;          // bf16 operand staging of the device-API bf16 GEMM
+    size_t gbuf_bytes = 0;
 };
 
 // The workspace of `stream` for the calling thread (created on first use).
@@ -92,6 +126,9 @@ int thread_stream(hipStream_t* out);
 // C := alpha*op(A)*op(B) + beta*C; tri: 0 full, 1 lower triangle only, 2 upper only.
 int gemm_f64(hipStream_t s, bool ta, bool tb, size_t m, size_t n, size_t k, double alpha, const double* A,
              size_t lda, const double* B, size_t ldb, double beta, double* C, size_t ldc, int tri = 0);
+// The same on FP32 matrix cores (v_mfma_f32_16x16x4_f32), FP32 accumulation.
+int gemm_f32(hipStream_t s, bool ta, bool tb, size_t m, size_t n, size_t k, float alpha, const float* A, size_t lda,
+             const float* B, size_t ldb, float beta, float* C, size_t ldc, int tri = 0);
 
 // Triangular solve op(T) X = alpha B (left) or X op(T) = alpha B (right), in place in B.
 // trans selects op(T) = T^T (used by the Cholesky panel); `info` (nullable) turns the call
@@ -104,6 +141,9 @@ int trsm_f64(hipStream_t s, bool left, bool lower, bool trans, bool unit, size_t
 // the row-origin vector perm -- with the result of the sequential swaps, as one gather.
 int swap_block_f64(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, size_t jb, double* A0, size_t n0,
                    double* A1, size_t n1, size_t lda, double* perm, const int* info);
+// The same for an FP32 matrix (no perm vector).
+int swap_block_f32(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, size_t jb, float* A0, size_t n0,
+                   float* A1, size_t n1, size_t lda, const int* info);
 int perm_init(hipStream_t s, double* perm, size_t n);                          // perm[i] = i
 int perm_from_ipiv(hipStream_t s, const int* ipiv, size_t n, double* perm);    // compose all pivots
 int gather_rows(hipStream_t s, const double* B, size_t ldb, size_t n, size_t nrhs, const double* perm, double* W);
@@ -114,10 +154,29 @@ int copy_block(hipStream_t s, const double* W, size_t n, size_t nrhs, double* B,
 int getrf_f64(hipStream_t s, Workspace* ws, size_t m, size_t n, double* A, size_t lda, int* ipiv, int* info,
               double* perm = nullptr);
 
+// One cooperative 64-wide (ncols <= 64) partial-pivoting panel of an FP32 matrix: the m x
+// ncols block at P (rows row0.. of the whole matrix), pivots written to ipiv[row0..] as
+// global row indices, swaps applied inside the panel only.
+int panel_f32(hipStream_t s, Workspace* ws, float* P, size_t lda, size_t row0, size_t m, size_t ncols, int* ipiv,
+              int* info);
+
 // Solve with a getrf factorization: B (n x nrhs) := A^{-1} B.  perm (nullable) is getrf's row
 // origin vector; without it the pivots are composed first.
 int getrs_f64(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const double* LU, size_t lda, const int* ipiv,
               const double* perm, double* B, size_t ldb, const int* info = nullptr);
+
+// bf16 staging of a GEMM's operands, rows of kp >= k entries (zero beyond k), rounded to
+// nearest even: At[r][p] = L[r + p*ld] (L m x k) and Bt[c][p] = U[p + c*ld] (U k x n).
+int bf16_rows(hipStream_t s, size_t m, size_t k, size_t kp, const float* L, size_t ld, __bf16* At);
+int bf16_cols(hipStream_t s, size_t k, size_t kp, size_t n, const float* U, size_t ld, __bf16* Bt);
+// C (m x n, FP32) := alpha sum_p At[i][p] Bt[j][p] + beta C, bf16 products accumulated in
+// FP32 on the matrix cores; k % 64 == 0.
+int gemm_bf16(hipStream_t s, size_t m, size_t n, size_t k, float alpha, const __bf16* At, const __bf16* Bt, float beta,
+              float* C, size_t ldc);
+
+// Mixed-precision solve X := A^{-1} B (A, B untouched; synchronizes s).
+int gesv_mixed(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const double* A, size_t lda, const double* B,
+               size_t ldb, double* X, size_t ldx, int factor, const al_ir_options* opt, al_ir_stats* stats);
 
 // Lower Cholesky in place; only the lower triangle is read or written.
 int potrf_f64(hipStream_t s, Workspace* ws, size_t n, double* A, size_t lda, int* info);

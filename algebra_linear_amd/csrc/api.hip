@@ -43,6 +43,10 @@ struct ThreadCtx {
             if (w.perm) (void)hipFree(w.perm);
             if (w.wbuf) (void)hipFree(w.wbuf);
             if (w.trsv_flags) (void)hipFree(w.trsv_flags);
+            if (w.mbuf) (void)hipFree(w.mbuf);
+            if (w.fbuf) (void)hipFree(w.fbuf);
+            if (w.hnorms) (void)hipHostFree(w.hnorms);
+            if (w.gbuf) (void)hipFree(w.gbuf);
         }
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -451,6 +455,31 @@ int al_dpotrf(double* a, size_t a_len, size_t n) {
     return status_from_info(info);
 }
 
+int al_dgesv_mixed(const double* a, size_t a_len, double* b, size_t b_len, size_t n, size_t nrhs, int factor,
+                   const al_ir_options* opt, al_ir_stats* stats) {
+    if (!len_is(a_len, n, n)) return bad_shape("dgesv_mixed: a must be n*n");
+    if (!len_is(b_len, n, nrhs)) return bad_shape("dgesv_mixed: b must be n*nrhs");
+    if (n == 0 || nrhs == 0) {
+        if (stats) *stats = al_ir_stats{0, 0, 0, 1, 0.0, 0.0};
+        return AL_OK;
+    }
+    hipStream_t s;
+    AL_CHECK(thread_stream(&s));
+    Workspace* ws;
+    AL_CHECK(workspace_for(s, &ws, 1));
+    double *dA, *dB;
+    void* dX;
+    AL_CHECK(upload(0, a, a_len, &dA, s));
+    AL_CHECK(upload(1, b, b_len, &dB, s));
+    AL_CHECK(buffer(2, b_len * sizeof(double), &dX));
+    const int st = gesv_mixed(s, ws, n, nrhs, dA, n, dB, n, (double*)dX, n, factor, opt, stats);
+    if (st == AL_OK || st == AL_CONVERGENCE) {
+        AL_CHECK(download(b, (double*)dX, b_len, s));
+        AL_CHECK(finish(s));
+    }
+    return st;
+}
+
 // ------------------------------------------------------------------------------------------
 // device-pointer API
 // ------------------------------------------------------------------------------------------
@@ -516,6 +545,49 @@ int al_dev_dpotrf(size_t n, double* A, size_t lda, int32_t* d_info, void* stream
     Workspace* ws;
     AL_CHECK(workspace_for(s, &ws, 1));
     return potrf_f64(s, ws, n, A, lda, d_info);
+}
+
+int al_dev_sgemm(int trans_a, int trans_b, size_t m, size_t n, size_t k, float alpha, const float* A, size_t lda,
+                 const float* B, size_t ldb, float beta, float* C, size_t ldc, void* stream) {
+    if (ldc < m || lda < (trans_a ? k : m) || ldb < (trans_b ? n : k)) return bad_shape("sgemm: leading dimension too small");
+    hipStream_t s;
+    AL_CHECK(dev_stream(stream, &s));
+    return gemm_f32(s, trans_a != 0, trans_b != 0, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+int al_dev_gemm_bf16(size_t m, size_t n, size_t k, float alpha, const float* A, size_t lda, const float* B, size_t ldb,
+                     float beta, float* C, size_t ldc, void* stream) {
+    if (ldc < m || lda < m || ldb < k) return bad_shape("gemm_bf16: leading dimension too small");
+    hipStream_t s;
+    AL_CHECK(dev_stream(stream, &s));
+    if (m == 0 || n == 0) return AL_OK;
+    Workspace* ws;
+    AL_CHECK(workspace_for(s, &ws, 1));
+    const size_t kp = (k + 63) / 64 * 64;
+    if (kp == 0) return gemm_f32(s, false, false, m, n, 0, alpha, A, lda, B, ldb, beta, C, ldc);  // C := beta C
+    const size_t abytes = (m * kp * 2 + 255) / 256 * 256, bytes = abytes + n * kp * 2;
+    if (ws->gbuf_bytes < bytes) {
+        if (ws->gbuf) AL_HIP(hipFree(ws->gbuf));
+        ws->gbuf = nullptr;
+        ws->gbuf_bytes = 0;
+        AL_HIP(hipMalloc(&ws->gbuf, bytes));
+        ws->gbuf_bytes = bytes;
+    }
+    __bf16* At = (__bf16*)ws->gbuf;
+    __bf16* Bt = (__bf16*)((char*)ws->gbuf + abytes);
+    AL_CHECK(bf16_rows(s, m, k, kp, A, lda, At));
+    AL_CHECK(bf16_cols(s, k, kp, n, B, ldb, Bt));
+    return gemm_bf16(s, m, n, kp, alpha, At, Bt, beta, C, ldc);
+}
+
+int al_dev_dgesv_mixed(size_t n, size_t nrhs, const double* A, size_t lda, const double* B, size_t ldb, double* X,
+                       size_t ldx, int factor, const al_ir_options* opt, al_ir_stats* stats, void* stream) {
+    if (lda < n || ldb < n || ldx < n) return bad_shape("dgesv_mixed: leading dimension too small");
+    hipStream_t s;
+    AL_CHECK(dev_stream(stream, &s));
+    Workspace* ws;
+    AL_CHECK(workspace_for(s, &ws, 1));
+    return gesv_mixed(s, ws, n, nrhs, A, lda, B, ldb, X, ldx, factor, opt, stats);
 }
 
 int al_dev_dlaswp(size_t ncols, double* A, size_t lda, size_t k0, size_t k1, const int32_t* d_ipiv, void* stream) {

@@ -23,12 +23,36 @@ namespace al {
 namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// Per-element-type MFMA: FP64 uses v_mfma_f64_16x16x4_f64, FP32 v_mfma_f32_16x16x4_f32.
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+    typedef d4 acc_t;
+    static __device__ __forceinline__ acc_t op(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // accumulator r of lane l holds D[(l >> 4) + 4r][l & 15]
+    static __device__ __forceinline__ int drow(int kq, int r) { return kq + 4 * r; }
+};
+template <>
+struct Mfma<float> {
+    typedef f4 acc_t;
+    static __device__ __forceinline__ acc_t op(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    // accumulator r of lane l holds D[4 (l >> 4) + r][l & 15]
+    static __device__ __forceinline__ int drow(int kq, int r) { return 4 * kq + r; }
+};
 
 constexpr int BM = 128, BN = 128, BK = 16;
 constexpr int NT = 256;
 constexpr int LDS_ROW = BM + 1;  // one-double pad breaks the strided stores' bank conflicts
 constexpr int TILE_DOUBLES = BK * LDS_ROW;
-constexpr size_t LDS_BYTES = 4 * TILE_DOUBLES * sizeof(double);  // A and B, double-buffered
+template <typename T>
+constexpr size_t lds_bytes() { return 4 * TILE_DOUBLES * sizeof(T); }  // A and B, double-buffered
 
 // Loads the 128 x BK operand tile whose element (r, p) is base[r + p*ld] (ROWCONTIG false:
 // the tile's columns are contiguous, e.g. a non-transposed A) or base[p + r*ld] (true: the
@@ -36,16 +60,16 @@ constexpr size_t LDS_BYTES = 4 * TILE_DOUBLES * sizeof(double);  // A and B, dou
 // outside [0, r_lim) x [0, kdim).
 constexpr int LPT = BM * BK / NT;  // operand elements per thread per tile
 
-template <bool ROWCONTIG>
-__device__ __forceinline__ void load_tile(const double* __restrict__ base, size_t ld, int r0, int r_lim, int k0,
-                                          int kdim, double (&reg)[LPT], int t) {
+template <bool ROWCONTIG, typename T>
+__device__ __forceinline__ void load_tile(const T* __restrict__ base, size_t ld, int r0, int r_lim, int k0,
+                                          int kdim, T (&reg)[LPT], int t) {
     if (!ROWCONTIG) {
         const int r = t & 127, p0 = t >> 7;
         const bool rok = (r0 + r) < r_lim;
 #pragma unroll
         for (int j = 0; j < LPT; ++j) {
             const int p = k0 + p0 + 2 * j;
-            reg[j] = (rok && p < kdim) ? base[(size_t)(r0 + r) + (size_t)p * ld] : 0.0;
+            reg[j] = (rok && p < kdim) ? base[(size_t)(r0 + r) + (size_t)p * ld] : (T)0;
         }
     } else {
         const int p = t % BK, rr = t / BK;
@@ -53,13 +77,13 @@ __device__ __forceinline__ void load_tile(const double* __restrict__ base, size_
 #pragma unroll
         for (int j = 0; j < LPT; ++j) {
             const int r = r0 + rr + (NT / BK) * j;
-            reg[j] = (pok && r < r_lim) ? base[(size_t)(k0 + p) + (size_t)r * ld] : 0.0;
+            reg[j] = (pok && r < r_lim) ? base[(size_t)(k0 + p) + (size_t)r * ld] : (T)0;
         }
     }
 }
 
-template <bool ROWCONTIG>
-__device__ __forceinline__ void store_tile(double* __restrict__ S, const double (&reg)[LPT], int t) {
+template <bool ROWCONTIG, typename T>
+__device__ __forceinline__ void store_tile(T* __restrict__ S, const T (&reg)[LPT], int t) {
     if (!ROWCONTIG) {
         const int r = t & 127, p0 = t >> 7;
 #pragma unroll
@@ -71,33 +95,17 @@ __device__ __forceinline__ void store_tile(double* __restrict__ S, const double 
     }
 }
 
-__device__ __forceinline__ void tile_coords(int tiles_m, int tiles_n, int& tm, int& tn) {
-    const int nb = tiles_m * tiles_n;
-    const int bid = blockIdx.x;
-    // blocks are dispatched round-robin over the 8 XCDs; give XCD x a contiguous range
-    const int x = bid & 7, slot = bid >> 3;
-    const int base = nb >> 3, rem = nb & 7;
-    const int lin = x * base + (x < rem ? x : rem) + slot;
-    // grouped ordering: 8 tile-rows per group, column-major inside the group
-    constexpr int G = 8;
-    const int per_group = G * tiles_n;
-    const int g = lin / per_group;
-    const int first_m = g * G;
-    const int gsize = (tiles_m - first_m) < G ? (tiles_m - first_m) : G;
-    const int in = lin - g * per_group;
-    tm = first_m + in % gsize;
-    tn = in / gsize;
-}
-
-template <bool TA, bool TB, int TRI>
-__global__ __launch_bounds__(NT, 2) void dgemm_mfma(int m, int n, int k, double alpha, const double* __restrict__ A,
-                                                 size_t lda, const double* __restrict__ B, size_t ldb, double beta,
-                                                 double* __restrict__ C, size_t ldc, int tiles_m, int tiles_n) {
-    extern __shared__ double lds[];
-    double* As0 = lds;
-    double* As1 = lds + TILE_DOUBLES;
-    double* Bs0 = lds + 2 * TILE_DOUBLES;
-    double* Bs1 = lds + 3 * TILE_DOUBLES;
+template <typename T, bool TA, bool TB, int TRI>
+__global__ __launch_bounds__(NT, 2) void dgemm_mfma(int m, int n, int k, T alpha, const T* __restrict__ A,
+                                                 size_t lda, const T* __restrict__ B, size_t ldb, T beta,
+                                                 T* __restrict__ C, size_t ldc, int tiles_m, int tiles_n) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    T* lds = (T*)lds_raw;
+    T* As0 = lds;
+    T* As1 = lds + TILE_DOUBLES;
+    T* Bs0 = lds + 2 * TILE_DOUBLES;
+    T* Bs1 = lds + 3 * TILE_DOUBLES;
+    typedef typename Mfma<T>::acc_t acc_t;
 
     int tm, tn;
     tile_coords(tiles_m, tiles_n, tm, tn);
@@ -115,13 +123,13 @@ __global__ __launch_bounds__(NT, 2) void dgemm_mfma(int m, int n, int k, double 
     constexpr bool A_RC = TA;
     constexpr bool B_RC = !TB;
 
-    d4 acc[4][4];
+    acc_t acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-    double ra[LPT], rb[LPT];
+    T ra[LPT], rb[LPT];
     const int ktiles = (k + BK - 1) / BK;
     load_tile<A_RC>(A, lda, bm, m, 0, k, ra, t);
     load_tile<B_RC>(B, ldb, bn, n, 0, k, rb, t);
@@ -136,12 +144,12 @@ __global__ __launch_bounds__(NT, 2) void dgemm_mfma(int m, int n, int k, double 
             load_tile<A_RC>(A, lda, bm, m, (kt + 1) * BK, k, ra, t);
             load_tile<B_RC>(B, ldb, bn, n, (kt + 1) * BK, k, rb, t);
         }
-        const double* As = (kt & 1) ? As1 : As0;
-        const double* Bs = (kt & 1) ? Bs1 : Bs0;
+        const T* As = (kt & 1) ? As1 : As0;
+        const T* Bs = (kt & 1) ? Bs1 : Bs0;
 #pragma unroll
         for (int s = 0; s < BK / 4; ++s) {
             const int kk = 4 * s + kq;
-            double af[4], bf[4];
+            T af[4], bf[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) af[i] = As[kk * LDS_ROW + wm * 64 + i * 16 + col];
 #pragma unroll
@@ -149,7 +157,7 @@ __global__ __launch_bounds__(NT, 2) void dgemm_mfma(int m, int n, int k, double 
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
+                for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::op(bf[j], af[i], acc[i][j]);
         }
         if (more) {
             store_tile<A_RC>((kt & 1) ? As0 : As1, ra, t);
@@ -158,9 +166,8 @@ __global__ __launch_bounds__(NT, 2) void dgemm_mfma(int m, int n, int k, double 
         __syncthreads();
     }
 
-    // v_mfma_f64_16x16x4_f64 leaves D[(lane >> 4) + 4*r][lane & 15] in accumulator r; with the
-    // operands fed as (B^T, A^T), D = C^T, so lane holds C[row][col], row = .. + (lane & 15),
-    // col = .. + (lane >> 4) + 4*r
+    // with the operands fed as (B^T, A^T), D = C^T: accumulator r of a lane holds C[row][col],
+    // row = .. + (lane & 15), col = .. + Mfma<T>::drow(lane >> 4, r)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int row = bm + wm * 64 + i * 16 + col;
@@ -169,13 +176,13 @@ __global__ __launch_bounds__(NT, 2) void dgemm_mfma(int m, int n, int k, double 
         for (int j = 0; j < 4; ++j) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int cc = bn + wn * 64 + j * 16 + kq + 4 * r;
+                const int cc = bn + wn * 64 + j * 16 + Mfma<T>::drow(kq, r);
                 if (cc >= n) continue;
                 if (TRI == 1 && row < cc) continue;
                 if (TRI == 2 && row > cc) continue;
-                double* cp = C + (size_t)row + (size_t)cc * ldc;
-                double v = alpha * acc[i][j][r];
-                if (beta != 0.0) v += beta * *cp;
+                T* cp = C + (size_t)row + (size_t)cc * ldc;
+                T v = alpha * acc[i][j][r];
+                if (beta != (T)0) v += beta * *cp;
                 *cp = v;
             }
         }
@@ -183,21 +190,23 @@ __global__ __launch_bounds__(NT, 2) void dgemm_mfma(int m, int n, int k, double 
 }
 
 // C := beta*C (0 when beta == 0) -- the crate's k == 0 / alpha == 0 path (src/blas.rs:297-308).
-__global__ void scale_c(int m, int n, double beta, double* C, size_t ldc, int tri) {
+template <typename T>
+__global__ void scale_c(int m, int n, T beta, T* C, size_t ldc, int tri) {
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (size_t)m * n) return;
     const int i = (int)(idx % m), j = (int)(idx / m);
     if (tri == 1 && i < j) return;
     if (tri == 2 && i > j) return;
-    double* cp = C + (size_t)i + (size_t)j * ldc;
-    *cp = (beta == 0.0) ? 0.0 : beta * *cp;
+    T* cp = C + (size_t)i + (size_t)j * ldc;
+    *cp = (beta == (T)0) ? (T)0 : beta * *cp;
 }
 
-template <bool TA, bool TB, int TRI>
-int launch(hipStream_t s, int m, int n, int k, double alpha, const double* A, size_t lda, const double* B,
-           size_t ldb, double beta, double* C, size_t ldc) {
+template <typename T, bool TA, bool TB, int TRI>
+int launch(hipStream_t s, int m, int n, int k, T alpha, const T* A, size_t lda, const T* B, size_t ldb, T beta, T* C,
+           size_t ldc) {
     static bool attr_set = false;  // benign race: every thread sets the same value
-    auto fn = dgemm_mfma<TA, TB, TRI>;
+    auto fn = dgemm_mfma<T, TA, TB, TRI>;
+    constexpr size_t LDS_BYTES = lds_bytes<T>();
     if (!attr_set) {
         AL_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_BYTES));
         attr_set = true;
@@ -214,44 +223,60 @@ int launch(hipStream_t s, int m, int n, int k, double alpha, const double* A, si
         const double cfrac = TRI ? (dn + 1.0) / (2.0 * dn) : 1.0;
         const double fl = 2.0 * dm * dn * dk * cfrac;
         const double by = 8.0 * (dm * dk + dk * dn + dm * dn * cfrac * (beta != 0.0 ? 2.0 : 1.0));
-        static const char* names[3][4] = {{"dgemm_nn", "dgemm_tn", "dgemm_nt", "dgemm_tt"},
-                                          {"dsyrk_lower", "dsyrk_lower", "dsyrk_lower", "dsyrk_lower"},
-                                          {"dsyrk_upper", "dsyrk_upper", "dsyrk_upper", "dsyrk_upper"}};
-        prof_end(ticket, s, names[TRI][(TA ? 1 : 0) + (TB ? 2 : 0)], fl, by);
+        static const char* names[2][3][4] = {
+            {{"dgemm_nn", "dgemm_tn", "dgemm_nt", "dgemm_tt"},
+             {"dsyrk_lower", "dsyrk_lower", "dsyrk_lower", "dsyrk_lower"},
+             {"dsyrk_upper", "dsyrk_upper", "dsyrk_upper", "dsyrk_upper"}},
+            {{"sgemm_nn", "sgemm_tn", "sgemm_nt", "sgemm_tt"},
+             {"ssyrk_lower", "ssyrk_lower", "ssyrk_lower", "ssyrk_lower"},
+             {"ssyrk_upper", "ssyrk_upper", "ssyrk_upper", "ssyrk_upper"}}};
+        const double by2 = by * (double)sizeof(T) / 8.0;
+        prof_end(ticket, s, names[sizeof(T) == 4 ? 1 : 0][TRI][(TA ? 1 : 0) + (TB ? 2 : 0)], fl, by2);
     }
     return AL_OK;
 }
 
-template <int TRI>
-int dispatch(hipStream_t s, bool ta, bool tb, int m, int n, int k, double alpha, const double* A, size_t lda,
-             const double* B, size_t ldb, double beta, double* C, size_t ldc) {
-    if (!ta && !tb) return launch<false, false, TRI>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
-    if (ta && !tb) return launch<true, false, TRI>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
-    if (!ta && tb) return launch<false, true, TRI>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
-    return launch<true, true, TRI>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+template <typename T, int TRI>
+int dispatch(hipStream_t s, bool ta, bool tb, int m, int n, int k, T alpha, const T* A, size_t lda, const T* B,
+             size_t ldb, T beta, T* C, size_t ldc) {
+    if (!ta && !tb) return launch<T, false, false, TRI>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    if (ta && !tb) return launch<T, true, false, TRI>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    if (!ta && tb) return launch<T, false, true, TRI>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    return launch<T, true, true, TRI>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+template <typename T>
+int gemm_t(hipStream_t s, bool ta, bool tb, size_t m, size_t n, size_t k, T alpha, const T* A, size_t lda, const T* B,
+           size_t ldb, T beta, T* C, size_t ldc, int tri) {
+    if (m == 0 || n == 0) return AL_OK;
+    if (m > INT32_MAX || n > INT32_MAX || k > INT32_MAX) {
+        set_error("gemm: dimensions beyond 2^31 are not supported");
+        return AL_ERR_ARG;
+    }
+    if (k == 0 || alpha == (T)0) {
+        if (beta == (T)1) return AL_OK;
+        const size_t total = m * n;
+        hipLaunchKernelGGL(scale_c<T>, dim3(ceil_div(total, 256)), dim3(256), 0, s, (int)m, (int)n, beta, C, ldc, tri);
+        AL_HIP(hipGetLastError());
+        return AL_OK;
+    }
+    switch (tri) {
+        case 1: return dispatch<T, 1>(s, ta, tb, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc);
+        case 2: return dispatch<T, 2>(s, ta, tb, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc);
+        default: return dispatch<T, 0>(s, ta, tb, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc);
+    }
 }
 
 }  // namespace
 
 int gemm_f64(hipStream_t s, bool ta, bool tb, size_t m, size_t n, size_t k, double alpha, const double* A,
              size_t lda, const double* B, size_t ldb, double beta, double* C, size_t ldc, int tri) {
-    if (m == 0 || n == 0) return AL_OK;
-    if (m > INT32_MAX || n > INT32_MAX || k > INT32_MAX) {
-        set_error("gemm: dimensions beyond 2^31 are not supported");
-        return AL_ERR_ARG;
-    }
-    if (k == 0 || alpha == 0.0) {
-        if (beta == 1.0) return AL_OK;
-        const size_t total = m * n;
-        hipLaunchKernelGGL(scale_c, dim3(ceil_div(total, 256)), dim3(256), 0, s, (int)m, (int)n, beta, C, ldc, tri);
-        AL_HIP(hipGetLastError());
-        return AL_OK;
-    }
-    switch (tri) {
-        case 1: return dispatch<1>(s, ta, tb, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc);
-        case 2: return dispatch<2>(s, ta, tb, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc);
-        default: return dispatch<0>(s, ta, tb, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc);
-    }
+    return gemm_t<double>(s, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri);
+}
+
+int gemm_f32(hipStream_t s, bool ta, bool tb, size_t m, size_t n, size_t k, float alpha, const float* A, size_t lda,
+             const float* B, size_t ldb, float beta, float* C, size_t ldc, int tri) {
+    return gemm_t<float>(s, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri);
 }
 
 }  // namespace al

@@ -390,6 +390,11 @@ int panel_blocked(hipStream_t s, Workspace* ws, double* A, size_t lda, size_t M,
 
 }  // namespace
 
+int panel_f32(hipStream_t s, Workspace* ws, float* P, size_t lda, size_t row0, size_t m, size_t ncols, int* ipiv,
+              int* info) {
+    return launch_panel<float>(s, ws, P, lda, row0, m, ncols, ipiv, info);
+}
+
 int getrf_f64(hipStream_t s, Workspace* ws, size_t M, size_t N, double* A, size_t lda, int* ipiv, int* info,
               double* perm) {
     const size_t kmax = M < N ? M : N;

@@ -1,0 +1,614 @@
+// Mixed-precision solve of A X = B: partial-pivoting LU in low precision, FP64 iterative
+// refinement, FP64 pivoted-LU fallback.  The crate solves in FP64 only (lapack::dgesv,
+// src/lapack/mod.rs:309-323); this is the GPU extension beside it, with the same result
+// contract measured as normwise backward error.
+//
+// Pipeline (all on one stream):
+//   |A|_inf and |B_j|_inf   row sums / column maxima, FP64
+//   F = fl32(A)             A rounded to FP32, padded to a multiple of 64 with |A|_inf/n * I
+//   P F = L U               blocked right-looking LU with partial pivoting in FP32: 64-wide
+//                           cooperative panels, 64-row U solves, FP32 MFMA updates inside a
+//                           256/512-wide outer block, and the outer trailing update on FP32
+//                           or bf16 matrix cores (FP32 accumulation)
+//   refinement              r = B - A X (FP64, A streamed once per step, deterministic
+//                           blocked sums), d = U^{-1} L^{-1} P r with the FP32 factors and
+//                           FP64 vectors, X += d
+//   fallback                FP64 pivoted LU (getrf_f64/getrs_f64) on a copy of A
+//
+// Partial pivoting is not optional here: without it (even after a random butterfly
+// transform) the LU of a dense random matrix has |L||U| / |A| of 1e4-1e5 at n = 1024, enough
+// to stall FP32 refinement at n = 16384 and to make a bf16 update useless.
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "al_internal.hpp"
+
+namespace al {
+namespace {
+
+constexpr int PB = 64;          // panel width and padding granule
+constexpr int GV_ROWS = 256;    // residual: rows per workgroup
+constexpr int GV_COLS = 256;    // residual: columns per partial sum
+constexpr int NR_MAX = 16;      // right-hand sides per refinement pass
+
+typedef unsigned long long u64;
+
+// ---------------------------------------------------------------------------------------
+// FP32 copy and permuted right-hand sides
+// ---------------------------------------------------------------------------------------
+
+// F (np x np, FP32, ld np) := fl32([A 0; 0 pad I]), pad = |A|_inf / n (1 if that is zero or
+// not finite).
+__global__ __launch_bounds__(256) void to_f32_pad_k(int n, int np, const double* __restrict__ A, size_t lda,
+                                                    const u64* __restrict__ norms, float* __restrict__ F) {
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (size_t)np * np) return;
+    const int i = (int)(idx % np), j = (int)(idx / np);
+    float v;
+    if (i < n && j < n) {
+        v = (float)A[(size_t)i + (size_t)j * lda];
+    } else {
+        const double anorm = __longlong_as_double((long long)norms[0]);
+        const double pad = (anorm > 0.0 && anorm <= DBL_MAX) ? anorm / n : 1.0;
+        v = i == j ? (float)pad : 0.f;
+    }
+    F[idx] = v;
+}
+
+// Y (np x nr, ld np) := P [R; 0]: Y[i] = R[perm[i]] (perm the row origins of the factorization).
+__global__ __launch_bounds__(256) void gather_perm_k(int n, int np, int nr, const double* __restrict__ perm,
+                                                     const double* __restrict__ R, size_t ldr, double* __restrict__ Y) {
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= np * nr) return;
+    const int i = idx % np, k = idx / np;
+    const int src = (int)perm[i];
+    Y[(size_t)i + (size_t)k * np] = src < n ? R[(size_t)src + (size_t)k * ldr] : 0.0;
+}
+
+// X (n x nr, ld ldx) := (accumulate ? X : 0) + Y[0:n).
+__global__ __launch_bounds__(256) void add_k(int n, int np, int nr, const double* __restrict__ Y,
+                                             double* __restrict__ X, size_t ldx, int accumulate) {
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= n * nr) return;
+    const int i = idx % n, k = idx / n;
+    double* p = X + (size_t)i + (size_t)k * ldx;
+    const double y = Y[(size_t)i + (size_t)k * np];
+    *p = accumulate ? *p + y : y;
+}
+
+// ---------------------------------------------------------------------------------------
+// FP64 residual and norms
+// ---------------------------------------------------------------------------------------
+
+// part[chunk][row][k] = sum over the chunk's 256 columns c of A[row, c] * X[c, k] (ABS:
+// |A[row, c]|, X unused).  Each thread owns one row and walks the chunk in order, so the
+// sums are reproducible; A is read once, coalesced down its columns.
+template <int NR, bool ABS>
+__global__ __launch_bounds__(256) void gemv_part_k(int n, int nr, const double* __restrict__ A, size_t lda,
+                                                   const double* __restrict__ X, size_t ldx,
+                                                   double* __restrict__ part) {
+    __shared__ double xs[ABS ? 1 : GV_COLS][NR];
+    const int t = threadIdx.x;
+    const int r = blockIdx.x * GV_ROWS + t;
+    const int c0 = blockIdx.y * GV_COLS;
+    const int cn = (n - c0) < GV_COLS ? (n - c0) : GV_COLS;
+    if (!ABS) {
+        for (int idx = t; idx < GV_COLS * NR; idx += 256) {
+            const int c = idx / NR, k = idx % NR;
+            xs[c][k] = (c < cn && k < nr) ? X[(size_t)(c0 + c) + (size_t)k * ldx] : 0.0;
+        }
+        __syncthreads();
+    }
+    double acc[NR];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) acc[k] = 0.0;
+    if (r >= n) return;
+    const double* ap = A + (size_t)r + (size_t)c0 * lda;
+    int c = 0;
+    for (; c + 8 <= cn; c += 8) {
+        double a[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] = ap[(size_t)(c + u) * lda];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (ABS) {
+                acc[0] += fabs(a[u]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < NR; ++k) acc[k] = fma(a[u], xs[c + u][k], acc[k]);
+            }
+        }
+    }
+    for (; c < cn; ++c) {
+        const double a = ap[(size_t)c * lda];
+        if (ABS) {
+            acc[0] += fabs(a);
+        } else {
+#pragma unroll
+            for (int k = 0; k < NR; ++k) acc[k] = fma(a, xs[c][k], acc[k]);
+        }
+    }
+    const int w = ABS ? 1 : nr;
+    double* out = part + ((size_t)blockIdx.y * n + r) * w;
+#pragma unroll
+    for (int k = 0; k < NR; ++k)
+        if (k < w) out[k] = acc[k];
+}
+
+__device__ __forceinline__ u64 abs_bits(double v) { return (u64)__double_as_longlong(fabs(v)); }
+
+// Max over the wave of non-negative doubles as their bit patterns (order-preserving; a NaN
+// sorts above +inf, so it survives the reduction).
+__device__ __forceinline__ u64 wave_max(u64 v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const u64 o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+// out[0] = max over rows of sum over the chunks of part (|A|_inf from the ABS partial sums).
+__global__ __launch_bounds__(256) void rowsum_max_k(int n, int nparts, const double* __restrict__ part,
+                                                    u64* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double s = 0.0;
+    if (i < n)
+        for (int p = 0; p < nparts; ++p) s += part[(size_t)p * n + i];
+    const u64 m = wave_max(abs_bits(s));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+// out[k] = max_i |B[i, k]|.
+__global__ __launch_bounds__(256) void colmax_k(int n, int nr, const double* __restrict__ B, size_t ldb,
+                                                u64* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    for (int k = 0; k < nr; ++k) {
+        const double v = i < n ? B[(size_t)i + (size_t)k * ldb] : 0.0;
+        const u64 m = wave_max(abs_bits(v));
+        if ((threadIdx.x & 63) == 0) atomicMax(&out[k], m);
+    }
+}
+
+// R = B - (sum of the partial products in chunk order); rmax[k] = max |R[:, k]|,
+// xmax[k] = max |X[:, k]|.
+__global__ __launch_bounds__(256) void resid_k(int n, int nr, int nparts, const double* __restrict__ part,
+                                               const double* __restrict__ B, size_t ldb,
+                                               const double* __restrict__ X, size_t ldx, double* __restrict__ R,
+                                               u64* __restrict__ rmax, u64* __restrict__ xmax) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    for (int k = 0; k < nr; ++k) {
+        double r = 0.0, x = 0.0;
+        if (i < n) {
+            double s = 0.0;
+            for (int p = 0; p < nparts; ++p) s += part[((size_t)p * n + i) * nr + k];
+            r = B[(size_t)i + (size_t)k * ldb] - s;
+            R[(size_t)i + (size_t)k * n] = r;
+            x = X[(size_t)i + (size_t)k * ldx];
+        }
+        const u64 rm = wave_max(abs_bits(r)), xm = wave_max(abs_bits(x));
+        if ((threadIdx.x & 63) == 0) {
+            atomicMax(&rmax[k], rm);
+            atomicMax(&xmax[k], xm);
+        }
+    }
+}
+
+__global__ void copy2d_k(int m, int n, const double* __restrict__ S, size_t lds, double* __restrict__ D, size_t ldd) {
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (size_t)m * n) return;
+    const int i = (int)(idx % m), j = (int)(idx / m);
+    D[(size_t)i + (size_t)j * ldd] = S[(size_t)i + (size_t)j * lds];
+}
+
+// ---------------------------------------------------------------------------------------
+// LU with partial pivoting, FP32
+// ---------------------------------------------------------------------------------------
+
+// X := L^{-1} X for X = F[J:J+64, c0:c1) and L the unit lower triangle of F[J:J+64, J:J+64].
+// One thread per column (64 values in registers), L broadcast from LDS.
+__global__ __launch_bounds__(256) void trsm_ll64_k(float* __restrict__ F, size_t ld, int J, int c0, int c1) {
+    __shared__ float Lc[PB][PB + 1];  // Lc[k][i] = L[i][k]
+    const int t = threadIdx.x;
+    for (int idx = t; idx < PB * PB; idx += 256) {
+        const int i = idx & 63, k = idx >> 6;
+        Lc[k][i] = (i > k) ? F[(size_t)(J + i) + (size_t)(J + k) * ld] : 0.f;
+    }
+    __syncthreads();
+    const int c = c0 + blockIdx.x * 256 + t;
+    if (c >= c1) return;
+    float* col = F + (size_t)J + (size_t)c * ld;
+    float x[PB];
+#pragma unroll
+    for (int v = 0; v < PB / 4; ++v) {
+        const float4 q = *(const float4*)(col + 4 * v);
+        x[4 * v] = q.x;
+        x[4 * v + 1] = q.y;
+        x[4 * v + 2] = q.z;
+        x[4 * v + 3] = q.w;
+    }
+#pragma unroll
+    for (int i = 0; i < PB; ++i)
+#pragma unroll
+        for (int k = i + 1; k < PB; ++k) x[k] = fmaf(-Lc[i][k], x[i], x[k]);
+#pragma unroll
+    for (int v = 0; v < PB / 4; ++v) *(float4*)(col + 4 * v) = float4{x[4 * v], x[4 * v + 1], x[4 * v + 2], x[4 * v + 3]};
+}
+
+// ---------------------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------------------
+
+struct Arena {
+    size_t n = 0, np = 0, nr = 0, nparts = 0;
+    float* F = nullptr;       // np x np factor
+    int* ipiv = nullptr;      // np pivots of the FP32 factorization
+    double* perm = nullptr;   // np row origins
+    double* R = nullptr;      // n x nr residual
+    double* Y = nullptr;      // np x nr permuted residual / correction
+    double* part = nullptr;   // nparts x n x nr partial sums
+    u64* norms = nullptr;     // [0] |A|_inf, [1..17) |B|, [17..33) |R|, [33..49) |X|
+    int* minfo = nullptr;     // [0] low-precision factorization status, [2] fallback status
+    __bf16* At = nullptr;     // bf16 staging of the outer block's L and U
+    __bf16* Bt = nullptr;
+};
+
+constexpr int NORM_A = 0, NORM_B = 1, NORM_R = 17, NORM_X = 33, NORM_WORDS = 64;
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+int grow(void** p, size_t* have, size_t want) {
+    if (*have >= want) return AL_OK;
+    if (*p) AL_HIP(hipFree(*p));
+    *p = nullptr;
+    *have = 0;
+    hipError_t e = hipMalloc(p, want);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        set_error("device allocation failed (mixed-precision workspace)");
+        return AL_ERR_OOM;
+    }
+    *have = want;
+    return AL_OK;
+}
+
+int make_arena(hipStream_t s, Workspace* ws, size_t n, size_t np, size_t nr, size_t nbo, bool lowp, bool bf16,
+               Arena* a) {
+    a->n = n;
+    a->np = np;
+    a->nr = nr;
+    a->nparts = ceil_div(n, GV_COLS);
+    size_t off = 0;
+    const size_t oF = off;
+    if (lowp) off += align256(np * np * sizeof(float));
+    const size_t oV = off;
+    if (lowp) off += align256(np * sizeof(int));
+    const size_t oW = off;
+    if (lowp) off += align256(np * sizeof(double));
+    const size_t oR = off;
+    off += align256(n * nr * sizeof(double));
+    const size_t oY = off;
+    off += align256(np * nr * sizeof(double));
+    const size_t oP = off;
+    off += align256(a->nparts * n * nr * sizeof(double));
+    const size_t oN = off;
+    off += align256(NORM_WORDS * sizeof(u64));
+    const size_t oI = off;
+    off += 256;
+    const size_t oA = off;
+    if (bf16) off += align256(np * nbo * 2);
+    const size_t oB = off;
+    if (bf16) off += align256(np * nbo * 2);
+    AL_CHECK(grow(&ws->mbuf, &ws->mbuf_bytes, off));
+    char* base = (char*)ws->mbuf;
+    a->F = (float*)(base + oF);
+    a->ipiv = (int*)(base + oV);
+    a->perm = (double*)(base + oW);
+    a->R = (double*)(base + oR);
+    a->Y = (double*)(base + oY);
+    a->part = (double*)(base + oP);
+    a->norms = (u64*)(base + oN);
+    a->minfo = (int*)(base + oI);
+    a->At = bf16 ? (__bf16*)(base + oA) : nullptr;
+    a->Bt = bf16 ? (__bf16*)(base + oB) : nullptr;
+    return AL_OK;
+}
+
+int trsm_ll64(hipStream_t s, Arena& a, int J, int c0, int c1) {
+    if (c1 <= c0) return AL_OK;
+    hipLaunchKernelGGL(trsm_ll64_k, dim3(ceil_div((size_t)(c1 - c0), 256)), dim3(256), 0, s, a.F, a.np, J, c0, c1);
+    AL_HIP(hipGetLastError());
+    return AL_OK;
+}
+
+// Blocked right-looking LU with partial pivoting of the np x np FP32 matrix a.F
+// (np % 64 == 0); pivots as global row indices in a.ipiv, status in a.minfo[0].
+int getrf_lowp(hipStream_t s, Workspace* ws, Arena& a, bool bf16) {
+    const int np = (int)a.np;
+    const int nbo = bf16 ? 512 : 256;
+    float* F = a.F;
+    int* info = a.minfo;
+    const size_t ld = a.np;
+    auto at = [&](int i, int j) { return F + (size_t)i + (size_t)j * ld; };
+    AL_CHECK(fill_i32(s, info, 1, 0));
+    AL_CHECK(fill_u32(s, ws->abort_flag, 1, 0u));
+    AL_CHECK(iota_i32(s, a.ipiv, a.np));
+    for (int K = 0; K < np; K += nbo) {
+        const int kb = (np - K) < nbo ? (np - K) : nbo;
+        const int KE = K + kb;
+        // the outer panel, 64 columns at a time; swaps inside the outer panel's columns
+        for (int J = K; J < KE; J += PB) {
+            AL_CHECK(panel_f32(s, ws, at(J, J), ld, J, np - J, PB, a.ipiv, info));
+            AL_CHECK(swap_block_f32(s, ws, a.ipiv, J, PB, at(0, K), J - K, at(0, J + PB), KE - J - PB, ld, info));
+            if (J + PB < KE) {
+                AL_CHECK(trsm_ll64(s, a, J, J + PB, KE));
+                AL_CHECK(gemm_f32(s, false, false, np - J - PB, KE - J - PB, PB, -1.f, at(J + PB, J), ld, at(J, J + PB),
+                                  ld, 1.f, at(J + PB, J + PB), ld));
+            }
+        }
+        // the outer panel's swaps on the columns left and right of it (at most 256 at a time)
+        for (int J = K; J < KE; J += 256) {
+            const int jb = (KE - J) < 256 ? (KE - J) : 256;
+            AL_CHECK(swap_block_f32(s, ws, a.ipiv, J, jb, at(0, 0), K, at(0, KE), np - KE, ld, info));
+        }
+        if (KE >= np) break;
+        // U12 := L11^{-1} A12, 64 rows at a time
+        for (int J = K; J < KE; J += PB) {
+            AL_CHECK(trsm_ll64(s, a, J, KE, np));
+            if (J + PB < KE)
+                AL_CHECK(gemm_f32(s, false, false, KE - J - PB, np - KE, PB, -1.f, at(J + PB, J), ld, at(J, KE), ld, 1.f,
+                                  at(J + PB, KE), ld));
+        }
+        // trailing update A22 -= L21 U12
+        const int m2 = np - KE;
+        if (bf16) {
+            AL_CHECK(bf16_rows(s, m2, kb, kb, at(KE, K), ld, a.At));
+            AL_CHECK(bf16_cols(s, kb, kb, m2, at(K, KE), ld, a.Bt));
+            AL_CHECK(gemm_bf16(s, m2, m2, kb, -1.f, a.At, a.Bt, 1.f, at(KE, KE), ld));
+        } else {
+            AL_CHECK(gemm_f32(s, false, false, m2, m2, kb, -1.f, at(KE, K), ld, at(K, KE), ld, 1.f, at(KE, KE), ld));
+        }
+    }
+    return perm_from_ipiv(s, a.ipiv, a.np, a.perm);
+}
+
+template <int NR>
+int launch_gemv(hipStream_t s, const Arena& a, size_t nr, const double* A, size_t lda, const double* X, size_t ldx) {
+    const dim3 grid(ceil_div(a.n, GV_ROWS), (unsigned)a.nparts);
+    hipLaunchKernelGGL((gemv_part_k<NR, false>), grid, dim3(256), 0, s, (int)a.n, (int)nr, A, lda, X, ldx, a.part);
+    AL_HIP(hipGetLastError());
+    return AL_OK;
+}
+
+// R = B - A X with the norms of R and X per column (into a.norms).
+int residual(hipStream_t s, const Arena& a, size_t nr, const double* A, size_t lda, const double* B, size_t ldb,
+             const double* X, size_t ldx) {
+    const int ticket = prof_begin(s);
+    if (nr <= 1)
+        AL_CHECK(launch_gemv<1>(s, a, nr, A, lda, X, ldx));
+    else if (nr <= 2)
+        AL_CHECK(launch_gemv<2>(s, a, nr, A, lda, X, ldx));
+    else if (nr <= 4)
+        AL_CHECK(launch_gemv<4>(s, a, nr, A, lda, X, ldx));
+    else if (nr <= 8)
+        AL_CHECK(launch_gemv<8>(s, a, nr, A, lda, X, ldx));
+    else
+        AL_CHECK(launch_gemv<16>(s, a, nr, A, lda, X, ldx));
+    if (ticket >= 0) {
+        const double dn = (double)a.n;
+        prof_end(ticket, s, "residual_gemv", 2.0 * dn * dn * nr, 8.0 * dn * dn + 8.0 * (double)a.nparts * dn * nr);
+    }
+    hipLaunchKernelGGL(resid_k, dim3(ceil_div(a.n, 256)), dim3(256), 0, s, (int)a.n, (int)nr, (int)a.nparts, a.part, B,
+                       ldb, X, ldx, a.R, a.norms + NORM_R, a.norms + NORM_X);
+    AL_HIP(hipGetLastError());
+    return AL_OK;
+}
+
+int read_norms(hipStream_t s, Workspace* ws, const Arena& a, u64* host) {
+    AL_HIP(hipMemcpyAsync(host, a.norms, NORM_WORDS * sizeof(u64), hipMemcpyDeviceToHost, s));
+    AL_HIP(hipStreamSynchronize(s));
+    (void)ws;
+    return AL_OK;
+}
+
+double as_double(u64 b) {
+    double d;
+    std::memcpy(&d, &b, sizeof d);
+    return d;
+}
+
+// Normwise backward error of one column; a zero residual counts as exact.
+double berr_of(double r, double anorm, double x, double b) {
+    if (r == 0.0) return 0.0;
+    const double den = anorm * x + b;
+    return r / den;  // NaN or inf when the iterate has blown up
+}
+
+// FP64 pivoted LU solve on a copy of A: X := A^{-1} B.
+int fallback_fp64(hipStream_t s, Workspace* ws, Arena& a, size_t n, size_t nrhs, const double* A, size_t lda,
+                  const double* B, size_t ldb, double* X, size_t ldx, int* info_out) {
+    const size_t mat = align256(n * n * sizeof(double));
+    AL_CHECK(grow(&ws->fbuf, &ws->fbuf_bytes, mat + align256(n * sizeof(int))));
+    double* Af = (double*)ws->fbuf;
+    int* ipiv = (int*)((char*)ws->fbuf + mat);
+    AL_CHECK(ws_reserve(ws, n, n * nrhs));
+    hipLaunchKernelGGL(copy2d_k, dim3(ceil_div(n * n, 256)), dim3(256), 0, s, (int)n, (int)n, A, lda, Af, n);
+    AL_HIP(hipGetLastError());
+    hipLaunchKernelGGL(copy2d_k, dim3(ceil_div(n * nrhs, 256)), dim3(256), 0, s, (int)n, (int)nrhs, B, ldb, X, ldx);
+    AL_HIP(hipGetLastError());
+    int* info = a.minfo + 2;
+    AL_CHECK(fill_i32(s, info, 1, 0));
+    AL_CHECK(iota_i32(s, ipiv, n));
+    AL_CHECK(getrf_f64(s, ws, n, n, Af, n, ipiv, info, ws->perm));
+    AL_CHECK(getrs_f64(s, ws, n, nrhs, Af, n, ipiv, ws->perm, X, ldx, info));
+    AL_HIP(hipMemcpyAsync(info_out, info, sizeof(int), hipMemcpyDeviceToHost, s));
+    AL_HIP(hipStreamSynchronize(s));
+    return AL_OK;
+}
+
+}  // namespace
+
+int gesv_mixed(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const double* A, size_t lda, const double* B,
+               size_t ldb, double* X, size_t ldx, int factor, const al_ir_options* opt_in, al_ir_stats* st) {
+    al_ir_options opt;
+    al_ir_default_options(&opt);
+    if (opt_in) {
+        opt = *opt_in;
+        if (opt.max_iter <= 0) opt.max_iter = 30;
+        if (!(opt.tol > 0.0)) opt.tol = 1e-14;
+        if (!(opt.stop_tol > 0.0)) opt.stop_tol = 1e-15;
+    }
+    al_ir_stats stats{};
+    if (n == 0 || nrhs == 0) {
+        stats.converged = 1;
+        if (st) *st = stats;
+        return AL_OK;
+    }
+    if (n > 32768 - PB) {
+        set_error("dgesv_mixed: n beyond 32704 is not supported (cooperative panel height)");
+        return AL_ERR_ARG;
+    }
+    if (factor != AL_FACTOR_FP64 && factor != AL_FACTOR_FP32 && factor != AL_FACTOR_BF16) {
+        set_error("dgesv_mixed: unknown factor precision");
+        return AL_ERR_ARG;
+    }
+    const bool bf16 = factor == AL_FACTOR_BF16;
+    const size_t np = (n + PB - 1) / PB * PB;
+    const size_t nr_max = nrhs < (size_t)NR_MAX ? nrhs : (size_t)NR_MAX;
+    Arena a;
+    AL_CHECK(make_arena(s, ws, n, np, nr_max, bf16 ? 512 : 256, factor != AL_FACTOR_FP64, bf16, &a));
+    if (!ws->hnorms) AL_HIP(hipHostMalloc((void**)&ws->hnorms, NORM_WORDS * sizeof(u64), hipHostMallocDefault));
+    u64* hn = (u64*)ws->hnorms;
+
+    // |A|_inf (also the padding scale)
+    AL_CHECK(fill_u32(s, (unsigned*)a.norms, 2 * NORM_WORDS, 0u));
+    {
+        const dim3 grid(ceil_div(n, GV_ROWS), (unsigned)a.nparts);
+        hipLaunchKernelGGL((gemv_part_k<1, true>), grid, dim3(256), 0, s, (int)n, 1, A, lda, (const double*)nullptr,
+                           (size_t)0, a.part);
+        AL_HIP(hipGetLastError());
+        hipLaunchKernelGGL(rowsum_max_k, dim3(ceil_div(n, 256)), dim3(256), 0, s, (int)n, (int)a.nparts, a.part,
+                           a.norms + NORM_A);
+        AL_HIP(hipGetLastError());
+    }
+
+    double berr_low = INFINITY;
+    bool ok_low = false;
+    int iters = 0;
+    if (factor != AL_FACTOR_FP64) {
+        hipLaunchKernelGGL(to_f32_pad_k, dim3(ceil_div(np * np, 256)), dim3(256), 0, s, (int)n, (int)np, A, lda, a.norms,
+                           a.F);
+        AL_HIP(hipGetLastError());
+        AL_CHECK(getrf_lowp(s, ws, a, bf16));
+        int minfo = 0;
+        AL_HIP(hipMemcpyAsync(&minfo, a.minfo, sizeof minfo, hipMemcpyDeviceToHost, s));
+        AL_HIP(hipStreamSynchronize(s));
+        if (minfo < 0) {
+            set_error("dgesv_mixed: a device-side wait timed out in the FP32 factorization");
+            return AL_ERR_DEVICE;
+        }
+        stats.factor_breakdown = minfo;
+        if (minfo == 0) {
+            berr_low = 0.0;
+            ok_low = true;
+            for (size_t c0 = 0; c0 < nrhs; c0 += NR_MAX) {
+                const size_t nr = (nrhs - c0) < (size_t)NR_MAX ? (nrhs - c0) : (size_t)NR_MAX;
+                const double* Bc = B + c0 * ldb;
+                double* Xc = X + c0 * ldx;
+                AL_CHECK(fill_u32(s, (unsigned*)(a.norms + NORM_B), 2 * (NORM_WORDS - NORM_B), 0u));
+                hipLaunchKernelGGL(colmax_k, dim3(ceil_div(n, 256)), dim3(256), 0, s, (int)n, (int)nr, Bc, ldb,
+                                   a.norms + NORM_B);
+                AL_HIP(hipGetLastError());
+                hipLaunchKernelGGL(copy2d_k, dim3(ceil_div(n * nr, 256)), dim3(256), 0, s, (int)n, (int)nr, Bc, ldb, a.R,
+                                   n);
+                AL_HIP(hipGetLastError());
+                double prev = INFINITY, berr = INFINITY;
+                int it = 0;
+                while (it < opt.max_iter) {
+                    ++it;
+                    hipLaunchKernelGGL(gather_perm_k, dim3(ceil_div(np * nr, 256)), dim3(256), 0, s, (int)n, (int)np,
+                                       (int)nr, a.perm, a.R, n, a.Y);
+                    AL_HIP(hipGetLastError());
+                    AL_CHECK(trsv_solve<float>(s, ws, true, true, np, nr, a.F, np, a.Y, np));
+                    AL_CHECK(trsv_solve<float>(s, ws, false, false, np, nr, a.F, np, a.Y, np));
+                    hipLaunchKernelGGL(add_k, dim3(ceil_div(n * nr, 256)), dim3(256), 0, s, (int)n, (int)np, (int)nr, a.Y,
+                                       Xc, ldx, it > 1 ? 1 : 0);
+                    AL_HIP(hipGetLastError());
+                    AL_CHECK(fill_u32(s, (unsigned*)(a.norms + NORM_R), 2 * (NORM_WORDS - NORM_R), 0u));
+                    AL_CHECK(residual(s, a, nr, A, lda, Bc, ldb, Xc, ldx));
+                    AL_CHECK(read_norms(s, ws, a, hn));
+                    const double anorm = as_double(hn[NORM_A]);
+                    berr = 0.0;
+                    for (size_t k = 0; k < nr; ++k) {
+                        const double e = berr_of(as_double(hn[NORM_R + k]), anorm, as_double(hn[NORM_X + k]),
+                                                 as_double(hn[NORM_B + k]));
+                        berr = (e > berr || std::isnan(e)) ? e : berr;
+                    }
+                    if (berr <= opt.stop_tol) break;
+                    if (!(berr < 0.5 * prev)) break;  // stalled or diverging (NaN included)
+                    prev = berr;
+                }
+                iters += it;
+                if (!(berr <= opt.tol)) ok_low = false;
+                berr_low = (berr > berr_low || std::isnan(berr)) ? berr : berr_low;
+            }
+        }
+    }
+    stats.iterations = iters;
+    stats.backward_error_low = berr_low;
+    stats.backward_error = berr_low;
+    stats.converged = ok_low ? 1 : 0;
+    int status = AL_OK;
+    if (!ok_low) {
+        if (factor == AL_FACTOR_FP64 || opt.allow_fallback) {
+            int info = 0;
+            AL_CHECK(fallback_fp64(s, ws, a, n, nrhs, A, lda, B, ldb, X, ldx, &info));
+            stats.fell_back = factor == AL_FACTOR_FP64 ? 0 : 1;
+            if (info > 0) {
+                status = AL_SINGULAR;
+                stats.converged = 0;
+            } else if (info < 0) {
+                set_error("dgesv_mixed: a device-side wait timed out in the FP64 factorization");
+                return AL_ERR_DEVICE;
+            } else {
+                // backward error of the FP64 result, for the record
+                double worst = 0.0;
+                for (size_t c0 = 0; c0 < nrhs; c0 += NR_MAX) {
+                    const size_t nr = (nrhs - c0) < (size_t)NR_MAX ? (nrhs - c0) : (size_t)NR_MAX;
+                    AL_CHECK(fill_u32(s, (unsigned*)(a.norms + NORM_B), 2 * (NORM_WORDS - NORM_B), 0u));
+                    hipLaunchKernelGGL(colmax_k, dim3(ceil_div(n, 256)), dim3(256), 0, s, (int)n, (int)nr, B + c0 * ldb,
+                                       ldb, a.norms + NORM_B);
+                    AL_HIP(hipGetLastError());
+                    AL_CHECK(residual(s, a, nr, A, lda, B + c0 * ldb, ldb, X + c0 * ldx, ldx));
+                    AL_CHECK(read_norms(s, ws, a, hn));
+                    const double anorm = as_double(hn[NORM_A]);
+                    for (size_t k = 0; k < nr; ++k) {
+                        const double e = berr_of(as_double(hn[NORM_R + k]), anorm, as_double(hn[NORM_X + k]),
+                                                 as_double(hn[NORM_B + k]));
+                        worst = (e > worst || std::isnan(e)) ? e : worst;
+                    }
+                }
+                stats.backward_error = worst;
+                stats.converged = worst <= opt.tol ? 1 : 0;
+            }
+        } else {
+            status = AL_CONVERGENCE;
+            set_error("dgesv_mixed: refinement did not reach the requested backward error");
+        }
+    }
+    if (st) *st = stats;
+    return status;
+}
+
+}  // namespace al
+
+extern "C" void al_ir_default_options(al_ir_options* opt) {
+    if (!opt) return;
+    opt->max_iter = 30;
+    opt->allow_fallback = 1;
+    opt->tol = 1e-14;
+    opt->stop_tol = 1e-15;
+}

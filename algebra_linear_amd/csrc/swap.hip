@@ -66,10 +66,11 @@ __global__ __launch_bounds__(512) void piv_pairs(const int* __restrict__ ipiv, i
 }
 
 // One wave per column: columns [0, n0) of A0, then [0, n1) of A1, then (if perm) the vector
-// perm as one more column.
-__global__ __launch_bounds__(256) void apply_pairs(double* __restrict__ A0, int n0, double* __restrict__ A1, int n1,
-                                                   size_t lda, double* __restrict__ perm,
-                                                   const int* __restrict__ pairs, const int* __restrict__ info) {
+// perm as one more column (FP64 matrices only).
+template <typename T>
+__global__ __launch_bounds__(256) void apply_pairs(T* __restrict__ A0, int n0, T* __restrict__ A1, int n1, size_t lda,
+                                                   double* __restrict__ perm, const int* __restrict__ pairs,
+                                                   const int* __restrict__ info) {
     __shared__ int pr[1 + PAIRS_MAX * 2];
     if (info && *info != 0) return;
     const int t = threadIdx.x;
@@ -79,18 +80,19 @@ __global__ __launch_bounds__(256) void apply_pairs(double* __restrict__ A0, int 
     const int lane = t & 63;
     const int c = blockIdx.x * 4 + (t >> 6);
     const int ntot = n0 + n1 + (perm ? 1 : 0);
-    double* col = nullptr;
+    T* col = nullptr;
     if (c < n0)
         col = A0 + (size_t)c * lda;
     else if (c < n0 + n1)
         col = A1 + (size_t)(c - n0) * lda;
-    else if (c < ntot)
-        col = perm;
-    double v[PAIRS_MAX / 64];
+    else if constexpr (sizeof(T) == sizeof(double)) {
+        if (c < ntot) col = (T*)perm;
+    }
+    T v[PAIRS_MAX / 64];
 #pragma unroll
     for (int q = 0; q < PAIRS_MAX / 64; ++q) {
         const int i = lane + 64 * q;
-        v[q] = (col && i < cnt) ? col[pr[2 + 2 * i]] : 0.0;
+        v[q] = (col && i < cnt) ? col[pr[2 + 2 * i]] : (T)0;
     }
     __syncthreads();  // every source read before any destination is written
 #pragma unroll
@@ -143,8 +145,9 @@ __global__ void copy_block_k(const double* __restrict__ W, int n, int nrhs, doub
 
 }  // namespace
 
-int swap_block_f64(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, size_t jb, double* A0, size_t n0,
-                   double* A1, size_t n1, size_t lda, double* perm, const int* info) {
+template <typename T>
+int swap_block_t(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, size_t jb, T* A0, size_t n0, T* A1,
+                 size_t n1, size_t lda, double* perm, const int* info) {
     if (jb == 0 || (n0 + n1 == 0 && !perm)) return AL_OK;
     if (jb > (size_t)JB_MAX) {
         set_error("swap_block: pivot block wider than 256");
@@ -153,10 +156,20 @@ int swap_block_f64(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, siz
     hipLaunchKernelGGL(piv_pairs, dim3(1), dim3(512), 0, s, ipiv, (int)k0, (int)jb, ws->pairs, info);
     AL_HIP(hipGetLastError());
     const size_t ncols = n0 + n1 + (perm ? 1 : 0);
-    hipLaunchKernelGGL(apply_pairs, dim3(ceil_div(ncols, 4)), dim3(256), 0, s, A0, (int)n0, A1, (int)n1, lda, perm,
+    hipLaunchKernelGGL(apply_pairs<T>, dim3(ceil_div(ncols, 4)), dim3(256), 0, s, A0, (int)n0, A1, (int)n1, lda, perm,
                        ws->pairs, info);
     AL_HIP(hipGetLastError());
     return AL_OK;
+}
+
+int swap_block_f64(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, size_t jb, double* A0, size_t n0,
+                   double* A1, size_t n1, size_t lda, double* perm, const int* info) {
+    return swap_block_t<double>(s, ws, ipiv, k0, jb, A0, n0, A1, n1, lda, perm, info);
+}
+
+int swap_block_f32(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, size_t jb, float* A0, size_t n0,
+                   float* A1, size_t n1, size_t lda, const int* info) {
+    return swap_block_t<float>(s, ws, ipiv, k0, jb, A0, n0, A1, n1, lda, nullptr, info);
 }
 
 int perm_init(hipStream_t s, double* perm, size_t n) {

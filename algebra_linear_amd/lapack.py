@@ -67,3 +67,27 @@ def dpotrf(a: np.ndarray, n: int) -> None:
 
 
 __all__ = ["dgetrf", "dgetrf_rect", "dgetrs", "dgesv", "dgesv_nrhs", "dpotrf", "LapackError", "BAD_SHAPE"]
+
+
+FACTORS = {"fp64": _lib.AL_FACTOR_FP64, "fp32": _lib.AL_FACTOR_FP32, "bf16": _lib.AL_FACTOR_BF16}
+
+
+def ir_options(max_iter: int = 30, allow_fallback: bool = True, tol: float = 1e-14,
+               stop_tol: float = 1e-15) -> "_lib.IrOptions":
+    """Refinement controls of :func:`dgesv_mixed` (the C defaults when called without arguments)."""
+    return _lib.IrOptions(int(max_iter), 1 if allow_fallback else 0, float(tol), float(stop_tol))
+
+
+def dgesv_mixed(a, b: np.ndarray, n: int, nrhs: int = 1, factor: str = "fp32", options=None) -> dict:
+    """Extension: solve A X = B (B n x nrhs, overwritten by X) with a low-precision pivoted
+    LU and FP64 iterative refinement, falling back to the FP64 pivoted LU; ``a`` is not
+    modified.  Returns the refinement statistics as a dict; raises ``LapackError`` with kind
+    ``Convergence`` if refinement fails and the fallback is disabled."""
+    aa, bb = _in(a), _out(b)
+    if factor not in FACTORS:
+        raise ValueError(f"factor must be one of {sorted(FACTORS)}")
+    opt = options if options is not None else ir_options()
+    st = _lib.IrStats()
+    check_lapack(_lib.load().al_dgesv_mixed(_p(aa), aa.size, _p(bb), bb.size, n, nrhs, FACTORS[factor],
+                                            ctypes.byref(opt), ctypes.byref(st)))
+    return {name: getattr(st, name) for name, _ in _lib.IrStats._fields_}

@@ -122,6 +122,69 @@ int al_dev_dpotrf(size_t n, double* A, size_t lda, int32_t* d_info, void* stream
  * full-row swap, src/lapack/mod.rs:141-146; LAPACK's dlaswp with incx = 1). */
 int al_dev_dlaswp(size_t ncols, double* A, size_t lda, size_t k0, size_t k1, const int32_t* d_ipiv, void* stream);
 
+/* Low-precision GEMMs (extension; the crate's GEMM is FP64 only, src/blas.rs:201-257), the
+ * building blocks of the mixed-precision factorization, column-major FP32 on the device:
+ *   al_dev_sgemm:     C := alpha op(A) op(B) + beta C on the FP32 matrix cores;
+ *   al_dev_gemm_bf16: C := alpha A B + beta C with A (m x k) and B (k x n) rounded to bfloat16
+ *                     (nearest even) and the products accumulated in FP32. */
+int al_dev_sgemm(int trans_a, int trans_b, size_t m, size_t n, size_t k, float alpha, const float* A, size_t lda,
+                 const float* B, size_t ldb, float beta, float* C, size_t ldc, void* stream);
+int al_dev_gemm_bf16(size_t m, size_t n, size_t k, float alpha, const float* A, size_t lda, const float* B, size_t ldb,
+                     float beta, float* C, size_t ldc, void* stream);
+
+/* ------------------------------------------------------------------------------------
+ * Mixed-precision solve (extension; the crate has no counterpart: its lapack::dgesv,
+ * src/lapack/mod.rs:309-323, factors and solves in FP64 only).
+ *
+ * A x = b is solved by factoring A in low precision and refining the solution in FP64:
+ *   1. F = A rounded to FP32 (padded with a scaled identity to a multiple of 64);
+ *   2. P F = L U with partial pivoting in FP32; the O(n^3) trailing updates run on the FP32
+ *      matrix cores (AL_FACTOR_FP32) or with bf16 operands and FP32 accumulation
+ *      (AL_FACTOR_BF16);
+ *   3. refinement: r = b - A x in FP64 (A itself, never rounded), d = U^{-1} L^{-1} P r
+ *      with FP32 factors and FP64 vectors, x += d, until the normwise backward error
+ *      max_j |r_j|_inf / (|A|_inf |x_j|_inf + |b_j|_inf) reaches stop_tol or stops
+ *      halving;
+ *   4. if the factorization met a zero pivot or the backward error is above tol, the
+ *      system is solved again with the FP64 pivoted LU (the path of al_dgesv) when
+ *      allow_fallback.
+ * AL_FACTOR_FP64 skips 1-3 and solves with the FP64 pivoted LU directly.  A is never
+ * modified.  Returns AL_OK when the final backward error is <= tol, AL_CONVERGENCE when it
+ * is not (refinement failed and fallback disabled), AL_SINGULAR when the FP64 LU meets an
+ * exactly zero pivot.  The refinement loop reads norms back to the host every step, so
+ * these calls synchronize their stream.
+ * ------------------------------------------------------------------------------------ */
+enum { AL_FACTOR_FP64 = 0, AL_FACTOR_FP32 = 1, AL_FACTOR_BF16 = 2 };
+
+typedef struct al_ir_options {
+    int max_iter;        /* refinement steps (0: 30) */
+    int allow_fallback;  /* nonzero: fall back to the FP64 pivoted LU when refinement fails */
+    double tol;          /* accepted backward error (0: 1e-14) */
+    double stop_tol;     /* refinement stops once every column is below it (0: 1e-15) */
+} al_ir_options;
+
+typedef struct al_ir_stats {
+    int iterations;         /* refinement steps taken (the first solve counts as one) */
+    int fell_back;          /* 1 if the FP64 pivoted LU produced the result */
+    int factor_breakdown;   /* 1-based column of an exactly zero pivot of the FP32 LU, else 0 */
+    int converged;          /* 1 if the returned solution meets tol */
+    double backward_error;  /* max over columns of the normwise backward error of the result */
+    double backward_error_low;  /* the same after the low-precision refinement (before any fallback) */
+} al_ir_stats;
+
+/* Fills *opt with the defaults (max_iter 30, allow_fallback 1, tol 1e-14, stop_tol 1e-15). */
+void al_ir_default_options(al_ir_options* opt);
+
+/* Host-slice form: a (n*n, read only), b (n*nrhs) is overwritten by x.  opt and stats may
+ * be NULL. */
+int al_dgesv_mixed(const double* a, size_t a_len, double* b, size_t b_len, size_t n, size_t nrhs, int factor,
+                   const al_ir_options* opt, al_ir_stats* stats);
+
+/* Device-pointer form on `stream` (NULL = null stream): X (ldx >= n) receives the solution
+ * of A X = B; A and B are not modified. */
+int al_dev_dgesv_mixed(size_t n, size_t nrhs, const double* A, size_t lda, const double* B, size_t ldb, double* X,
+                       size_t ldx, int factor, const al_ir_options* opt, al_ir_stats* stats, void* stream);
+
 /* ------------------------------------------------------------------------------------
  * Kernel instrumentation (extension): while enabled, every launch of an instrumented
  * kernel class is bracketed by HIP events on the stream it runs on, with its algorithmic

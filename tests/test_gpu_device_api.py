@@ -100,3 +100,86 @@ def test_profiler_counts_launches():
     assert LIB.al_prof_query(b"dgemm_nn", ctypes.byref(L), ctypes.byref(ms), ctypes.byref(fl), ctypes.byref(by)) == 0
     LIB.al_prof_reset()
     assert L.value == 3 and ms.value > 0 and fl.value == 3 * 2.0 * m * n * k
+
+
+# ------------------------------------------------------------------------------------------
+# low-precision GEMMs (the building blocks of the mixed-precision factorization)
+# ------------------------------------------------------------------------------------------
+
+def bf16_round(x32: np.ndarray) -> np.ndarray:
+    """float32 -> nearest-even bfloat16, returned as float32 (numpy restatement)."""
+    b = np.ascontiguousarray(x32, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16) << 16
+    return (r & 0xFFFFFFFF).astype(np.uint32).view(np.float32)
+
+
+def _f32_dev(a2d: np.ndarray, ld: int, dev):
+    """Column-major FP32 device copy of a 2-D array with leading dimension ld."""
+    m, n = a2d.shape
+    buf = np.zeros(ld * max(n, 1), dtype=np.float32)
+    for j in range(n):
+        buf[j * ld:j * ld + m] = a2d[:, j]
+    return torch.from_numpy(buf).to(dev)
+
+
+def _from_dev(t, m, n, ld):
+    h = t.cpu().numpy()
+    return np.stack([h[j * ld:j * ld + m] for j in range(n)], axis=1) if n else np.zeros((m, 0), np.float32)
+
+
+def test_bf16_rounding_restatement_is_nearest_even():
+    x = np.array([1.0, 1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8, 1.0 + 2.0 ** -9, -2.5], dtype=np.float32)
+    got = bf16_round(x)
+    assert got[0] == 1.0
+    assert got[1] == 1.0            # tie between 1 and 1 + 2^-7: to even
+    assert got[2] == 1.0 + 2.0 ** -6  # tie: to even (upwards)
+    assert got[3] == 1.0
+    assert got[4] == -2.5
+    assert np.array_equal(got, torch.from_numpy(x).to(torch.bfloat16).float().numpy())
+
+
+@pytest.mark.parametrize("m,n,k", [(130, 70, 100), (256, 256, 512), (1000, 513, 300), (64, 64, 64), (5, 3, 1)])
+@pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (-1.0, 1.0), (0.75, -0.5)])
+def test_gemm_bf16_equals_fp32_accumulation_of_rounded_operands(m, n, k, alpha, beta):
+    dev = _dev()
+    rng = np.random.default_rng(m * 7 + n * 3 + k)
+    A = rng.uniform(-1, 1, (m, k)).astype(np.float32)
+    B = rng.uniform(-1, 1, (k, n)).astype(np.float32)
+    C0 = rng.uniform(-1, 1, (m, n)).astype(np.float32)
+    lda, ldb, ldc = m + 3, k + 1, m + 5
+    dA, dB, dC = _f32_dev(A, lda, dev), _f32_dev(B, ldb, dev), _f32_dev(C0, ldc, dev)
+    rc = LIB.al_dev_gemm_bf16(m, n, k, alpha, dA.data_ptr(), lda, dB.data_ptr(), ldb, beta, dC.data_ptr(), ldc, None)
+    assert rc == 0, _lib.last_error()
+    torch.cuda.synchronize()
+    got = _from_dev(dC, m, n, ldc).astype(np.float64)
+    Ab, Bb = bf16_round(A).astype(np.float64), bf16_round(B).astype(np.float64)
+    exact = alpha * (Ab @ Bb) + beta * C0.astype(np.float64)
+    # FP32 accumulation of exact bf16 x bf16 products, then the FP32 alpha/beta epilogue
+    bound = 2 * (k + 2) * 2.0 ** -24 * (abs(alpha) * (np.abs(Ab) @ np.abs(Bb)) + abs(beta) * np.abs(C0))
+    assert np.all(np.abs(got - exact) <= bound + 1e-30)
+    # and it is NOT the FP32 product when the rounding matters (operands really are bf16)
+    if k >= 64:
+        full = alpha * (A.astype(np.float64) @ B.astype(np.float64)) + beta * C0
+        assert np.max(np.abs(got - full)) > np.max(bound)
+
+
+@pytest.mark.parametrize("ta", [0, 1])
+@pytest.mark.parametrize("tb", [0, 1])
+@pytest.mark.parametrize("m,n,k", [(130, 70, 100), (512, 256, 384), (7, 300, 65)])
+def test_sgemm_fp32_matrix_cores(ta, tb, m, n, k):
+    dev = _dev()
+    rng = np.random.default_rng(m + n + k + 10 * ta + 20 * tb)
+    Aop = rng.uniform(-1, 1, (m, k)).astype(np.float32)
+    Bop = rng.uniform(-1, 1, (k, n)).astype(np.float32)
+    C0 = rng.uniform(-1, 1, (m, n)).astype(np.float32)
+    A = Aop.T.copy() if ta else Aop
+    B = Bop.T.copy() if tb else Bop
+    lda, ldb, ldc = A.shape[0] + 2, B.shape[0] + 1, m
+    dA, dB, dC = _f32_dev(A, lda, dev), _f32_dev(B, ldb, dev), _f32_dev(C0, ldc, dev)
+    rc = LIB.al_dev_sgemm(ta, tb, m, n, k, 1.5, dA.data_ptr(), lda, dB.data_ptr(), ldb, -0.5, dC.data_ptr(), ldc, None)
+    assert rc == 0, _lib.last_error()
+    torch.cuda.synchronize()
+    got = _from_dev(dC, m, n, ldc).astype(np.float64)
+    exact = 1.5 * (Aop.astype(np.float64) @ Bop.astype(np.float64)) - 0.5 * C0
+    bound = 2 * (k + 2) * 2.0 ** -24 * (1.5 * (np.abs(Aop).astype(np.float64) @ np.abs(Bop)) + 0.5 * np.abs(C0))
+    assert np.all(np.abs(got - exact) <= bound)

@@ -160,6 +160,16 @@ int getrf_f64(hipStream_t s, Workspace* ws, size_t m, size_t n, double* A, size_
 int panel_f32(hipStream_t s, Workspace* ws, float* P, size_t lda, size_t row0, size_t m, size_t ncols, int* ipiv,
               int* info);
 
+// Tournament-pivoted 64-column panel of an FP32 matrix F (rows J..J+m, columns J..J+64):
+// tslu_panel chooses the pivots (ipiv[J..J+64), global indices) and the LU of the pivot rows;
+// after the interchanges are applied to the panel's columns, tslu_finish writes L11 \ U11 and
+// solves the rows below.  scratch: tslu_scratch_bytes(m) bytes; counters: zero-filled words
+// (one per tree node, left at zero by every panel).
+size_t tslu_scratch_bytes(size_t m);
+int tslu_panel(hipStream_t s, float* F, size_t ld, size_t J, size_t m, int* ipiv, int* info, void* scratch,
+               unsigned* counters);
+int tslu_finish(hipStream_t s, float* F, size_t ld, size_t J, size_t m, const void* scratch);
+
 // Solve with a getrf factorization: B (n x nrhs) := A^{-1} B.  perm (nullable) is getrf's row
 // origin vector; without it the pivots are composed first.
 int getrs_f64(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const double* LU, size_t lda, const int* ipiv,

@@ -175,6 +175,10 @@ int workspace_for(hipStream_t s, Workspace** out, size_t ipiv_len) {
         w.abort_flag = (unsigned*)((char*)p + 64);
         AL_HIP(hipMalloc((void**)&w.flags, panel_flags_count() * sizeof(unsigned)));
         AL_HIP(hipMemset(w.flags, 0, panel_flags_count() * sizeof(unsigned)));
+        // hipMemset runs on the legacy null stream, which does not order work on non-blocking
+        // streams (the library's own and, typically, the caller's): finish it before any kernel
+        // can read these words
+        AL_HIP(hipDeviceSynchronize());
         w.gen = 0;
         w.slots_bytes = panel_slots_bytes();
         AL_HIP(hipMalloc((void**)&w.slots, w.slots_bytes));
@@ -263,6 +267,7 @@ int ws_reserve_flags(Workspace* ws, size_t entries) {
         ws->trsv_flags_len = 0;
         AL_HIP(hipMalloc((void**)&ws->trsv_flags, entries * sizeof(unsigned)));
         AL_HIP(hipMemset(ws->trsv_flags, 0, entries * sizeof(unsigned)));
+        AL_HIP(hipDeviceSynchronize());  // see workspace_for: the null stream orders nothing here
         ws->trsv_flags_len = entries;
     }
     return AL_OK;

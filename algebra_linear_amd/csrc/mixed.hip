@@ -253,6 +253,8 @@ struct Arena {
     int* minfo = nullptr;     // [0] low-precision factorization status, [2] fallback status
     __bf16* At = nullptr;     // bf16 staging of the outer block's L and U
     __bf16* Bt = nullptr;
+    void* tslu = nullptr;     // tournament panel scratch (nominees, L11 \ U11)
+    unsigned* counters = nullptr;  // its arrival counters (zero between panels)
 };
 
 constexpr int NORM_A = 0, NORM_B = 1, NORM_R = 17, NORM_X = 33, NORM_WORDS = 64;
@@ -281,7 +283,7 @@ int make_arena(hipStream_t s, Workspace* ws, size_t n, size_t np, size_t nr, siz
     a->np = np;
     a->nr = nr;
     a->nparts = ceil_div(n, GV_COLS);
-    size_t off = 0;
+    size_t off = 4096;  // [0, 4096): the tournament's arrival counters, zeroed by every call
     const size_t oF = off;
     if (lowp) off += align256(np * np * sizeof(float));
     const size_t oV = off;
@@ -302,8 +304,13 @@ int make_arena(hipStream_t s, Workspace* ws, size_t n, size_t np, size_t nr, siz
     if (bf16) off += align256(np * nbo * 2);
     const size_t oB = off;
     if (bf16) off += align256(np * nbo * 2);
+    const size_t oT = off;
+    if (lowp) off += align256(tslu_scratch_bytes(np));
     AL_CHECK(grow(&ws->mbuf, &ws->mbuf_bytes, off));
     char* base = (char*)ws->mbuf;
+    AL_CHECK(fill_u32(s, (unsigned*)base, 1024, 0u));
+    a->tslu = base + oT;
+    a->counters = (unsigned*)base;
     a->F = (float*)(base + oF);
     a->ipiv = (int*)(base + oV);
     a->perm = (double*)(base + oW);
@@ -339,10 +346,12 @@ int getrf_lowp(hipStream_t s, Workspace* ws, Arena& a, bool bf16) {
     for (int K = 0; K < np; K += nbo) {
         const int kb = (np - K) < nbo ? (np - K) : nbo;
         const int KE = K + kb;
-        // the outer panel, 64 columns at a time; swaps inside the outer panel's columns
+        // the outer panel, 64 columns at a time: tournament pivots, the interchanges applied to
+        // the outer panel's columns, then L21 of the 64 columns
         for (int J = K; J < KE; J += PB) {
-            AL_CHECK(panel_f32(s, ws, at(J, J), ld, J, np - J, PB, a.ipiv, info));
-            AL_CHECK(swap_block_f32(s, ws, a.ipiv, J, PB, at(0, K), J - K, at(0, J + PB), KE - J - PB, ld, info));
+            AL_CHECK(tslu_panel(s, F, ld, J, np - J, a.ipiv, info, a.tslu, a.counters));
+            AL_CHECK(swap_block_f32(s, ws, a.ipiv, J, PB, at(0, K), KE - K, nullptr, 0, ld, info));
+            AL_CHECK(tslu_finish(s, F, ld, J, np - J, a.tslu));
             if (J + PB < KE) {
                 AL_CHECK(trsm_ll64(s, a, J, J + PB, KE));
                 AL_CHECK(gemm_f32(s, false, false, np - J - PB, KE - J - PB, PB, -1.f, at(J + PB, J), ld, at(J, J + PB),

@@ -29,12 +29,14 @@ __device__ __forceinline__ double bcast_lane(double v, int src) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-template <typename TA, bool LOWER, bool UNIT>
+// NR: right-hand sides the registers are sized for (nrhs <= NR); every loop over them is
+// unrolled, so no accumulator is indexed dynamically.
+template <typename TA, bool LOWER, bool UNIT, int NR>
 __global__ __launch_bounds__(256) void trsv_blocked(int n, int nrhs, const TA* __restrict__ A, size_t lda,
                                                     double* __restrict__ X, size_t ldx, unsigned* __restrict__ flags,
                                                     unsigned gen, unsigned* __restrict__ abort_flag,
                                                     long long timeout) {
-    __shared__ double red[4][TB][NRHS_MAX];
+    __shared__ double red[4][TB][NR];
     __shared__ double D[TB][TB + 1];
     __shared__ int ok_lds;
     const int nblk = (n + TB - 1) / TB;
@@ -51,9 +53,9 @@ __global__ __launch_bounds__(256) void trsv_blocked(int n, int nrhs, const TA* _
                                                                   : 0.0;
     }
 
-    double acc[NRHS_MAX];
+    double acc[NR];
 #pragma unroll
-    for (int k = 0; k < NRHS_MAX; ++k) acc[k] = 0.0;
+    for (int k = 0; k < NR; ++k) acc[k] = 0.0;
 
     // column blocks this block depends on: [0, b) for lower, (b, nblk) for upper, visited in
     // the order they are solved.  The matrix part of the next block is loaded into registers
@@ -99,25 +101,26 @@ __global__ __launch_bounds__(256) void trsv_blocked(int n, int nrhs, const TA* _
         if (!ok) break;
         const int c0 = c * TB + part * 16;
 #pragma unroll
-        for (int k = 0; k < NRHS_MAX; ++k) {
-            if (k >= nrhs) break;
-            double loc = 0.0;  // this block's partial sum, then one add into the running total
+        for (int k = 0; k < NR; ++k) {
+            if (k < nrhs) {
+                double loc = 0.0;  // this block's partial sum, then one add into the running total
 #pragma unroll
-            for (int jj = 0; jj < 16; ++jj)
-                loc += a_cur[jj] * ((c0 + jj < n) ? X[(size_t)(c0 + jj) + (size_t)k * ldx] : 0.0);
-            acc[k] += loc;
+                for (int jj = 0; jj < 16; ++jj)
+                    loc += a_cur[jj] * ((c0 + jj < n) ? X[(size_t)(c0 + jj) + (size_t)k * ldx] : 0.0);
+                acc[k] += loc;
+            }
         }
     }
     if (!ok) return;
 
 #pragma unroll
-    for (int k = 0; k < NRHS_MAX; ++k) red[part][row][k] = acc[k];
+    for (int k = 0; k < NR; ++k) red[part][row][k] = acc[k];
     __syncthreads();
     if (part == 0) {
         // y = b - (p0 + p1 + p2 + p3), in this fixed order
-        double y[NRHS_MAX];
+        double y[NR];
 #pragma unroll
-        for (int k = 0; k < NRHS_MAX; ++k) {
+        for (int k = 0; k < NR; ++k) {
             const double sum = ((red[0][row][k] + red[1][row][k]) + red[2][row][k]) + red[3][row][k];
             y[k] = (k < nrhs && row < nb) ? X[(size_t)(r0 + row) + (size_t)k * ldx] - sum : 0.0;
         }
@@ -128,30 +131,56 @@ __global__ __launch_bounds__(256) void trsv_blocked(int n, int nrhs, const TA* _
             const double lik = D[row][i];
             const bool below = LOWER ? (row > i) : (row < i);
 #pragma unroll
-            for (int k = 0; k < NRHS_MAX; ++k) {
-                if (k >= nrhs) break;
-                if (!UNIT && row == i) y[k] = y[k] / D[i][i];
-                const double xi = bcast_lane(y[k], i);
-                if (below) y[k] -= lik * xi;
+            for (int k = 0; k < NR; ++k) {
+                if (k < nrhs) {
+                    if (!UNIT && row == i) y[k] = y[k] / D[i][i];
+                    const double xi = bcast_lane(y[k], i);
+                    if (below) y[k] -= lik * xi;
+                }
             }
         }
         if (row < nb)
 #pragma unroll
-            for (int k = 0; k < NRHS_MAX; ++k)
+            for (int k = 0; k < NR; ++k)
                 if (k < nrhs) X[(size_t)(r0 + row) + (size_t)k * ldx] = y[k];
         if (row == 0) __hip_atomic_store(&flags[b * 16], gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
+}
+
+template <typename TA, bool LOWER, bool UNIT, int NR>
+void launch_nr(hipStream_t s, unsigned nblk, int n, int nrhs, const TA* A, size_t lda, double* X, size_t ldx,
+               unsigned* flags, unsigned gen, unsigned* abort_flag, long long timeout) {
+    hipLaunchKernelGGL((trsv_blocked<TA, LOWER, UNIT, NR>), dim3(nblk), dim3(256), 0, s, n, nrhs, A, lda, X, ldx, flags,
+                       gen, abort_flag, timeout);
 }
 
 template <typename TA, bool LOWER, bool UNIT>
 int launch_trsv(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const TA* A, size_t lda, double* X, size_t ldx) {
     const size_t nblk = (n + TB - 1) / TB;
     AL_CHECK(ws_reserve_flags(ws, nblk * 16));
+    if (ws->trsv_gen > 0xF0000000u) {  // keep generations unique: restart from a cleared array
+        AL_CHECK(fill_u32(s, ws->trsv_flags, nblk * 16, 0u));
+        ws->trsv_gen = 0;
+    }
     const unsigned gen = ++ws->trsv_gen;
     const long long timeout = 200000000LL;  // 2 s at 100 MHz
-    hipLaunchKernelGGL((trsv_blocked<TA, LOWER, UNIT>), dim3((unsigned)nblk), dim3(256), 0, s, (int)n, (int)nrhs, A, lda,
-                       X, ldx, ws->trsv_flags, gen, ws->abort_flag, timeout);
+    const int ticket = prof_begin(s);
+    const unsigned g = (unsigned)nblk;
+    if (nrhs <= 1)
+        launch_nr<TA, LOWER, UNIT, 1>(s, g, (int)n, (int)nrhs, A, lda, X, ldx, ws->trsv_flags, gen, ws->abort_flag, timeout);
+    else if (nrhs <= 2)
+        launch_nr<TA, LOWER, UNIT, 2>(s, g, (int)n, (int)nrhs, A, lda, X, ldx, ws->trsv_flags, gen, ws->abort_flag, timeout);
+    else if (nrhs <= 4)
+        launch_nr<TA, LOWER, UNIT, 4>(s, g, (int)n, (int)nrhs, A, lda, X, ldx, ws->trsv_flags, gen, ws->abort_flag, timeout);
+    else if (nrhs <= 8)
+        launch_nr<TA, LOWER, UNIT, 8>(s, g, (int)n, (int)nrhs, A, lda, X, ldx, ws->trsv_flags, gen, ws->abort_flag, timeout);
+    else
+        launch_nr<TA, LOWER, UNIT, 16>(s, g, (int)n, (int)nrhs, A, lda, X, ldx, ws->trsv_flags, gen, ws->abort_flag, timeout);
     AL_HIP(hipGetLastError());
+    if (ticket >= 0) {
+        const double dn = (double)n;
+        prof_end(ticket, s, "trsv", dn * dn * (double)nrhs, 0.5 * dn * dn * sizeof(TA) + 16.0 * dn * (double)nrhs);
+    }
     return AL_OK;
 }
 

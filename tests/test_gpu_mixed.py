@@ -130,16 +130,31 @@ def test_fp64_factor_is_the_plain_pivoted_solve():
     assert st["fell_back"] == 0 and st["iterations"] == 0 and st["converged"] == 1
 
 
-@pytest.mark.parametrize("consistent", [False, True])
-def test_singular_matrix_raises_singular_like_dgesv(consistent):
-    # the crate's singular matrix (src/lapack/mod.rs:497-508): the FP32 pivoted LU meets the
-    # same exactly zero pivot, the FP64 fallback confirms it, and the call reports Singular as
-    # lapack::dgesv does -- whether or not b lies in the range of A
+def test_inconsistent_singular_system_raises_singular():
+    # the crate's singular matrix (src/lapack/mod.rs:497-508) with b outside its range: no
+    # iterate reaches a small backward error, and the FP64 fallback meets the crate's exactly
+    # zero pivot, so the call reports Singular as lapack::dgesv does
     a = singular3()
-    b = mat(a, 3, 3) @ np.array([1.0, 1.0, 1.0]) if consistent else np.array([1.0, 0.0, 0.0])
+    b = np.array([1.0, 0.0, 0.0])
     with pytest.raises(LapackError) as e:
         lapack.dgesv_mixed(a, b.copy(), 3, 1, "fp32")
     assert e.value.kind == "Singular"
+
+
+def test_consistent_singular_system_is_singular_or_solved_backward_stably():
+    # b in the range of the singular matrix: either the FP32 elimination meets an exactly zero
+    # pivot (then the FP64 fallback reports Singular, as lapack::dgesv), or refinement returns
+    # one of the solutions with an FP64-level backward error
+    a = singular3()
+    b = mat(a, 3, 3) @ np.array([1.0, 1.0, 1.0])
+    x = b.copy()
+    try:
+        st = lapack.dgesv_mixed(a, x, 3, 1, "fp32")
+    except LapackError as e:
+        assert e.kind == "Singular"
+        return
+    assert st["converged"] == 1
+    assert O.backward_error(a, x, b, 3) <= TOL
 
 
 def test_bad_shapes_and_empty():
@@ -188,3 +203,25 @@ def test_device_api_on_a_torch_stream_with_padded_leading_dimensions():
         xj = xh[j * ldb:j * ldb + n]
         assert O.backward_error(cm(a), xj, b[:, j].copy(), n) <= TOL
         assert np.all(xh[j * ldb + n:(j + 1) * ldb] == 7.0)  # rows beyond n untouched
+
+
+def test_first_call_on_a_fresh_thread_refines_without_fallback():
+    # a new thread gets a fresh stream and workspace; the first solve on them must see
+    # initialized device-side flags (a regression test: the flag arrays were cleared on the
+    # null stream, which does not order the library's non-blocking stream)
+    import threading
+    n = 600
+    a = conditioned(n, 1e2)
+    b = normal(n, [3, 5, 7, 9])
+    out = {}
+
+    def run():
+        x = b.copy()
+        out["st"] = lapack.dgesv_mixed(a, x, n, 1, "fp32")
+        out["x"] = x
+
+    th = threading.Thread(target=run)
+    th.start()
+    th.join()
+    assert out["st"]["fell_back"] == 0 and out["st"]["converged"] == 1, out["st"]
+    assert O.backward_error(a, out["x"], b, n) <= TOL

@@ -34,17 +34,18 @@ struct Tree {
     int offset[MAX_LEVELS];       // first node id of each level
 };
 
-constexpr int CW = W / 4;    // columns per wave
+constexpr int KB = 8;        // columns per elimination block
 
 // LDS of one panel workgroup.
 struct TsluLds {
-    float lmul[2][LEAF];  // multipliers of the current step, by local row
-    float prow[2][W];     // the pivot row; each wave writes and reads only its own 16 columns
-    int win[2];           // local index of the current pivot row
-    float piv[W];         // pivots in elimination order
-    int winners[W];       // global row of each pivot
+    float mag[2][4];          // per wave: its best candidate's |x| ..
+    int crow[2][4];           // .. its row ..
+    float cval[2][4][KB];     // .. and that row's values in the current block
+    __attribute__((aligned(16))) float ur[KB][W];  // the block's pivot rows right of the block (U12 after the solve)
+    float l11[KB][KB];        // the block's multipliers among its pivot rows
+    float piv[W];             // pivots in elimination order
+    int winners[W];           // global row of each pivot
     int last;
-    int occ[W], where_[W];
 };
 
 // Maximum over the wave of values >= -1, on the VALU: prefix maxima inside each row of 16
@@ -61,161 +62,170 @@ __device__ __forceinline__ float wave_max_dpp(float v) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
-// Gaussian elimination with partial pivoting of up to 256 rows x 64 columns held by the
-// workgroup: lane l of wave q holds rows l, l+64, l+128, l+192 (slots a = 0..3) in columns
-// 16q..16q+15.  Step j: the wave owning column j finds the first maximum of |x| over the active
-// rows (lane-local over the slots, then a wave max and the lowest lane holding it), scales the
-// column by the pivot's reciprocal and publishes the multipliers; after one barrier every wave
-// takes the pivot row's values in its own columns from the lane that holds them and updates
-// its active rows.  A NaN counts as the largest magnitude, so it ends up on the diagonal.
-// On return step[a] is the step at which slot a's row was chosen (-1 if never) and the chosen
-// rows hold their L \ U entries.
-// The owner wave's half of step j (column cj = j % 16 of this wave, a compile-time index so the
-// row registers are never indexed dynamically): first maximum of |x| over the active rows
-// (lane-local over the slots, then the wave), the column scaled by the pivot's reciprocal,
-// and the multipliers, the pivot's position, value and row published through LDS.  A NaN
-// counts as the largest magnitude, so it ends up on the diagonal.
-template <int CJ>
-__device__ __forceinline__ void owner_part(float (&x)[4][CW], const bool (&active)[4], const int (&row)[4],
-                                           TsluLds& L, int lane, int b, int j) {
-    float m = -1.f, mx = 0.f;
-    int ma = 0, mrow = 0;
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        const float v = !active[a] ? -1.f : (x[a][CJ] != x[a][CJ] ? INFINITY : fabsf(x[a][CJ]));
-        const bool gt = v > m;
-        m = gt ? v : m;
-        ma = gt ? a : ma;
-        mx = gt ? x[a][CJ] : mx;
-        mrow = gt ? row[a] : mrow;
-    }
+__device__ __forceinline__ float lane_value(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// Step J of the elimination, inside the block of columns [C0, C0 + 8): every wave finds its
+// first maximum of |x[J]| over its active rows (DPP max, then the lowest lane holding it) and
+// publishes it with that row's block values; after one barrier every thread takes the first
+// maximum over the waves (ties to the lower wave, i.e. the lower row), and the active rows
+// eliminate column J inside the block.  A NaN counts as the largest magnitude.
+template <int C0, int J>
+__device__ __forceinline__ void narrow_step(float (&x)[W], bool& active, int row, int& step, TsluLds& L, int lane,
+                                            int wave) {
+    constexpr int b = J & 1;
+    const float m = !active ? -1.f : (x[J] != x[J] ? INFINITY : fabsf(x[J]));
     const float wm = wave_max_dpp(m);
     const unsigned long long bal = __ballot(m == wm);
     int wl = __ffsll((long long)bal) - 1;
     wl = wl < 0 ? 0 : wl;
-    const int wa = __builtin_amdgcn_readlane(ma, wl);
-    const float piv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mx), wl));
-    const float rp = 1.f / piv;
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        const bool upd = active[a] && !(a == wa && lane == wl);
-        const float l = upd ? x[a][CJ] * rp : 0.f;
-        x[a][CJ] = upd ? l : x[a][CJ];
-        L.lmul[b][lane + 64 * a] = l;
+    if (lane == 0) {
+        L.mag[b][wave] = wm;
+        L.crow[b][wave] = __builtin_amdgcn_readlane(row, wl);
     }
-    if (lane == wl) {
-        L.win[b] = wl + 64 * wa;
-        L.piv[j] = piv;
-        L.winners[j] = mrow;
+    if (lane < KB - (J - C0)) {
+        // the candidate's values in columns J .. C0 + 7, one per lane
+        float v = 0.f;
+#pragma unroll
+        for (int c = J; c < C0 + KB; ++c) v = lane == c - J ? lane_value(x[c], wl) : v;
+        L.cval[b][wave][lane] = v;
+    }
+    __syncthreads();
+    int best = 0;
+    float bm = L.mag[b][0];
+#pragma unroll
+    for (int w = 1; w < 4; ++w) {
+        const float mw = L.mag[b][w];
+        best = mw > bm ? w : best;
+        bm = mw > bm ? mw : bm;
+    }
+    const int prow = L.crow[b][best];
+    const float piv = L.cval[b][best][0];
+    if (active && row == prow) {
+        active = false;
+        step = J;
+    }
+    if (active) {
+        const float l = x[J] / piv;
+        x[J] = l;
+#pragma unroll
+        for (int c = J + 1; c < C0 + KB; ++c) x[c] = fmaf(-l, L.cval[b][best][c - J], x[c]);
+    }
+    if (threadIdx.x == 0) {
+        L.piv[J] = piv;
+        L.winners[J] = prow;
     }
 }
 
-template <int... Cs>
-__device__ __forceinline__ void owner_dispatch(std::integer_sequence<int, Cs...>, int cj, float (&x)[4][CW],
-                                               const bool (&active)[4], const int (&row)[4], TsluLds& L, int lane,
-                                               int b, int j) {
-    ((cj == Cs ? owner_part<Cs>(x, active, row, L, lane, b, j) : void()), ...);
-}
-
-// Gaussian elimination with partial pivoting of up to 256 rows x 64 columns held by the
-// workgroup: lane l of wave q holds rows l, l+64, l+128, l+192 (slots a = 0..3) in columns
-// 16q..16q+15.  Step j: the wave owning column j picks the pivot and publishes the
-// multipliers; after one barrier every wave takes the pivot row's values in its own columns
-// (written to LDS by the lane that holds the row) and updates its active rows.  The step
-// loop stays rolled so the kernel's code fits the instruction cache.  On return step[a] is
-// the step at which slot a's row was chosen (-1 if never); the chosen rows hold L \ U.
-__device__ __forceinline__ void gepp2d(float (&x)[4][CW], bool (&active)[4], const int (&row)[4], int (&step)[4],
-                                       TsluLds& L) {
-    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+// After the 8 steps of block [C0, C0 + 8): the pivot rows' entries right of the block become
+// U12 = L11^{-1} A12 (one thread per column), every other active row subtracts its 8
+// multipliers times U12, and the pivot rows take their U12 rows.
+template <int C0>
+__device__ __forceinline__ void block_update(float (&x)[W], bool active, int step, TsluLds& L) {
+    constexpr int REST = W - C0 - KB;
+    if (REST == 0) return;
+    const int k = step - C0;
+    const bool piv_here = step >= C0 && step < C0 + KB;
+    __syncthreads();  // every thread is done with the block's step buffers
+    if (piv_here) {
 #pragma unroll
-    for (int a = 0; a < 4; ++a) step[a] = -1;
-#pragma nounroll
-    for (int j = 0; j < W; ++j) {
-        const int b = j & 1, qj = j >> 4, cj = j & 15;
-        if (q == qj) owner_dispatch(std::make_integer_sequence<int, CW>{}, cj, x, active, row, L, lane, b, j);
-        __syncthreads();
-        const int local = L.win[b];
+        for (int c = 0; c < REST; ++c) L.ur[k][c] = x[C0 + KB + c];
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const bool hit = lane + 64 * a == local;
-            step[a] = hit ? j : step[a];
-            active[a] = active[a] && !hit;
+        for (int i = 0; i < KB; ++i) L.l11[k][i] = i < k ? x[C0 + i] : 0.f;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < REST) {
+        const int c = threadIdx.x;
+        float u[KB];
+#pragma unroll
+        for (int kk = 0; kk < KB; ++kk) {
+            float v = L.ur[kk][c];
+#pragma unroll
+            for (int i = 0; i < kk; ++i) v = fmaf(-L.l11[kk][i], u[i], v);
+            u[kk] = v;
         }
-        // columns of this wave right of j: all of them beyond the owner, c > cj in the owner
-        const int c_from = q > qj ? 0 : (q == qj ? cj + 1 : CW);
-        if (c_from < CW) {
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
-                if (lane + 64 * a == local)
+        for (int kk = 0; kk < KB; ++kk) L.ur[kk][c] = u[kk];
+    }
+    __syncthreads();
+    if (piv_here) {
 #pragma unroll
-                    for (int c = 0; c < CW; ++c) L.prow[b][q * CW + c] = x[a][c];
-            __builtin_amdgcn_wave_barrier();
-            float u[CW];
+        for (int c = 0; c < REST; ++c) x[C0 + KB + c] = L.ur[k][c];
+    } else if (active) {
 #pragma unroll
-            for (int c = 0; c < CW; ++c) u[c] = L.prow[b][q * CW + c];
+        for (int c = 0; c < REST; ++c) {
+            float v = x[C0 + KB + c];
 #pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                const float l = L.lmul[b][lane + 64 * a];
-#pragma unroll
-                for (int c = 0; c < CW; ++c) {
-                    const float v = fmaf(-l, u[c], x[a][c]);
-                    x[a][c] = (active[a] && c >= c_from) ? v : x[a][c];
-                }
-            }
+            for (int kk = 0; kk < KB; ++kk) v = fmaf(-x[C0 + kk], L.ur[kk][c], v);
+            x[C0 + KB + c] = v;
         }
     }
+}
+
+template <int C0, int... Ks>
+__device__ __forceinline__ void elim_block(std::integer_sequence<int, Ks...>, float (&x)[W], bool& active, int row,
+                                           int& step, TsluLds& L, int lane, int wave) {
+    (narrow_step<C0, C0 + Ks>(x, active, row, step, L, lane, wave), ...);
+    block_update<C0>(x, active, step, L);
+}
+
+template <int... Bs>
+__device__ __forceinline__ void elim_all(std::integer_sequence<int, Bs...>, float (&x)[W], bool& active, int row,
+                                         int& step, TsluLds& L, int lane, int wave) {
+    (elim_block<Bs * KB>(std::make_integer_sequence<int, KB>{}, x, active, row, step, L, lane, wave), ...);
+}
+
+// Gaussian elimination with partial pivoting of up to 256 rows (one per thread, 64 columns in
+// registers), blocked by 8 columns: inside a block each step updates only the block's
+// columns; the rest of the row is updated once per block with a rank-8 product.  On return
+// `step` is the step at which this thread's row was chosen (-1 if never); chosen rows hold
+// L \ U.
+__device__ __forceinline__ void gepp_rows(float (&x)[W], bool active, int row, int& step, TsluLds& L) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    step = -1;
+    elim_all(std::make_integer_sequence<int, W / KB>{}, x, active, row, step, L, lane, wave);
 }
 
 // One launch per panel.  Leaves are workgroups 0..count[0]-1; node n of level l stores its 64
 // nominees' original rows at cand[node][s][0..64) and their global row indices at cidx[node][s].
 __global__ __launch_bounds__(256, 1) void tslu_k(const float* __restrict__ F, size_t ld, int J, int m, Tree tree,
-                                              float* __restrict__ cand, int* __restrict__ cidx,
-                                              unsigned* __restrict__ counters, float* __restrict__ D,
-                                              int* __restrict__ ipiv, int* __restrict__ info) {
+                                                 float* __restrict__ cand, int* __restrict__ cidx,
+                                                 unsigned* __restrict__ counters, float* __restrict__ D,
+                                                 int* __restrict__ ipiv, int* __restrict__ info) {
     __shared__ TsluLds L;
     const int t = threadIdx.x, lane = t & 63, q = t >> 6;
 
-    // leaf: local rows i = lane + 64 a of rows J + 256 b ..
+    // leaf: thread t holds row J + 256 b + t
     int node = blockIdx.x;
     int level = 0;
-    int row[4];
-    bool active[4];
-    float x[4][CW], x0[4][CW];
+    float x[W];
+    const int r0 = blockIdx.x * LEAF + t;
+    bool active = r0 < m;
+    int row = J + r0;
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        const int r = blockIdx.x * LEAF + lane + 64 * a;
-        active[a] = r < m;
-        row[a] = J + r;
-#pragma unroll
-        for (int c = 0; c < CW; ++c)
-            x[a][c] = active[a] ? F[(size_t)row[a] + (size_t)(J + q * CW + c) * ld] : 0.f;
-    }
+    for (int c = 0; c < W; ++c) x[c] = active ? F[(size_t)row + (size_t)(J + c) * ld] : 0.f;
     for (;;) {
+        // the original values travel with a nomination: keep them in LDS-free registers
+        float x0[W];
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int c = 0; c < CW; ++c) x0[a][c] = x[a][c];
-        int step[4];
-        gepp2d(x, active, row, step, L);
+        for (int c = 0; c < W; ++c) x0[c] = x[c];
+        int step;
+        gepp_rows(x, active, row, step, L);
         if (level == tree.levels - 1) {
             // root: L11 \ U11 in pivot order (winners and pivots are in LDS already)
+            if (step >= 0)
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
-                if (step[a] >= 0)
-#pragma unroll
-                    for (int c = 0; c < CW; ++c) D[step[a] + (q * CW + c) * W] = x[a][c];
+                for (int c = 0; c < W; ++c) D[step + c * W] = x[c];
             break;
         }
         // nominate: original values and row index of the rows chosen as pivots
+        if (step >= 0) {
+            float4* dst = (float4*)(cand + ((size_t)node * W + step) * W);
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
-            if (step[a] >= 0) {
-                float4* dst = (float4*)(cand + ((size_t)node * W + step[a]) * W + q * CW);
-#pragma unroll
-                for (int v = 0; v < CW / 4; ++v)
-                    dst[v] = float4{x0[a][4 * v], x0[a][4 * v + 1], x0[a][4 * v + 2], x0[a][4 * v + 3]};
-                if (q == 0) cidx[node * W + step[a]] = row[a];
-            }
+            for (int v = 0; v < W / 4; ++v) dst[v] = float4{x0[4 * v], x0[4 * v + 1], x0[4 * v + 2], x0[4 * v + 3]};
+            cidx[node * W + step] = row;
+        }
         __threadfence();
         __syncthreads();
         const int parent_level = level + 1;
@@ -233,27 +243,24 @@ __global__ __launch_bounds__(256, 1) void tslu_k(const float* __restrict__ F, si
         }
         __syncthreads();
         if (!L.last) return;
-        // this workgroup merges the parent's children: slot a holds child a, lane = nominee
+        // this workgroup merges the parent's children: thread t holds nominee t
+        const int child = tree.offset[level] + first_child + t / W;
+        active = t < nchild * W;
+        if (active) {
+            const float4* src = (const float4*)(cand + ((size_t)child * W + (t % W)) * W);
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const int child = tree.offset[level] + first_child + a;
-            active[a] = a < nchild;
-            if (active[a]) {
-                const float4* src = (const float4*)(cand + ((size_t)child * W + lane) * W + q * CW);
-#pragma unroll
-                for (int v = 0; v < CW / 4; ++v) {
-                    const float4 f = src[v];
-                    x[a][4 * v] = f.x;
-                    x[a][4 * v + 1] = f.y;
-                    x[a][4 * v + 2] = f.z;
-                    x[a][4 * v + 3] = f.w;
-                }
-                row[a] = cidx[child * W + lane];
-            } else {
-                row[a] = 0x7FFFFFFF;
-#pragma unroll
-                for (int c = 0; c < CW; ++c) x[a][c] = 0.f;
+            for (int v = 0; v < W / 4; ++v) {
+                const float4 f = src[v];
+                x[4 * v] = f.x;
+                x[4 * v + 1] = f.y;
+                x[4 * v + 2] = f.z;
+                x[4 * v + 3] = f.w;
             }
+            row = cidx[child * W + (t % W)];
+        } else {
+            row = 0x7FFFFFFF;
+#pragma unroll
+            for (int c = 0; c < W; ++c) x[c] = 0.f;
         }
         node = parent;
         level = parent_level;

@@ -33,6 +33,13 @@ class IrStats(ctypes.Structure):
 
 
 AL_FACTOR_FP64, AL_FACTOR_FP32, AL_FACTOR_BF16 = 0, 1, 2
+AL_BATCH_GESV, AL_BATCH_POSV = 0, 1
+
+
+class BatchEntry(ctypes.Structure):
+    """``al_batch_entry``: one system of a batched solve."""
+    _fields_ = [("a_off", ctypes.c_size_t), ("b_off", ctypes.c_size_t), ("n", ctypes.c_int32),
+                ("nrhs", ctypes.c_int32), ("kind", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 _sp = ctypes.POINTER(ctypes.c_size_t)
 _ip = ctypes.POINTER(ctypes.c_int32)
 _d, _sz, _i, _v = ctypes.c_double, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p
@@ -63,6 +70,7 @@ SIGNATURES = {
     "al_dev_dlaswp": (_i, [_sz, _v, _sz, _sz, _sz, _v, _v]),
     "al_dev_sgemm": (_i, [_i, _i, _sz, _sz, _sz, ctypes.c_float, _v, _sz, _v, _sz, ctypes.c_float, _v, _sz, _v]),
     "al_dev_gemm_bf16": (_i, [_sz, _sz, _sz, ctypes.c_float, _v, _sz, _v, _sz, ctypes.c_float, _v, _sz, _v]),
+    "al_dev_batch_solve": (_i, [_sz, _v, _v, _sz, _v, _v, _v, _i, _v]),
     "al_ir_default_options": (None, [ctypes.POINTER(IrOptions)]),
     "al_dgesv_mixed": (_i, [_dp, _sz, _dp, _sz, _sz, _sz, _i, ctypes.POINTER(IrOptions), ctypes.POINTER(IrStats)]),
     "al_dev_dgesv_mixed": (_i, [_sz, _sz, _v, _sz, _v, _sz, _v, _sz, _i, ctypes.POINTER(IrOptions),

@@ -94,6 +94,8 @@ struct Workspace {
     unsigned mgen = 0;             // last generation of the no-pivot panel flag (host side)
     void* gbuf = nullptr;          // bf16 operand staging of the device-API bf16 GEMM
     size_t gbuf_bytes = 0;
+    void* bbuf = nullptr;          // batched solver: FP32 work matrices
+    size_t bbuf_bytes = 0;
 };
 
 // The workspace of `stream` for the calling thread (created on first use).
@@ -187,6 +189,11 @@ int gemm_bf16(hipStream_t s, size_t m, size_t n, size_t k, float alpha, const __
 // Mixed-precision solve X := A^{-1} B (A, B untouched; synchronizes s).
 int gesv_mixed(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const double* A, size_t lda, const double* B,
                size_t ldb, double* X, size_t ldx, int factor, const al_ir_options* opt, al_ir_stats* stats);
+
+// Persistent batched solver (orders <= 512): see batch.hip.  queue: one device word.
+size_t batch_lds_bytes();
+int batch_solve(hipStream_t s, size_t count, const al_batch_entry* d_entries, double* A_pool, double* B_pool,
+                float* W_pool, int* d_info, int* d_steps, unsigned* queue, bool mixed, int max_iter);
 
 // Lower Cholesky in place; only the lower triangle is read or written.
 int potrf_f64(hipStream_t s, Workspace* ws, size_t n, double* A, size_t lda, int* info);

@@ -47,6 +47,7 @@ struct ThreadCtx {
             if (w.fbuf) (void)hipFree(w.fbuf);
             if (w.hnorms) (void)hipHostFree(w.hnorms);
             if (w.gbuf) (void)hipFree(w.gbuf);
+            if (w.bbuf) (void)hipFree(w.bbuf);
         }
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -593,6 +594,35 @@ int al_dev_dgesv_mixed(size_t n, size_t nrhs, const double* A, size_t lda, const
     Workspace* ws;
     AL_CHECK(workspace_for(s, &ws, 1));
     return gesv_mixed(s, ws, n, nrhs, A, lda, B, ldb, X, ldx, factor, opt, stats);
+}
+
+int al_dev_batch_solve(size_t count, const al_batch_entry* d_entries, double* A_pool, size_t a_pool_len,
+                       double* B_pool, int32_t* d_info, int32_t* d_steps, int factor, void* stream) {
+    if (factor != AL_FACTOR_FP64 && factor != AL_FACTOR_FP32) {
+        set_error("batch_solve: factor must be AL_FACTOR_FP64 or AL_FACTOR_FP32");
+        return AL_ERR_ARG;
+    }
+    if (count > (size_t)INT32_MAX) {
+        set_error("batch_solve: too many systems");
+        return AL_ERR_ARG;
+    }
+    hipStream_t s;
+    AL_CHECK(dev_stream(stream, &s));
+    if (count == 0) return AL_OK;
+    Workspace* ws;
+    AL_CHECK(workspace_for(s, &ws, 1));
+    const bool mixed = factor == AL_FACTOR_FP32;
+    const size_t need = 256 + (mixed ? a_pool_len * sizeof(float) : 0);
+    if (ws->bbuf_bytes < need) {
+        if (ws->bbuf) AL_HIP(hipFree(ws->bbuf));
+        ws->bbuf = nullptr;
+        ws->bbuf_bytes = 0;
+        AL_HIP(hipMalloc(&ws->bbuf, need));
+        ws->bbuf_bytes = need;
+    }
+    unsigned* queue = (unsigned*)ws->bbuf;
+    float* W = (float*)((char*)ws->bbuf + 256);
+    return batch_solve(s, count, d_entries, A_pool, B_pool, W, d_info, d_steps, queue, mixed, 10);
 }
 
 int al_dev_dlaswp(size_t ncols, double* A, size_t lda, size_t k0, size_t k1, const int32_t* d_ipiv, void* stream) {

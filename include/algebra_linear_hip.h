@@ -186,6 +186,36 @@ int al_dev_dgesv_mixed(size_t n, size_t nrhs, const double* A, size_t lda, const
                        size_t ldx, int factor, const al_ir_options* opt, al_ir_stats* stats, void* stream);
 
 /* ------------------------------------------------------------------------------------
+ * Batched solves (extension; the crate solves one system per call): many independent
+ * systems of orders 1..512 in one persistent launch, each a dgesv (LU with partial pivoting,
+ * lapack::dgesv, src/lapack/mod.rs:309-323) or a dposv (lower Cholesky, lapack::dpotrf,
+ * :368-399, then the two triangular solves).  Matrices live column-major in one device pool
+ * (entry i's n x n matrix at A_pool + a_off), right-hand sides in another (n x nrhs at
+ * B_pool + b_off, overwritten by the solutions); entries are a device array, processed in the
+ * order given (decreasing size balances best).  factor AL_FACTOR_FP64 factors in place in
+ * FP64; AL_FACTOR_FP32 factors an FP32 copy, refines in FP64 to a backward error of 4 eps and
+ * falls back to FP64 (in place) per system when refinement fails -- A is then left intact
+ * unless that system fell back.  d_info[i]: 0, the 1-based column of a zero pivot /
+ * non-positive diagonal (the crate's Singular), or -4 for an order above 512.  d_steps[i]
+ * (nullable): refinement steps taken, 0 in FP64 mode, -1 if the system fell back.  The call
+ * allocates nothing after its first use with a given pool size and makes no host
+ * synchronization, so it can be captured in a HIP graph.
+ * ------------------------------------------------------------------------------------ */
+enum { AL_BATCH_GESV = 0, AL_BATCH_POSV = 1 };
+
+typedef struct al_batch_entry {
+    size_t a_off;  /* element offset of the n x n matrix in A_pool */
+    size_t b_off;  /* element offset of the n x nrhs right-hand sides in B_pool */
+    int32_t n;
+    int32_t nrhs;
+    int32_t kind;  /* AL_BATCH_GESV or AL_BATCH_POSV */
+    int32_t reserved;
+} al_batch_entry;
+
+int al_dev_batch_solve(size_t count, const al_batch_entry* d_entries, double* A_pool, size_t a_pool_len,
+                       double* B_pool, int32_t* d_info, int32_t* d_steps, int factor, void* stream);
+
+/* ------------------------------------------------------------------------------------
  * Kernel instrumentation (extension): while enabled, every launch of an instrumented
  * kernel class is bracketed by HIP events on the stream it runs on, with its algorithmic
  * flops and bytes.  Classes: "dgemm_nn", "dgemm_tn", "dgemm_nt", "dgemm_tt",

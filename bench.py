@@ -19,6 +19,12 @@ second of wall time of a step.  Prints ONE JSON line; extra objects:
                   / its launches' summed duration, from HIP events around every launch on the
                   stream it runs on, during the timed steps);
   kernels      -- time per step of the instrumented kernel classes;
+  batched      -- the batched configuration: 20,000 independent systems of orders
+                  log-uniform in [8, 512] (half dgesv with A uniform(-1,1) + 0.5 n I and 1..8
+                  right-hand sides, half dposv with A = G G^T + n I), solved by one
+                  HIP-graph-captured persistent launch in FP64 and as FP32 + FP64 refinement,
+                  against the same systems solved one launch sequence per system (a bounded
+                  sample, extrapolated as solves/s);
   cpu_baseline -- the CPU oracle (a C restatement of the crate's unblocked dgesv) timed on a
                   bounded sample on one host core.
 """
@@ -58,6 +64,121 @@ def cpu_baseline(sample_n: int) -> dict:
                       f"nrhs=1, one core, {dt:.1f} s"}
 
 
+def run_batched(count: int, steps: int, warmup: int, lib, _lib, stream, sp) -> dict:
+    """The batched configuration; see the module docstring."""
+    import numpy as np
+    import torch
+
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(1988)
+    ns = np.rint(np.exp(rng.uniform(np.log(8), np.log(512), count))).astype(np.int64)
+    kinds = np.where(np.arange(count) % 2 == 0, _lib.AL_BATCH_GESV, _lib.AL_BATCH_POSV)
+    nrhs = np.where(kinds == _lib.AL_BATCH_GESV, rng.integers(1, 9, count), rng.integers(1, 9, count))
+    order = np.argsort(-ns, kind="stable")  # largest first: balances the persistent queue's tail
+    ns, kinds, nrhs = ns[order], kinds[order], nrhs[order]
+    a_off = np.concatenate([[0], np.cumsum(ns * ns)[:-1]])
+    b_off = np.concatenate([[0], np.cumsum(ns * nrhs)[:-1]])
+    a_len, b_len = int((ns * ns).sum()), int((ns * nrhs).sum())
+    g = torch.Generator(device=dev)
+    g.manual_seed(1989)
+    A0 = torch.rand(a_len, dtype=torch.float64, device=dev, generator=g) * 2.0 - 1.0
+    B0 = torch.randn(b_len, dtype=torch.float64, device=dev, generator=g)
+    for i in range(count):
+        n, o = int(ns[i]), int(a_off[i])
+        blk = A0[o:o + n * n].view(n, n)  # column-major storage: blk[j, i] = A[i, j]
+        if kinds[i] == _lib.AL_BATCH_POSV:
+            blk.copy_(blk @ blk.T + n * torch.eye(n, dtype=torch.float64, device=dev))
+        else:
+            blk.diagonal().add_(0.5 * n)
+    torch.cuda.synchronize()
+    ent = (_lib.BatchEntry * count)(*[_lib.BatchEntry(int(a_off[i]), int(b_off[i]), int(ns[i]), int(nrhs[i]),
+                                                      int(kinds[i]), 0) for i in range(count)])
+    dE = torch.from_numpy(np.frombuffer(bytes(ent), dtype=np.uint8).copy()).to(dev)
+    A = torch.empty_like(A0)
+    B = torch.empty_like(B0)
+    info = torch.zeros(count, dtype=torch.int32, device=dev)
+    nsteps = torch.zeros(count, dtype=torch.int32, device=dev)
+    flops = float(sum((2.0 / 3.0 if k == _lib.AL_BATCH_GESV else 1.0 / 3.0) * n ** 3 + 2.0 * n * n * r
+                      for n, k, r in zip(ns, kinds, nrhs)))
+    res = {"systems": count, "orders": "log-uniform [8, 512], largest first",
+           "total_flops_hpl": flops}
+    for name, factor in (("fp64", _lib.AL_FACTOR_FP64), ("fp32_ir", _lib.AL_FACTOR_FP32)):
+        def launch():
+            A.copy_(A0)
+            B.copy_(B0)
+            st = lib.al_dev_batch_solve(count, dE.data_ptr(), A.data_ptr(), a_len, B.data_ptr(), info.data_ptr(),
+                                        nsteps.data_ptr(), factor, sp)
+            if st != 0:
+                raise RuntimeError(f"batch_solve: {st} {_lib.last_error()}")
+        for _ in range(max(warmup, 1)):
+            launch()
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=stream):
+            launch()
+        times = []
+        for _ in range(steps):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            graph.replay()
+            e1.record(stream)
+            e1.synchronize()
+            times.append(e0.elapsed_time(e1))
+        inf = info.cpu().numpy()
+        st = nsteps.cpu().numpy()
+        # accuracy on a sample of systems, on the host in FP64
+        Ah, Bh, Xh = A0.cpu().numpy(), B0.cpu().numpy(), B.cpu().numpy()
+        worst = 0.0
+        for i in list(range(0, count, max(count // 40, 1)))[:40]:
+            n, r = int(ns[i]), int(nrhs[i])
+            a = Ah[a_off[i]:a_off[i] + n * n].reshape(n, n).T  # a[i, j] = A[i, j]
+            if kinds[i] == _lib.AL_BATCH_POSV:
+                a = np.tril(a) + np.tril(a, -1).T
+            x = Xh[b_off[i]:b_off[i] + n * r].reshape(r, n).T
+            b = Bh[b_off[i]:b_off[i] + n * r].reshape(r, n).T
+            rr = b - a @ x
+            den = np.abs(a).sum(axis=1).max() * np.abs(x).max(axis=0) + np.abs(b).max(axis=0)
+            worst = max(worst, float((np.abs(rr).max(axis=0) / den).max()))
+        ms = float(np.mean(times))
+        res[name] = {"ms_per_batch": round(ms, 3), "solves_per_s": round(count / (ms * 1e-3), 1),
+                     "tflops_hpl": round(flops / (ms * 1e-3) / 1e12, 3),
+                     "status_nonzero": int((inf != 0).sum()),
+                     "fell_back": int((st < 0).sum()) if factor == _lib.AL_FACTOR_FP32 else 0,
+                     "max_backward_error_sampled": worst}
+        del graph
+    # per-instance FP64: the same systems, one library launch sequence per system (a sample)
+    sample = list(range(0, count, max(count // 400, 1)))[:400]
+    Ai = torch.empty(512 * 512, dtype=torch.float64, device=dev)
+    Bi = torch.empty(512 * 8, dtype=torch.float64, device=dev)
+    ipiv = torch.empty(512, dtype=torch.int32, device=dev)
+    inf1 = torch.zeros(1, dtype=torch.int32, device=dev)
+
+    def one(i):
+        # the crate has no dposv: every system goes through its dgesv path (getrf + getrs)
+        n, r, o, ob = int(ns[i]), int(nrhs[i]), int(a_off[i]), int(b_off[i])
+        Ai[:n * n].copy_(A0[o:o + n * n])
+        Bi[:n * r].copy_(B0[ob:ob + n * r])
+        lib.al_dev_dgetrf(n, n, Ai.data_ptr(), n, ipiv.data_ptr(), inf1.data_ptr(), sp)
+        lib.al_dev_dgetrs(n, r, Ai.data_ptr(), n, ipiv.data_ptr(), Bi.data_ptr(), n, sp)
+    for i in sample[:20]:
+        one(i)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in sample:
+        one(i)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    per_solve = dt / len(sample)
+    res["per_instance_fp64"] = {"sample_systems": len(sample), "s_per_solve": per_solve,
+                                "solves_per_s": round(1.0 / per_solve, 1),
+                                "path": "al_dev_dgetrf + al_dev_dgetrs per system (copies included), every "
+                                        f"{max(count // 400, 1)}th system of the batch"}
+    res["speedup_fp64_batched_vs_per_instance"] = round(res["fp64"]["solves_per_s"] / (1.0 / per_solve), 2)
+    res["speedup_fp32_ir_batched_vs_per_instance"] = round(res["fp32_ir"]["solves_per_s"] / (1.0 / per_solve), 2)
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +190,7 @@ def main() -> int:
     ap.add_argument("--form", choices=["fp32", "bf16"], default="fp32")
     ap.add_argument("--also", default="bf16,fp64", help="other forms measured beside the headline")
     ap.add_argument("--cpu-sample-n", type=int, default=1536)
+    ap.add_argument("--batch", type=int, default=20000, help="systems in the batched leg (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -218,6 +340,10 @@ def main() -> int:
     }
     if "fp64" in forms:
         out["speedup_vs_fp64"] = round(forms["fp64"]["ms_per_step"] / head["ms_per_step"], 3)
+    del At, B0t
+    torch.cuda.empty_cache()
+    if args.batch > 0:
+        out["batched"] = run_batched(args.batch, args.steps, args.warmup, lib, _lib, stream, sp)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample_n)
     print(json.dumps(out), flush=True)

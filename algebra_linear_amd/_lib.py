@@ -36,6 +36,20 @@ AL_FACTOR_FP64, AL_FACTOR_FP32, AL_FACTOR_BF16 = 0, 1, 2
 AL_BATCH_GESV, AL_BATCH_POSV = 0, 1
 
 
+class Request(ctypes.Structure):
+    """``al_request``: one request to the solve server."""
+    _fields_ = [("kind", ctypes.c_int32), ("factor", ctypes.c_int32), ("n", ctypes.c_size_t),
+                ("nrhs", ctypes.c_size_t), ("a", ctypes.c_void_p), ("b", ctypes.c_void_p), ("x", ctypes.c_void_p),
+                ("factor_id", ctypes.c_int64), ("keep", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class Result(ctypes.Structure):
+    """``al_result``: what the solve server did with a request."""
+    _fields_ = [("status", ctypes.c_int32), ("steps", ctypes.c_int32), ("factor_id", ctypes.c_int64),
+                ("batch_size", ctypes.c_int32), ("reserved", ctypes.c_int32), ("queue_ms", ctypes.c_double),
+                ("service_ms", ctypes.c_double)]
+
+
 class BatchEntry(ctypes.Structure):
     """``al_batch_entry``: one system of a batched solve."""
     _fields_ = [("a_off", ctypes.c_size_t), ("b_off", ctypes.c_size_t), ("n", ctypes.c_int32),
@@ -71,6 +85,11 @@ SIGNATURES = {
     "al_dev_sgemm": (_i, [_i, _i, _sz, _sz, _sz, ctypes.c_float, _v, _sz, _v, _sz, ctypes.c_float, _v, _sz, _v]),
     "al_dev_gemm_bf16": (_i, [_sz, _sz, _sz, ctypes.c_float, _v, _sz, _v, _sz, ctypes.c_float, _v, _sz, _v]),
     "al_dev_batch_solve": (_i, [_sz, _v, _v, _sz, _v, _v, _v, _i, _v]),
+    "al_server_create": (_v, [_sz, _i]),
+    "al_server_submit": (_i, [_v, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]),
+    "al_server_wait": (_i, [_v, ctypes.c_uint64, ctypes.POINTER(Result)]),
+    "al_server_release_factor": (_i, [_v, ctypes.c_int64]),
+    "al_server_destroy": (None, [_v]),
     "al_ir_default_options": (None, [ctypes.POINTER(IrOptions)]),
     "al_dgesv_mixed": (_i, [_dp, _sz, _dp, _sz, _sz, _sz, _i, ctypes.POINTER(IrOptions), ctypes.POINTER(IrStats)]),
     "al_dev_dgesv_mixed": (_i, [_sz, _sz, _v, _sz, _v, _sz, _v, _sz, _i, ctypes.POINTER(IrOptions),

@@ -186,9 +186,22 @@ int bf16_cols(hipStream_t s, size_t k, size_t kp, size_t n, const float* U, size
 int gemm_bf16(hipStream_t s, size_t m, size_t n, size_t k, float alpha, const __bf16* At, const __bf16* Bt, float beta,
               float* C, size_t ldc);
 
-// Mixed-precision solve X := A^{-1} B (A, B untouched; synchronizes s).
+// A low-precision factorization kept for reuse: the FP32 LU (np x np, np = n rounded up to a
+// multiple of 64) and its row origins.
+struct MixedFactor {
+    float* F = nullptr;
+    double* perm = nullptr;
+    size_t n = 0, np = 0;
+    int breakdown = 0;  // 1-based column of an exactly zero pivot, or 0
+};
+inline size_t mixed_padded(size_t n) { return (n + 63) / 64 * 64; }
+
+// Mixed-precision solve X := A^{-1} B (A, B untouched; synchronizes s).  With `reuse` the
+// factorization is taken from it instead of computed; with `keep` (buffers allocated by the
+// caller for mixed_padded(n)) a computed factorization is copied there.
 int gesv_mixed(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const double* A, size_t lda, const double* B,
-               size_t ldb, double* X, size_t ldx, int factor, const al_ir_options* opt, al_ir_stats* stats);
+               size_t ldb, double* X, size_t ldx, int factor, const al_ir_options* opt, al_ir_stats* stats,
+               const MixedFactor* reuse = nullptr, MixedFactor* keep = nullptr);
 
 // Persistent batched solver (orders <= 512): see batch.hip.  queue: one device word.
 size_t batch_lds_bytes();

@@ -461,7 +461,8 @@ int fallback_fp64(hipStream_t s, Workspace* ws, Arena& a, size_t n, size_t nrhs,
 }  // namespace
 
 int gesv_mixed(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const double* A, size_t lda, const double* B,
-               size_t ldb, double* X, size_t ldx, int factor, const al_ir_options* opt_in, al_ir_stats* st) {
+               size_t ldb, double* X, size_t ldx, int factor, const al_ir_options* opt_in, al_ir_stats* st,
+               const MixedFactor* reuse, MixedFactor* keep) {
     al_ir_options opt;
     al_ir_default_options(&opt);
     if (opt_in) {
@@ -487,8 +488,16 @@ int gesv_mixed(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const double
     const bool bf16 = factor == AL_FACTOR_BF16;
     const size_t np = (n + PB - 1) / PB * PB;
     const size_t nr_max = nrhs < (size_t)NR_MAX ? nrhs : (size_t)NR_MAX;
+    if (reuse && (reuse->n != n || reuse->np != np || factor == AL_FACTOR_FP64)) {
+        set_error("dgesv_mixed: the resident factor does not match the system");
+        return AL_ERR_ARG;
+    }
     Arena a;
-    AL_CHECK(make_arena(s, ws, n, np, nr_max, bf16 ? 512 : 256, factor != AL_FACTOR_FP64, bf16, &a));
+    AL_CHECK(make_arena(s, ws, n, np, nr_max, bf16 ? 512 : 256, factor != AL_FACTOR_FP64 && !reuse, bf16, &a));
+    if (reuse) {
+        a.F = reuse->F;
+        a.perm = reuse->perm;
+    }
     if (!ws->hnorms) AL_HIP(hipHostMalloc((void**)&ws->hnorms, NORM_WORDS * sizeof(u64), hipHostMallocDefault));
     u64* hn = (u64*)ws->hnorms;
 
@@ -508,16 +517,27 @@ int gesv_mixed(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const double
     bool ok_low = false;
     int iters = 0;
     if (factor != AL_FACTOR_FP64) {
-        hipLaunchKernelGGL(to_f32_pad_k, dim3(ceil_div(np * np, 256)), dim3(256), 0, s, (int)n, (int)np, A, lda, a.norms,
-                           a.F);
-        AL_HIP(hipGetLastError());
-        AL_CHECK(getrf_lowp(s, ws, a, bf16));
         int minfo = 0;
-        AL_HIP(hipMemcpyAsync(&minfo, a.minfo, sizeof minfo, hipMemcpyDeviceToHost, s));
-        AL_HIP(hipStreamSynchronize(s));
-        if (minfo < 0) {
-            set_error("dgesv_mixed: a device-side wait timed out in the FP32 factorization");
-            return AL_ERR_DEVICE;
+        if (reuse) {
+            minfo = reuse->breakdown;
+        } else {
+            hipLaunchKernelGGL(to_f32_pad_k, dim3(ceil_div(np * np, 256)), dim3(256), 0, s, (int)n, (int)np, A, lda,
+                               a.norms, a.F);
+            AL_HIP(hipGetLastError());
+            AL_CHECK(getrf_lowp(s, ws, a, bf16));
+            AL_HIP(hipMemcpyAsync(&minfo, a.minfo, sizeof minfo, hipMemcpyDeviceToHost, s));
+            AL_HIP(hipStreamSynchronize(s));
+            if (minfo < 0) {
+                set_error("dgesv_mixed: a device-side wait timed out in the FP32 factorization");
+                return AL_ERR_DEVICE;
+            }
+            if (keep && keep->F && keep->perm) {
+                AL_HIP(hipMemcpyAsync(keep->F, a.F, np * np * sizeof(float), hipMemcpyDeviceToDevice, s));
+                AL_HIP(hipMemcpyAsync(keep->perm, a.perm, np * sizeof(double), hipMemcpyDeviceToDevice, s));
+                keep->n = n;
+                keep->np = np;
+                keep->breakdown = minfo;
+            }
         }
         stats.factor_breakdown = minfo;
         if (minfo == 0) {

@@ -216,6 +216,48 @@ int al_dev_batch_solve(size_t count, const al_batch_entry* d_entries, double* A_
                        double* B_pool, int32_t* d_info, int32_t* d_steps, int factor, void* stream);
 
 /* ------------------------------------------------------------------------------------
+ * Solve server (extension): continuous batching of independent requests on one device.
+ * Requests (dgesv or dposv, FP64 or FP32 + FP64 refinement, any order up to 32704, any
+ * number of right-hand sides) are admitted in arrival order while their HBM footprint fits
+ * the budget, run concurrently on `slots` worker streams, and small ones (order <= 512, no
+ * resident factorization) waiting together are solved by one batched launch.  A request may
+ * keep its factorization resident (its footprint stays reserved until released) and later
+ * requests may solve new right-hand sides with it.  Host buffers a, b, x must stay valid until
+ * al_server_wait returns for the request.  Results are bit-for-bit the same alone or in a mix.
+ * Do not release a factorization while requests that use it are in flight.
+ * ------------------------------------------------------------------------------------ */
+typedef struct al_server al_server;
+
+typedef struct al_request {
+    int32_t kind;       /* AL_BATCH_GESV or AL_BATCH_POSV */
+    int32_t factor;     /* AL_FACTOR_FP64, or AL_FACTOR_FP32 (FP32 LU + FP64 refinement, both kinds) */
+    size_t n;
+    size_t nrhs;
+    const double* a;    /* n x n column-major (unused when factor_id >= 0) */
+    const double* b;    /* n x nrhs */
+    double* x;          /* n x nrhs: receives the solution */
+    int64_t factor_id;  /* >= 0: solve with this resident factorization; -1: factor a */
+    int32_t keep;       /* nonzero: keep this request's factorization resident */
+    int32_t reserved;
+} al_request;
+
+typedef struct al_result {
+    int32_t status;     /* AL_OK, AL_SINGULAR, or a negative error */
+    int32_t steps;      /* refinement steps (FP32 mode), -1 if it fell back to FP64, 0 in FP64 mode */
+    int64_t factor_id;  /* the factorization this request kept resident, or -1 */
+    int32_t batch_size; /* requests solved by the same launch as this one */
+    int32_t reserved;
+    double queue_ms;    /* submission to start of service */
+    double service_ms;  /* start to end of service */
+} al_result;
+
+al_server* al_server_create(size_t hbm_budget_bytes, int slots);
+int al_server_submit(al_server* server, const al_request* request, uint64_t* ticket);
+int al_server_wait(al_server* server, uint64_t ticket, al_result* result);
+int al_server_release_factor(al_server* server, int64_t factor_id);
+void al_server_destroy(al_server* server);
+
+/* ------------------------------------------------------------------------------------
  * Kernel instrumentation (extension): while enabled, every launch of an instrumented
  * kernel class is bracketed by HIP events on the stream it runs on, with its algorithmic
  * flops and bytes.  Classes: "dgemm_nn", "dgemm_tn", "dgemm_nt", "dgemm_tt",

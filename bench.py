@@ -25,6 +25,13 @@ second of wall time of a step.  Prints ONE JSON line; extra objects:
                   HIP-graph-captured persistent launch in FP64 and as FP32 + FP64 refinement,
                   against the same systems solved one launch sequence per system (a bounded
                   sample, extrapolated as solves/s);
+  served       -- the serving configuration: Poisson arrivals at swept rates of dgesv
+                  requests, orders log-uniform in [32, 4096], 1..8 right-hand sides, kappa
+                  log-uniform in [1e2, 1e8] (matrices as above, pre-generated in a pool), 30% of
+                  the requests solving new right-hand sides with a resident factorization; the
+                  continuous-batching server in FP64 and in FP32 + FP64 refinement, latency
+                  (arrival to result on the host) p50/p99 and solves/s per rate, and the same
+                  request stream solved one drop-in call at a time;
   cpu_baseline -- the CPU oracle (a C restatement of the crate's unblocked dgesv) timed on a
                   bounded sample on one host core.
 """
@@ -179,6 +186,123 @@ def run_batched(count: int, steps: int, warmup: int, lib, _lib, stream, sp) -> d
     return res
 
 
+def run_served(per_rate: int, lib, _lib) -> dict:
+    """The serving configuration; see the module docstring."""
+    import threading
+
+    import numpy as np
+    import torch
+
+    from algebra_linear_amd import serve
+
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(1990)
+    g = torch.Generator(device=dev)
+    g.manual_seed(1990)
+    pool = []
+    for n in np.rint(np.exp(rng.uniform(np.log(32), np.log(4096), 40))).astype(int):
+        kappa = float(np.exp(rng.uniform(np.log(1e2), np.log(1e8))))
+        q1, r1 = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device=dev, generator=g))
+        q2, r2 = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device=dev, generator=g))
+        U = q1 * torch.sign(torch.diagonal(r1))
+        V = q2 * torch.sign(torch.diagonal(r2))
+        sig = kappa ** (-torch.arange(n, dtype=torch.float64, device=dev) / max(n - 1, 1))
+        a = ((U * sig) @ V.T).T.contiguous().cpu().numpy().reshape(-1)  # column-major
+        pool.append((int(n), kappa, a))
+    torch.cuda.empty_cache()
+
+    def stream_of(count, seed):
+        r = np.random.default_rng(seed)
+        reqs = []
+        for _ in range(count):
+            idx = int(r.integers(len(pool)))
+            nrhs = int(r.integers(1, 9))
+            reuse = bool(r.uniform() < 0.3)
+            b = r.standard_normal(pool[idx][0] * nrhs)
+            reqs.append((idx, nrhs, reuse, b))
+        return reqs
+
+    res = {"pool": {"matrices": len(pool), "orders": "log-uniform [32, 4096]", "kappa": "log-uniform [1e2, 1e8]"},
+           "reuse_fraction": 0.3}
+    rates = [20.0, 60.0, 200.0, float("inf")]
+    for mode, factor in (("fp64", serve.FP64), ("fp32_ir", serve.FP32)):
+        per = {}
+        with serve.Server(hbm_budget_bytes=64 << 30, slots=4) as srv:
+            # resident factorizations of 8 pool matrices (set up before the timed stream)
+            resident = {}
+            for idx in range(8):
+                n, _, a = pool[idx]
+                _, r0 = srv.wait(srv.submit(a, np.zeros(n), n, 1, serve.GESV, factor, keep=True))
+                resident[idx] = r0["factor_id"]
+            for ri, rate in enumerate(rates):
+                reqs = stream_of(per_rate, 100 + ri)
+                arr = np.cumsum(np.random.default_rng(200 + ri).exponential(1.0 / rate, per_rate)) \
+                    if np.isfinite(rate) else np.zeros(per_rate)
+                tickets = [None] * per_rate
+                t0 = time.perf_counter()
+                for i, (idx, nrhs, reuse, b) in enumerate(reqs):
+                    delay = t0 + arr[i] - time.perf_counter()
+                    if delay > 0:
+                        time.sleep(delay)
+                    if reuse:
+                        fidx = idx % 8
+                        n = pool[fidx][0]
+                        tickets[i] = srv.submit(None, b[:n * nrhs] if b.size >= n * nrhs else
+                                                np.resize(b, n * nrhs), n, nrhs, serve.GESV, factor,
+                                                factor_id=resident[fidx])
+                    else:
+                        n, _, a = pool[idx]
+                        tickets[i] = srv.submit(a, b, n, nrhs, serve.GESV, factor)
+                lat, ok, steps, fb = [], 0, [], 0
+                for t in tickets:
+                    _, r = srv.wait(t)
+                    lat.append(r["queue_ms"] + r["service_ms"])
+                    ok += r["status"] == 0
+                    if r["steps"] > 0:
+                        steps.append(r["steps"])
+                    fb += r["steps"] < 0
+                wall = time.perf_counter() - t0
+                per["saturated" if not np.isfinite(rate) else f"{rate:g}/s"] = {
+                    "requests": per_rate, "ok": int(ok), "solves_per_s": round(per_rate / wall, 2),
+                    "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                    "mean_refinement_steps": round(float(np.mean(steps)), 2) if steps else 0, "fell_back": int(fb)}
+            for fid in resident.values():
+                srv.release(fid)
+        res[mode] = per
+    # the same (saturated) stream solved one drop-in call at a time on the host-slice API
+    reqs = stream_of(per_rate, 100 + len(rates) - 1)
+    lu_cache = {}
+    t0 = time.perf_counter()
+    for idx, nrhs, reuse, b in reqs:
+        if reuse:
+            fidx = idx % 8
+            n, _, a = pool[fidx]
+            if fidx not in lu_cache:
+                lu = a.copy()
+                ipiv = (ctypes.c_size_t * n)()
+                lib.al_dgetrf(lu.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), lu.size, n, ipiv, n,
+                              ctypes.byref(ctypes.c_double()))
+                lu_cache[fidx] = (lu, ipiv)
+            lu, ipiv = lu_cache[fidx]
+            bb = np.resize(b, n * nrhs).copy()
+            lib.al_dgetrs_nrhs(lu.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), lu.size, ipiv, n,
+                               bb.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), bb.size, n, nrhs)
+        else:
+            n, _, a = pool[idx]
+            aa, bb = a.copy(), b.copy()
+            lib.al_dgesv_nrhs(aa.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), aa.size,
+                              bb.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), bb.size, n, nrhs, None)
+    wall = time.perf_counter() - t0
+    res["per_instance_fp64"] = {"requests": per_rate, "solves_per_s": round(per_rate / wall, 2),
+                                "path": "al_dgesv_nrhs (al_dgetrs_nrhs on a host-held LU for reuse requests), "
+                                        "one call at a time"}
+    res["speedup_fp64_served_vs_per_instance"] = round(res["fp64"]["saturated"]["solves_per_s"] /
+                                                       res["per_instance_fp64"]["solves_per_s"], 2)
+    res["speedup_fp32_ir_served_vs_per_instance"] = round(res["fp32_ir"]["saturated"]["solves_per_s"] /
+                                                          res["per_instance_fp64"]["solves_per_s"], 2)
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -191,6 +315,7 @@ def main() -> int:
     ap.add_argument("--also", default="bf16,fp64", help="other forms measured beside the headline")
     ap.add_argument("--cpu-sample-n", type=int, default=1536)
     ap.add_argument("--batch", type=int, default=20000, help="systems in the batched leg (0: skip)")
+    ap.add_argument("--served", type=int, default=120, help="requests per rate in the served leg (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -344,6 +469,8 @@ def main() -> int:
     torch.cuda.empty_cache()
     if args.batch > 0:
         out["batched"] = run_batched(args.batch, args.steps, args.warmup, lib, _lib, stream, sp)
+    if args.served > 0:
+        out["served"] = run_served(args.served, lib, _lib)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample_n)
     print(json.dumps(out), flush=True)

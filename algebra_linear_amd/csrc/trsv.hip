@@ -4,8 +4,8 @@
 // A blocked substitution launched per diagonal block costs two launches per 64 rows; at
 // n = 16384 that is 512 launches whose diagonal solves are each a long dependency chain.
 // Here one workgroup owns one 64-row block.  It accumulates y = b - sum_c A[b,c] x_c over the
-// column blocks c in solve order (matrix reads coalesced down columns and prefetched one block
-// ahead of the wait), reduces the four partial sums in a fixed order, solves its diagonal block
+// column blocks c in solve order (matrix reads coalesced down columns, the next block's
+// requested while the current one is summed), reduces the four partial sums in a fixed order, solves its diagonal block
 // with one wave (lane = row; x_i broadcast by readlane), writes x_b over b and raises the
 // block's flag with a release store.  Workgroups are dispatched in increasing id and a block
 // waits only on blocks of smaller id, so the smallest unfinished block is always resident:
@@ -75,7 +75,6 @@ __global__ __launch_bounds__(256) void trsv_blocked(int n, int nrhs, const TA* _
         double a_cur[16];
 #pragma unroll
         for (int jj = 0; jj < 16; ++jj) a_cur[jj] = a_next[jj];
-        if (q + 1 < ndep) load_a(q + 1, a_next);
         const int c = LOWER ? q : nblk - 1 - q;
         if (t == 0) {
             int good = 1;
@@ -99,6 +98,9 @@ __global__ __launch_bounds__(256) void trsv_blocked(int n, int nrhs, const TA* _
         ok = ok_lds != 0;
         __syncthreads();
         if (!ok) break;
+        // the next block's matrix part is requested only now: the acquire fence above waits for
+        // every load in flight, so a prefetch issued before it would serialize with the wait
+        if (q + 1 < ndep) load_a(q + 1, a_next);
         const int c0 = c * TB + part * 16;
 #pragma unroll
         for (int k = 0; k < NR; ++k) {

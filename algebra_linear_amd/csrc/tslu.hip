@@ -66,30 +66,27 @@ __device__ __forceinline__ float lane_value(float v, int lane) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-// Step J of the elimination, inside the block of columns [C0, C0 + 8): every wave finds its
+// Step J of the elimination, inside the block of columns [C0, C0 + KB): every wave finds its
 // first maximum of |x[J]| over its active rows (DPP max, then the lowest lane holding it) and
-// publishes it with that row's block values; after one barrier every thread takes the first
-// maximum over the waves (ties to the lower wave, i.e. the lower row), and the active rows
-// eliminate column J inside the block.  A NaN counts as the largest magnitude.
+// that lane publishes it with its row's block values; after one barrier every thread takes the
+// first maximum over the waves (ties to the lower wave, i.e. the lower row), and the active rows
+// eliminate column J inside the block.  A NaN counts as the largest magnitude.  The step is
+// one dependency chain, so it is kept free of branches: the multiplier is x[J] times the
+// pivot's reciprocal (v_rcp_f32, 1 ulp) and rows that do not update select their old values.
 template <int C0, int J>
 __device__ __forceinline__ void narrow_step(float (&x)[W], bool& active, int row, int& step, TsluLds& L, int lane,
                                             int wave) {
     constexpr int b = J & 1;
+    constexpr int NV = C0 + KB - J;  // block columns from J on
     const float m = !active ? -1.f : (x[J] != x[J] ? INFINITY : fabsf(x[J]));
     const float wm = wave_max_dpp(m);
-    const unsigned long long bal = __ballot(m == wm);
-    int wl = __ffsll((long long)bal) - 1;
-    wl = wl < 0 ? 0 : wl;
-    if (lane == 0) {
+    const unsigned long long bal = __ballot(m == wm);  // never empty: some lane holds the maximum
+    const int wl = __ffsll((long long)bal) - 1;
+    if (lane == wl) {
         L.mag[b][wave] = wm;
-        L.crow[b][wave] = __builtin_amdgcn_readlane(row, wl);
-    }
-    if (lane < KB - (J - C0)) {
-        // the candidate's values in columns J .. C0 + 7, one per lane
-        float v = 0.f;
+        L.crow[b][wave] = row;
 #pragma unroll
-        for (int c = J; c < C0 + KB; ++c) v = lane == c - J ? lane_value(x[c], wl) : v;
-        L.cval[b][wave][lane] = v;
+        for (int c = 0; c < NV; ++c) L.cval[b][wave][c] = x[J + c];
     }
     __syncthreads();
     int best = 0;
@@ -102,32 +99,38 @@ __device__ __forceinline__ void narrow_step(float (&x)[W], bool& active, int row
     }
     const int prow = L.crow[b][best];
     const float piv = L.cval[b][best][0];
-    if (active && row == prow) {
-        active = false;
-        step = J;
-    }
-    if (active) {
-        const float l = x[J] / piv;
-        x[J] = l;
+    float u[NV];
 #pragma unroll
-        for (int c = J + 1; c < C0 + KB; ++c) x[c] = fmaf(-l, L.cval[b][best][c - J], x[c]);
+    for (int c = 1; c < NV; ++c) u[c] = L.cval[b][best][c];
+    const float r = __builtin_amdgcn_rcpf(piv);
+    const bool is_piv = active && row == prow;
+    const bool upd = active && !is_piv;
+    const float l = x[J] * r;
+    x[J] = upd ? l : x[J];
+#pragma unroll
+    for (int c = 1; c < NV; ++c) {
+        const float v = fmaf(-l, u[c], x[J + c]);
+        x[J + c] = upd ? v : x[J + c];
     }
+    step = is_piv ? J : step;
+    active = upd;
     if (threadIdx.x == 0) {
         L.piv[J] = piv;
         L.winners[J] = prow;
     }
 }
 
-// After the 8 steps of block [C0, C0 + 8): the pivot rows' entries right of the block become
+// After the KB steps of block [C0, C0 + KB): the pivot rows' entries right of the block become
 // U12 = L11^{-1} A12 (one thread per column), every other active row subtracts its 8
 // multipliers times U12, and the pivot rows take their U12 rows.
+// No barrier is needed before the pivot rows write ur and l11: every thread last read ur in
+// the previous block's update, and each narrow step since then ends at a barrier.
 template <int C0>
 __device__ __forceinline__ void block_update(float (&x)[W], bool active, int step, TsluLds& L) {
     constexpr int REST = W - C0 - KB;
     if (REST == 0) return;
     const int k = step - C0;
     const bool piv_here = step >= C0 && step < C0 + KB;
-    __syncthreads();  // every thread is done with the block's step buffers
     if (piv_here) {
 #pragma unroll
         for (int c = 0; c < REST; ++c) L.ur[k][c] = x[C0 + KB + c];

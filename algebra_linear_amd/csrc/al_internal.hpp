@@ -131,6 +131,11 @@ int gemm_f64(hipStream_t s, bool ta, bool tb, size_t m, size_t n, size_t k, doub
 // The same on FP32 matrix cores (v_mfma_f32_16x16x4_f32), FP32 accumulation.
 int gemm_f32(hipStream_t s, bool ta, bool tb, size_t m, size_t n, size_t k, float alpha, const float* A, size_t lda,
              const float* B, size_t ldb, float beta, float* C, size_t ldc, int tri = 0);
+// FP32 untransposed GEMM with 16-byte accesses (gemm_f32 routes to it when sgemm_nn_fits).
+bool sgemm_nn_fits(size_t m, size_t k, const float* A, size_t lda, const float* B, size_t ldb, const float* C,
+                   size_t ldc);
+int sgemm_nn(hipStream_t s, int m, int n, int k, float alpha, const float* A, size_t lda, const float* B, size_t ldb,
+             float beta, float* C, size_t ldc);
 
 // Triangular solve op(T) X = alpha B (left) or X op(T) = alpha B (right), in place in B.
 // trans selects op(T) = T^T (used by the Cholesky panel); `info` (nullable) turns the call

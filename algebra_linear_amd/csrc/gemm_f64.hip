@@ -276,6 +276,10 @@ int gemm_f64(hipStream_t s, bool ta, bool tb, size_t m, size_t n, size_t k, doub
 
 int gemm_f32(hipStream_t s, bool ta, bool tb, size_t m, size_t n, size_t k, float alpha, const float* A, size_t lda,
              const float* B, size_t ldb, float beta, float* C, size_t ldc, int tri) {
+    // the 16-byte-access kernel takes the untransposed product when the shapes allow it
+    if (!ta && !tb && tri == 0 && m > 0 && n > 0 && k > 0 && alpha != 0.f && m <= INT32_MAX && n <= INT32_MAX &&
+        k <= INT32_MAX && sgemm_nn_fits(m, k, A, lda, B, ldb, C, ldc))
+        return sgemm_nn(s, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc);
     return gemm_t<float>(s, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri);
 }
 

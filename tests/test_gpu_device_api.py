@@ -183,3 +183,32 @@ def test_sgemm_fp32_matrix_cores(ta, tb, m, n, k):
     exact = 1.5 * (Aop.astype(np.float64) @ Bop.astype(np.float64)) - 0.5 * C0
     bound = 2 * (k + 2) * 2.0 ** -24 * (1.5 * (np.abs(Aop).astype(np.float64) @ np.abs(Bop)) + 0.5 * np.abs(C0))
     assert np.all(np.abs(got - exact) <= bound)
+
+
+@pytest.mark.parametrize("beta", [0.0, -0.5])
+@pytest.mark.parametrize("m,n,k,pad", [(132, 70, 100, 4), (512, 257, 384, 0), (4, 1, 4, 8), (260, 130, 36, 12),
+                                       (16320, 64, 64, 64)])
+def test_sgemm_untransposed_16_byte_path(m, n, k, pad, beta):
+    # shapes with m % 4 == 0, k % 4 == 0 and leading dimensions that are multiples of 4 take the
+    # float4 kernel; edge tiles in every dimension and K not a multiple of its 32-step
+    dev = _dev()
+    rng = np.random.default_rng(m + 3 * n + 7 * k)
+    A = rng.uniform(-1, 1, (m, k)).astype(np.float32)
+    B = rng.uniform(-1, 1, (k, n)).astype(np.float32)
+    C0 = rng.uniform(-1, 1, (m, n)).astype(np.float32)
+    lda, ldb, ldc = m + pad, k + pad, m + pad
+    dA, dB, dC = _f32_dev(A, lda, dev), _f32_dev(B, ldb, dev), _f32_dev(C0, ldc, dev)
+    c_before = dC.cpu().numpy()
+    rc = LIB.al_dev_sgemm(0, 0, m, n, k, 1.5, dA.data_ptr(), lda, dB.data_ptr(), ldb, beta, dC.data_ptr(), ldc, None)
+    assert rc == 0, _lib.last_error()
+    torch.cuda.synchronize()
+    got = _from_dev(dC, m, n, ldc).astype(np.float64)
+    exact = 1.5 * (A.astype(np.float64) @ B.astype(np.float64)) + beta * C0
+    bound = 2 * (k + 2) * 2.0 ** -24 * (1.5 * (np.abs(A).astype(np.float64) @ np.abs(B)) + abs(beta) * np.abs(C0))
+    assert np.all(np.abs(got - exact) <= bound)
+    # the padding rows between columns are untouched
+    after = dC.cpu().numpy()
+    mask = np.ones_like(after, dtype=bool)
+    for j in range(n):
+        mask[j * ldc:j * ldc + m] = False
+    np.testing.assert_array_equal(after[mask], c_before[mask])

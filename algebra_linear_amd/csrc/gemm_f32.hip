@@ -44,21 +44,40 @@ __global__ __launch_bounds__(NT, 2) void sgemm_nn_k(int m, int n, int k, float a
     // staging: float4 number f = t + 256 j of a tile.  A: column f >> 5, rows 4 (f & 31) ..;
     // B: column f >> 3, k 4 (f & 7) ..
     f4 ra[4], rb[4];
+    // out-of-range float4s read the operand's first element (always valid) and are replaced
+    // by zeros, so the loads carry no branches and issue back to back
     auto load = [&](int k0) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int f = t + NT * j;
             const int p = f >> 5, r = 4 * (f & 31);
             const bool ok = (bm + r) < m && (k0 + p) < k;
-            ra[j] = ok ? *(const f4*)(A + (size_t)(bm + r) + (size_t)(k0 + p) * lda) : f4{0.f, 0.f, 0.f, 0.f};
+            const size_t off = ok ? (size_t)(bm + r) + (size_t)(k0 + p) * lda : 0;
+            const f4 v = *(const f4*)(A + off);
+            ra[j] = ok ? v : f4{0.f, 0.f, 0.f, 0.f};
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int f = t + NT * j;
             const int col = f >> 3, kk = 4 * (f & 7);
             const bool ok = (bn + col) < n && (k0 + kk) < k;
-            rb[j] = ok ? *(const f4*)(B + (size_t)(k0 + kk) + (size_t)(bn + col) * ldb) : f4{0.f, 0.f, 0.f, 0.f};
+            const size_t off = ok ? (size_t)(k0 + kk) + (size_t)(bn + col) * ldb : 0;
+            const f4 v = *(const f4*)(B + off);
+            rb[j] = ok ? v : f4{0.f, 0.f, 0.f, 0.f};
         }
+    };
+    // C (when beta != 0) is requested during the last K step, all sixteen float4s at once
+    f4 cv[4][4];
+    auto load_c = [&]() {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int col = bn + wn * 64 + 4 * c + j, row = bm + wm * 64 + 16 * q + 4 * r;
+                const bool ok = col < n && row < m;
+                const f4 v = *(const f4*)(C + (ok ? (size_t)row + (size_t)col * ldc : 0));
+                cv[j][r] = ok ? v : f4{0.f, 0.f, 0.f, 0.f};
+            }
     };
     auto store = [&](float* As, float* Bs) {
 #pragma unroll
@@ -83,13 +102,7 @@ __global__ __launch_bounds__(NT, 2) void sgemm_nn_k(int m, int n, int k, float a
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 
-    const int ktiles = (k + BK - 1) / BK;
-    load(0);
-    store(lds, lds + 2 * TILE);
-    __syncthreads();
-    for (int kt = 0; kt < ktiles; ++kt) {
-        const bool more = (kt + 1) < ktiles;
-        if (more) load((kt + 1) * BK);
+    auto mfma_step = [&](int kt) {
         const float* As = lds + (kt & 1) * TILE;
         const float* Bs = lds + 2 * TILE + (kt & 1) * TILE;
 #pragma unroll
@@ -103,9 +116,21 @@ __global__ __launch_bounds__(NT, 2) void sgemm_nn_k(int m, int n, int k, float a
                 for (int j = 0; j < 4; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i], bv[j], acc[i][j], 0, 0, 0);
         }
-        if (more) store(lds + ((kt + 1) & 1) * TILE, lds + 2 * TILE + ((kt + 1) & 1) * TILE);
+    };
+
+    const int ktiles = (k + BK - 1) / BK;
+    load(0);
+    store(lds, lds + 2 * TILE);
+    __syncthreads();
+    for (int kt = 0; kt + 1 < ktiles; ++kt) {
+        load((kt + 1) * BK);
+        mfma_step(kt);
+        store(lds + ((kt + 1) & 1) * TILE, lds + 2 * TILE + ((kt + 1) & 1) * TILE);
         __syncthreads();
     }
+    // the last K step, peeled so C's registers are live only here
+    if (beta != 0.f) load_c();
+    mfma_step(ktiles - 1);
 
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -115,10 +140,9 @@ __global__ __launch_bounds__(NT, 2) void sgemm_nn_k(int m, int n, int k, float a
         for (int r = 0; r < 4; ++r) {
             const int row = bm + wm * 64 + 16 * q + 4 * r;
             if (row >= m) continue;
-            f4* cp = (f4*)(C + (size_t)row + (size_t)col * ldc);
             f4 v = f4{acc[0][j][r], acc[1][j][r], acc[2][j][r], acc[3][j][r]} * alpha;
-            if (beta != 0.f) v += beta * *cp;
-            *cp = v;
+            if (beta != 0.f) v += beta * cv[j][r];
+            *(f4*)(C + (size_t)row + (size_t)col * ldc) = v;
         }
     }
 }

@@ -13,6 +13,8 @@
 //
 // The matrix may be FP64 or FP32 (the low-precision factors of the mixed-precision solver);
 // right-hand sides and all accumulation are FP64.
+#include <type_traits>
+
 #include "al_internal.hpp"
 
 namespace al {
@@ -62,11 +64,25 @@ __global__ __launch_bounds__(256) void trsv_blocked(int n, int nrhs, const TA* _
     // before waiting for its flag, so only the x_c reads remain after the wait.
     const int ndep = LOWER ? b : nblk - 1 - b;
     const bool rok = (r0 + row) < n;
+    const int row_c = rok ? r0 + row : 0;  // rows past n read row 0 and are replaced by zeros
     auto load_a = [&](int q, double (&av)[16]) {
         const int c0 = (LOWER ? q : nblk - 1 - q) * TB + part * 16;
+        if (c0 + 16 <= n) {
+            const TA* p = A + (size_t)row_c + (size_t)c0 * lda;
 #pragma unroll
-        for (int jj = 0; jj < 16; ++jj)
-            av[jj] = (rok && c0 + jj < n) ? (double)A[(size_t)(r0 + row) + (size_t)(c0 + jj) * lda] : 0.0;
+            for (int jj = 0; jj < 16; ++jj) {
+                const double v = (double)*p;
+                av[jj] = rok ? v : 0.0;
+                p += lda;
+            }
+        } else {
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) {
+                const bool okc = c0 + jj < n;
+                const double v = (double)A[(size_t)row_c + (okc ? (size_t)(c0 + jj) : 0) * lda];
+                av[jj] = (rok && okc) ? v : 0.0;
+            }
+        }
     };
     double a_next[16];
     if (ndep > 0) load_a(0, a_next);
@@ -149,6 +165,231 @@ __global__ __launch_bounds__(256) void trsv_blocked(int n, int nrhs, const TA* _
     }
 }
 
+// FP32 factors (the correction solves of the mixed-precision solver) take 128-row blocks: half
+// as many workgroup-to-workgroup hops on the critical path as 64-row blocks, each hop costing a
+// flag round trip through the memory system.  A workgroup thread holds rows r and r + 64 of
+// its block for 32 of the 128 columns of each block it depends on (FP64 sums of FP32 entries
+// times FP64 x, x_c read once per lane and broadcast by readlane).  The diagonal block is
+// solved by one wave, lane l owning rows l and l + 64: the half solved first also updates the
+// other half in the same steps, and a nonunit diagonal multiplies by its FP64 reciprocal.
+constexpr int TB2 = 128;
+constexpr int NR2_MAX = 8;
+
+template <int NR>
+constexpr size_t trsv128_lds_bytes() {
+    return (size_t)TB2 * (TB2 + 1) * sizeof(float) + (size_t)4 * TB2 * NR * sizeof(double) + TB2 * sizeof(double) + 16;
+}
+
+template <bool LOWER, bool UNIT, int NR>
+__global__ __launch_bounds__(256) void trsv128_f32(int n, int nrhs, const float* __restrict__ A, size_t lda,
+                                                   double* __restrict__ X, size_t ldx, unsigned* __restrict__ flags,
+                                                   unsigned gen, unsigned* __restrict__ abort_flag, long long timeout) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    double* red = (double*)lds_raw;                       // [4][TB2][NR]
+    double* rdiag = red + 4 * TB2 * NR;                   // [TB2]
+    float* D = (float*)(rdiag + TB2);                     // [TB2][TB2 + 1], strictly triangular
+    int* ok_lds = (int*)(D + TB2 * (TB2 + 1));
+    const int nblk = (n + TB2 - 1) / TB2;
+    const int b = LOWER ? (int)blockIdx.x : nblk - 1 - (int)blockIdx.x;
+    const int r0 = b * TB2;
+    const int nb = (n - r0) < TB2 ? (n - r0) : TB2;
+    const int t = threadIdx.x, lane = t & 63, part = t >> 6;
+
+    for (int idx = t; idx < TB2 * TB2; idx += 256) {
+        const int i = idx & (TB2 - 1), k = idx >> 7;
+        const bool tri = LOWER ? (k < i) : (k > i);
+        D[i * (TB2 + 1) + k] = (i < nb && k < nb && tri) ? A[(size_t)(r0 + i) + (size_t)(r0 + k) * lda] : 0.f;
+    }
+    if (!UNIT && t < TB2) rdiag[t] = t < nb ? 1.0 / (double)A[(size_t)(r0 + t) + (size_t)(r0 + t) * lda] : 0.0;
+
+    double acc0[NR], acc1[NR];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) acc0[k] = acc1[k] = 0.0;
+
+    const int ndep = LOWER ? b : nblk - 1 - b;
+    const int ra = r0 + lane, rb = r0 + 64 + lane;
+    const bool oka = ra < n, okb = rb < n;
+    // rows past n read row 0 and columns past n (only in an upper solve's partial last block)
+    // column 0, and are replaced by zeros: the loads are branch-free and walk the columns with a
+    // running pointer
+    const int ra_c = oka ? ra : 0, rb_c = okb ? rb : 0;
+    auto load_a = [&](int q, float (&av)[2][32]) {
+        const int c0 = (LOWER ? q : nblk - 1 - q) * TB2 + part * 32;
+        if (c0 + 32 <= n) {
+            const float* p0 = A + (size_t)ra_c + (size_t)c0 * lda;
+            const float* p1 = A + (size_t)rb_c + (size_t)c0 * lda;
+#pragma unroll
+            for (int jj = 0; jj < 32; ++jj) {
+                const float v0 = *p0, v1 = *p1;
+                av[0][jj] = oka ? v0 : 0.f;
+                av[1][jj] = okb ? v1 : 0.f;
+                p0 += lda;
+                p1 += lda;
+            }
+        } else {
+#pragma unroll
+            for (int jj = 0; jj < 32; ++jj) {
+                const bool okc = c0 + jj < n;
+                const size_t col = okc ? (size_t)(c0 + jj) : 0;
+                const float v0 = A[(size_t)ra_c + col * lda], v1 = A[(size_t)rb_c + col * lda];
+                av[0][jj] = (oka && okc) ? v0 : 0.f;
+                av[1][jj] = (okb && okc) ? v1 : 0.f;
+            }
+        }
+    };
+    float a_next[2][32];
+    if (ndep > 0) load_a(0, a_next);
+    bool ok = true;
+    for (int q = 0; q < ndep; ++q) {
+        float a_cur[2][32];
+#pragma unroll
+        for (int jj = 0; jj < 32; ++jj) {
+            a_cur[0][jj] = a_next[0][jj];
+            a_cur[1][jj] = a_next[1][jj];
+        }
+        const int c = LOWER ? q : nblk - 1 - q;
+        if (t == 0) {
+            int good = 1;
+            const long long t0 = wall_clock64();
+            while (__hip_atomic_load(&flags[c * 16], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != gen) {
+                if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+                    good = 0;
+                    break;
+                }
+                if (wall_clock64() - t0 > timeout) {
+                    __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    good = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (good) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            *ok_lds = good;
+        }
+        __syncthreads();
+        ok = *ok_lds != 0;
+        __syncthreads();
+        if (!ok) break;
+        if (q + 1 < ndep) load_a(q + 1, a_next);
+        const int c0 = c * TB2 + part * 32;
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            if (k < nrhs) {
+                const double xv = (lane < 32 && c0 + lane < n) ? X[(size_t)(c0 + lane) + (size_t)k * ldx] : 0.0;
+                double l0 = 0.0, l1 = 0.0;
+#pragma unroll
+                for (int jj = 0; jj < 32; ++jj) {
+                    const double xj = bcast_lane(xv, jj);
+                    l0 += (double)a_cur[0][jj] * xj;
+                    l1 += (double)a_cur[1][jj] * xj;
+                }
+                acc0[k] += l0;
+                acc1[k] += l1;
+            }
+        }
+    }
+    if (!ok) return;
+
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        red[(part * TB2 + lane) * NR + k] = acc0[k];
+        red[(part * TB2 + 64 + lane) * NR + k] = acc1[k];
+    }
+    __syncthreads();
+    if (part != 0) return;
+    double ya[NR], yb[NR];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        const double sa = ((red[lane * NR + k] + red[(TB2 + lane) * NR + k]) + red[(2 * TB2 + lane) * NR + k]) +
+                          red[(3 * TB2 + lane) * NR + k];
+        const double sb = ((red[(64 + lane) * NR + k] + red[(TB2 + 64 + lane) * NR + k]) +
+                           red[(2 * TB2 + 64 + lane) * NR + k]) +
+                          red[(3 * TB2 + 64 + lane) * NR + k];
+        ya[k] = (k < nrhs && oka) ? X[(size_t)ra + (size_t)k * ldx] - sa : 0.0;
+        yb[k] = (k < nrhs && okb) ? X[(size_t)rb + (size_t)k * ldx] - sb : 0.0;
+    }
+    // every step runs for all 64 + 64 rows (rows past nb hold y = 0, zero D entries and a zero
+    // reciprocal, so they change nothing): the loops are fully unrolled, lane indices are
+    // constants and the D reads are scheduled ahead of the dependency chain
+    const float* Da = D + lane * (TB2 + 1);
+    const float* Db = D + (64 + lane) * (TB2 + 1);
+    if (LOWER) {
+#pragma unroll
+        for (int i = 0; i < 64; ++i) {  // rows 0..63: both halves are updated
+            const double da = (double)Da[i], db = (double)Db[i];
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                if (k < nrhs) {
+                    if (!UNIT && lane == i) ya[k] *= rdiag[i];
+                    const double xi = bcast_lane(ya[k], i);
+                    if (lane > i) ya[k] -= da * xi;
+                    yb[k] -= db * xi;
+                }
+            }
+        }
+#pragma unroll
+        for (int li = 0; li < 64; ++li) {
+            const double db = (double)Db[64 + li];
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                if (k < nrhs) {
+                    if (!UNIT && lane == li) yb[k] *= rdiag[64 + li];
+                    const double xi = bcast_lane(yb[k], li);
+                    if (lane > li) yb[k] -= db * xi;
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (int li = 63; li >= 0; --li) {  // rows 127..64: both halves are updated
+            const double da = (double)Da[64 + li], db = (double)Db[64 + li];
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                if (k < nrhs) {
+                    if (!UNIT && lane == li) yb[k] *= rdiag[64 + li];
+                    const double xi = bcast_lane(yb[k], li);
+                    if (lane < li) yb[k] -= db * xi;
+                    ya[k] -= da * xi;
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 63; i >= 0; --i) {
+            const double da = (double)Da[i];
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                if (k < nrhs) {
+                    if (!UNIT && lane == i) ya[k] *= rdiag[i];
+                    const double xi = bcast_lane(ya[k], i);
+                    if (lane < i) ya[k] -= da * xi;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        if (k < nrhs) {
+            if (oka) X[(size_t)ra + (size_t)k * ldx] = ya[k];
+            if (okb) X[(size_t)rb + (size_t)k * ldx] = yb[k];
+        }
+    }
+    if (lane == 0) __hip_atomic_store(&flags[b * 16], gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <bool LOWER, bool UNIT, int NR>
+int launch128(hipStream_t s, unsigned nblk, int n, int nrhs, const float* A, size_t lda, double* X, size_t ldx,
+              unsigned* flags, unsigned gen, unsigned* abort_flag, long long timeout) {
+    static bool attr_set = false;  // benign race: every thread sets the same value
+    auto fn = trsv128_f32<LOWER, UNIT, NR>;
+    constexpr size_t bytes = trsv128_lds_bytes<NR>();
+    if (!attr_set) {
+        AL_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(fn, dim3(nblk), dim3(256), bytes, s, n, nrhs, A, lda, X, ldx, flags, gen, abort_flag, timeout);
+    return AL_OK;
+}
+
 template <typename TA, bool LOWER, bool UNIT, int NR>
 void launch_nr(hipStream_t s, unsigned nblk, int n, int nrhs, const TA* A, size_t lda, double* X, size_t ldx,
                unsigned* flags, unsigned gen, unsigned* abort_flag, long long timeout) {
@@ -168,7 +409,22 @@ int launch_trsv(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const TA* A
     const long long timeout = 200000000LL;  // 2 s at 100 MHz
     const int ticket = prof_begin(s);
     const unsigned g = (unsigned)nblk;
-    if (nrhs <= 1)
+    if (std::is_same<TA, float>::value && nrhs <= (size_t)NR2_MAX) {
+        const unsigned g2 = (unsigned)((n + TB2 - 1) / TB2);
+        const float* Af = (const float*)A;
+        if (nrhs <= 1)
+            AL_CHECK((launch128<LOWER, UNIT, 1>(s, g2, (int)n, (int)nrhs, Af, lda, X, ldx, ws->trsv_flags, gen,
+                                                ws->abort_flag, timeout)));
+        else if (nrhs <= 2)
+            AL_CHECK((launch128<LOWER, UNIT, 2>(s, g2, (int)n, (int)nrhs, Af, lda, X, ldx, ws->trsv_flags, gen,
+                                                ws->abort_flag, timeout)));
+        else if (nrhs <= 4)
+            AL_CHECK((launch128<LOWER, UNIT, 4>(s, g2, (int)n, (int)nrhs, Af, lda, X, ldx, ws->trsv_flags, gen,
+                                                ws->abort_flag, timeout)));
+        else
+            AL_CHECK((launch128<LOWER, UNIT, 8>(s, g2, (int)n, (int)nrhs, Af, lda, X, ldx, ws->trsv_flags, gen,
+                                                ws->abort_flag, timeout)));
+    } else if (nrhs <= 1)
         launch_nr<TA, LOWER, UNIT, 1>(s, g, (int)n, (int)nrhs, A, lda, X, ldx, ws->trsv_flags, gen, ws->abort_flag, timeout);
     else if (nrhs <= 2)
         launch_nr<TA, LOWER, UNIT, 2>(s, g, (int)n, (int)nrhs, A, lda, X, ldx, ws->trsv_flags, gen, ws->abort_flag, timeout);

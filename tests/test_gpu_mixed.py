@@ -101,7 +101,7 @@ def test_bf16_factor_reaches_fp64_accuracy(n, kappa):
     assert O.backward_error(a, x, b, n) <= TOL
 
 
-@pytest.mark.parametrize("nrhs", [4, 16, 20])
+@pytest.mark.parametrize("nrhs", [4, 6, 16, 20])
 def test_multiple_right_hand_sides(nrhs):
     n = 300
     a = uniform(n * n)

@@ -96,6 +96,10 @@ struct Workspace {
     size_t gbuf_bytes = 0;
     void* bbuf = nullptr;          // batched solver: FP32 work matrices
     size_t bbuf_bytes = 0;
+    // a second stream (default priority) for the panel factorization that runs ahead of the
+    // trailing update in the FP32 LU, and the two events that order the streams
+    hipStream_t side = nullptr;
+    hipEvent_t ev_side = nullptr, ev_main = nullptr;
 };
 
 // The workspace of `stream` for the calling thread (created on first use).
@@ -149,8 +153,10 @@ int trsm_f64(hipStream_t s, bool left, bool lower, bool trans, bool unit, size_t
 int swap_block_f64(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, size_t jb, double* A0, size_t n0,
                    double* A1, size_t n1, size_t lda, double* perm, const int* info);
 // The same for an FP32 matrix (no perm vector).
+// pairs: the composed-permutation buffer (1 + 2 * 512 ints); ws->pairs when null.  Swaps
+// that may run concurrently on two streams need buffers of their own.
 int swap_block_f32(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, size_t jb, float* A0, size_t n0,
-                   float* A1, size_t n1, size_t lda, const int* info);
+                   float* A1, size_t n1, size_t lda, const int* info, int* pairs = nullptr);
 int perm_init(hipStream_t s, double* perm, size_t n);                          // perm[i] = i
 int perm_from_ipiv(hipStream_t s, const int* ipiv, size_t n, double* perm);    // compose all pivots
 int gather_rows(hipStream_t s, const double* B, size_t ldb, size_t n, size_t nrhs, const double* perm, double* W);

@@ -48,6 +48,9 @@ struct ThreadCtx {
             if (w.hnorms) (void)hipHostFree(w.hnorms);
             if (w.gbuf) (void)hipFree(w.gbuf);
             if (w.bbuf) (void)hipFree(w.bbuf);
+            if (w.ev_side) (void)hipEventDestroy(w.ev_side);
+            if (w.ev_main) (void)hipEventDestroy(w.ev_main);
+            if (w.side) (void)hipStreamDestroy(w.side);
         }
         if (stream) (void)hipStreamDestroy(stream);
     }

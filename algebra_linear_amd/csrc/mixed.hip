@@ -255,6 +255,7 @@ struct Arena {
     __bf16* Bt = nullptr;
     void* tslu = nullptr;     // tournament panel scratch (nominees, L11 \ U11)
     unsigned* counters = nullptr;  // its arrival counters (zero between panels)
+    int* pairs_side = nullptr;     // row-interchange pairs of the panel stream's swaps
 };
 
 constexpr int NORM_A = 0, NORM_B = 1, NORM_R = 17, NORM_X = 33, NORM_WORDS = 64;
@@ -306,10 +307,13 @@ int make_arena(hipStream_t s, Workspace* ws, size_t n, size_t np, size_t nr, siz
     if (bf16) off += align256(np * nbo * 2);
     const size_t oT = off;
     if (lowp) off += align256(tslu_scratch_bytes(np));
+    const size_t oS = off;
+    if (lowp) off += align256((1 + 2 * 512) * sizeof(int));
     AL_CHECK(grow(&ws->mbuf, &ws->mbuf_bytes, off));
     char* base = (char*)ws->mbuf;
     AL_CHECK(fill_u32(s, (unsigned*)base, 1024, 0u));
     a->tslu = base + oT;
+    a->pairs_side = lowp ? (int*)(base + oS) : nullptr;
     a->counters = (unsigned*)base;
     a->F = (float*)(base + oF);
     a->ipiv = (int*)(base + oV);
@@ -331,8 +335,61 @@ int trsm_ll64(hipStream_t s, Arena& a, int J, int c0, int c1) {
     return AL_OK;
 }
 
+// One outer panel of the FP32 LU, columns [K, KE) and rows K..np, 64 columns at a time:
+// tournament pivots, the interchanges applied to the outer panel's columns, L21 of the 64
+// columns, then the rest of the outer panel updated (64-row U solve and a k = 64 product).
+int factor_panel(hipStream_t q, Workspace* ws, Arena& a, int K, int KE, int* pairs) {
+    const int np = (int)a.np;
+    const size_t ld = a.np;
+    float* F = a.F;
+    auto at = [&](int i, int j) { return F + (size_t)i + (size_t)j * ld; };
+    for (int J = K; J < KE; J += PB) {
+        AL_CHECK(tslu_panel(q, F, ld, J, np - J, a.ipiv, a.minfo, a.tslu, a.counters));
+        AL_CHECK(swap_block_f32(q, ws, a.ipiv, J, PB, at(0, K), KE - K, nullptr, 0, ld, a.minfo, pairs));
+        AL_CHECK(tslu_finish(q, F, ld, J, np - J, a.tslu));
+        if (J + PB < KE) {
+            AL_CHECK(trsm_ll64(q, a, J, J + PB, KE));
+            AL_CHECK(gemm_f32(q, false, false, np - J - PB, KE - J - PB, PB, -1.f, at(J + PB, J), ld, at(J, J + PB), ld,
+                              1.f, at(J + PB, J + PB), ld));
+        }
+    }
+    return AL_OK;
+}
+
+// A22[:, c0:c1) -= L21 U12[:, c0:c1) for the outer panel [K, KE): FP32 matrix cores, or bf16
+// operands (staged once per outer panel, c0 == KE) with FP32 accumulation.
+int trailing_update(hipStream_t s, Arena& a, bool bf16, int K, int KE, int c0, int c1) {
+    const int np = (int)a.np;
+    const size_t ld = a.np;
+    const int kb = KE - K, m2 = np - KE;
+    float* F = a.F;
+    auto at = [&](int i, int j) { return F + (size_t)i + (size_t)j * ld; };
+    if (c1 <= c0) return AL_OK;
+    if (bf16) {
+        if (c0 == KE) {
+            AL_CHECK(bf16_rows(s, m2, kb, kb, at(KE, K), ld, a.At));
+            AL_CHECK(bf16_cols(s, kb, kb, m2, at(K, KE), ld, a.Bt));
+        }
+        return gemm_bf16(s, m2, c1 - c0, kb, -1.f, a.At, a.Bt + (size_t)(c0 - KE) * kb, 1.f, at(KE, c0), ld);
+    }
+    return gemm_f32(s, false, false, m2, c1 - c0, kb, -1.f, at(KE, K), ld, at(K, c0), ld, 1.f, at(KE, c0), ld);
+}
+
+int ensure_side_stream(Workspace* ws) {
+    if (!ws->side) AL_HIP(hipStreamCreateWithFlags(&ws->side, hipStreamNonBlocking));
+    if (!ws->ev_side) AL_HIP(hipEventCreateWithFlags(&ws->ev_side, hipEventDisableTiming));
+    if (!ws->ev_main) AL_HIP(hipEventCreateWithFlags(&ws->ev_main, hipEventDisableTiming));
+    return AL_OK;
+}
+
 // Blocked right-looking LU with partial pivoting of the np x np FP32 matrix a.F
 // (np % 64 == 0); pivots as global row indices in a.ipiv, status in a.minfo[0].
+//
+// Look-ahead on two streams: the next outer panel's columns are updated first, and while the
+// main stream applies the rest of the trailing update the side stream factors that panel (its
+// tournaments, interchanges inside the panel, L21 and inner updates touch only the panel's
+// columns, which the rest of the update does not read or write).  The main stream waits for
+// the panel before applying its interchanges to the other columns.
 int getrf_lowp(hipStream_t s, Workspace* ws, Arena& a, bool bf16) {
     const int np = (int)a.np;
     const int nbo = bf16 ? 512 : 256;
@@ -340,25 +397,20 @@ int getrf_lowp(hipStream_t s, Workspace* ws, Arena& a, bool bf16) {
     int* info = a.minfo;
     const size_t ld = a.np;
     auto at = [&](int i, int j) { return F + (size_t)i + (size_t)j * ld; };
+    AL_CHECK(ensure_side_stream(ws));
+    hipStream_t side = ws->side;
     AL_CHECK(fill_i32(s, info, 1, 0));
     AL_CHECK(fill_u32(s, ws->abort_flag, 1, 0u));
     AL_CHECK(iota_i32(s, a.ipiv, a.np));
+    AL_HIP(hipEventRecord(ws->ev_main, s));
+    AL_HIP(hipStreamWaitEvent(side, ws->ev_main, 0));
+    AL_CHECK(factor_panel(side, ws, a, 0, np < nbo ? np : nbo, a.pairs_side));
+    AL_HIP(hipEventRecord(ws->ev_side, side));
     for (int K = 0; K < np; K += nbo) {
         const int kb = (np - K) < nbo ? (np - K) : nbo;
         const int KE = K + kb;
-        // the outer panel, 64 columns at a time: tournament pivots, the interchanges applied to
-        // the outer panel's columns, then L21 of the 64 columns
-        for (int J = K; J < KE; J += PB) {
-            AL_CHECK(tslu_panel(s, F, ld, J, np - J, a.ipiv, info, a.tslu, a.counters));
-            AL_CHECK(swap_block_f32(s, ws, a.ipiv, J, PB, at(0, K), KE - K, nullptr, 0, ld, info));
-            AL_CHECK(tslu_finish(s, F, ld, J, np - J, a.tslu));
-            if (J + PB < KE) {
-                AL_CHECK(trsm_ll64(s, a, J, J + PB, KE));
-                AL_CHECK(gemm_f32(s, false, false, np - J - PB, KE - J - PB, PB, -1.f, at(J + PB, J), ld, at(J, J + PB),
-                                  ld, 1.f, at(J + PB, J + PB), ld));
-            }
-        }
-        // the outer panel's swaps on the columns left and right of it (at most 256 at a time)
+        AL_HIP(hipStreamWaitEvent(s, ws->ev_side, 0));  // the outer panel [K, KE) is factored
+        // its interchanges on the columns left and right of it (at most 256 at a time)
         for (int J = K; J < KE; J += 256) {
             const int jb = (KE - J) < 256 ? (KE - J) : 256;
             AL_CHECK(swap_block_f32(s, ws, a.ipiv, J, jb, at(0, 0), K, at(0, KE), np - KE, ld, info));
@@ -371,15 +423,15 @@ int getrf_lowp(hipStream_t s, Workspace* ws, Arena& a, bool bf16) {
                 AL_CHECK(gemm_f32(s, false, false, KE - J - PB, np - KE, PB, -1.f, at(J + PB, J), ld, at(J, KE), ld, 1.f,
                                   at(J + PB, KE), ld));
         }
-        // trailing update A22 -= L21 U12
-        const int m2 = np - KE;
-        if (bf16) {
-            AL_CHECK(bf16_rows(s, m2, kb, kb, at(KE, K), ld, a.At));
-            AL_CHECK(bf16_cols(s, kb, kb, m2, at(K, KE), ld, a.Bt));
-            AL_CHECK(gemm_bf16(s, m2, m2, kb, -1.f, a.At, a.Bt, 1.f, at(KE, KE), ld));
-        } else {
-            AL_CHECK(gemm_f32(s, false, false, m2, m2, kb, -1.f, at(KE, K), ld, at(K, KE), ld, 1.f, at(KE, KE), ld));
-        }
+        // trailing update: the next outer panel's columns, then the side stream factors that
+        // panel while the main stream updates the rest
+        const int KE2 = (np - KE) < nbo ? np : KE + nbo;
+        AL_CHECK(trailing_update(s, a, bf16, K, KE, KE, KE2));
+        AL_HIP(hipEventRecord(ws->ev_main, s));
+        AL_HIP(hipStreamWaitEvent(side, ws->ev_main, 0));
+        AL_CHECK(factor_panel(side, ws, a, KE, KE2, a.pairs_side));
+        AL_HIP(hipEventRecord(ws->ev_side, side));
+        AL_CHECK(trailing_update(s, a, bf16, K, KE, KE2, np));
     }
     return perm_from_ipiv(s, a.ipiv, a.np, a.perm);
 }

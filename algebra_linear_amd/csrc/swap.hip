@@ -147,29 +147,30 @@ __global__ void copy_block_k(const double* __restrict__ W, int n, int nrhs, doub
 
 template <typename T>
 int swap_block_t(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, size_t jb, T* A0, size_t n0, T* A1,
-                 size_t n1, size_t lda, double* perm, const int* info) {
+                 size_t n1, size_t lda, double* perm, const int* info, int* pairs) {
     if (jb == 0 || (n0 + n1 == 0 && !perm)) return AL_OK;
     if (jb > (size_t)JB_MAX) {
         set_error("swap_block: pivot block wider than 256");
         return AL_ERR_ARG;
     }
-    hipLaunchKernelGGL(piv_pairs, dim3(1), dim3(512), 0, s, ipiv, (int)k0, (int)jb, ws->pairs, info);
+    if (!pairs) pairs = ws->pairs;
+    hipLaunchKernelGGL(piv_pairs, dim3(1), dim3(512), 0, s, ipiv, (int)k0, (int)jb, pairs, info);
     AL_HIP(hipGetLastError());
     const size_t ncols = n0 + n1 + (perm ? 1 : 0);
     hipLaunchKernelGGL(apply_pairs<T>, dim3(ceil_div(ncols, 4)), dim3(256), 0, s, A0, (int)n0, A1, (int)n1, lda, perm,
-                       ws->pairs, info);
+                       pairs, info);
     AL_HIP(hipGetLastError());
     return AL_OK;
 }
 
 int swap_block_f64(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, size_t jb, double* A0, size_t n0,
                    double* A1, size_t n1, size_t lda, double* perm, const int* info) {
-    return swap_block_t<double>(s, ws, ipiv, k0, jb, A0, n0, A1, n1, lda, perm, info);
+    return swap_block_t<double>(s, ws, ipiv, k0, jb, A0, n0, A1, n1, lda, perm, info, nullptr);
 }
 
 int swap_block_f32(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, size_t jb, float* A0, size_t n0,
-                   float* A1, size_t n1, size_t lda, const int* info) {
-    return swap_block_t<float>(s, ws, ipiv, k0, jb, A0, n0, A1, n1, lda, nullptr, info);
+                   float* A1, size_t n1, size_t lda, const int* info, int* pairs) {
+    return swap_block_t<float>(s, ws, ipiv, k0, jb, A0, n0, A1, n1, lda, nullptr, info, pairs);
 }
 
 int perm_init(hipStream_t s, double* perm, size_t n) {

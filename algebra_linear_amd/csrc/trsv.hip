@@ -14,6 +14,7 @@
 // The matrix may be FP64 or FP32 (the low-precision factors of the mixed-precision solver);
 // right-hand sides and all accumulation are FP64.
 #include <type_traits>
+#include <utility>
 
 #include "al_internal.hpp"
 
@@ -175,6 +176,9 @@ __global__ __launch_bounds__(256) void trsv_blocked(int n, int nrhs, const TA* _
 constexpr int TB2 = 128;
 constexpr int NR2_MAX = 8;
 
+template <int V>
+using ic = std::integral_constant<int, V>;
+
 template <int NR>
 constexpr size_t trsv128_lds_bytes() {
     return (size_t)TB2 * (TB2 + 1) * sizeof(float) + (size_t)4 * TB2 * NR * sizeof(double) + TB2 * sizeof(double) + 16;
@@ -308,63 +312,84 @@ __global__ __launch_bounds__(256) void trsv128_f32(int n, int nrhs, const float*
         ya[k] = (k < nrhs && oka) ? X[(size_t)ra + (size_t)k * ldx] - sa : 0.0;
         yb[k] = (k < nrhs && okb) ? X[(size_t)rb + (size_t)k * ldx] - sb : 0.0;
     }
-    // every step runs for all 64 + 64 rows (rows past nb hold y = 0, zero D entries and a zero
-    // reciprocal, so they change nothing): the loops are fully unrolled, lane indices are
-    // constants and the D reads are scheduled ahead of the dependency chain
+    // The diagonal block in groups of 8 rows, in solve order.  For a group every lane reads the
+    // 8 current values by readlane and solves the 8 x 8 triangle itself (a dependency chain of 8
+    // short steps instead of 8 lane-to-lane broadcasts), then updates the rows it owns that come
+    // later in the solve.  Rows past nb hold y = 0, zero D entries and a zero reciprocal, so
+    // running every group of both halves changes nothing for them.  The D reads do not depend on
+    // y, so they are issued ahead of the chain.
     const float* Da = D + lane * (TB2 + 1);
     const float* Db = D + (64 + lane) * (TB2 + 1);
+    // one group of the half whose rows start at H (its lanes' values in y), later rows of the
+    // same half in y, every row of the other half in yo (when OTHER)
+    auto group = [&](auto Hc, int G, auto OTHERc, double (&y)[NR], double (&yo)[NR]) {
+        constexpr int H = decltype(Hc)::value;
+        constexpr bool OTHER = decltype(OTHERc)::value;
+        const int R0 = H + 8 * G;  // first row of the group
+        float dg[8][8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+#pragma unroll
+            for (int c2 = 0; c2 < 8; ++c2) dg[r][c2] = ((LOWER && c2 < r) || (!LOWER && c2 > r)) ? D[(R0 + r) * (TB2 + 1) + R0 + c2] : 0.f;
+        double rd[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) rd[r] = UNIT ? 1.0 : rdiag[R0 + r];
+        const float* Dy = H == 0 ? Da : Db;   // this lane's row in the group's half
+        const float* Do = H == 0 ? Db : Da;   // .. and in the other half
+        float dy[8], dother[8];
+#pragma unroll
+        for (int c2 = 0; c2 < 8; ++c2) {
+            dy[c2] = Dy[R0 + c2];
+            dother[c2] = OTHER ? Do[R0 + c2] : 0.f;
+        }
+        const int gl = 8 * G;  // the group's first lane
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            if (k < nrhs) {
+                double xs[8];
+#pragma unroll
+                for (int r = 0; r < 8; ++r) xs[r] = bcast_lane(y[k], gl + r);
+                if (LOWER) {
+#pragma unroll
+                    for (int r = 0; r < 8; ++r) {
+#pragma unroll
+                        for (int c2 = 0; c2 < r; ++c2) xs[r] = fma(-(double)dg[r][c2], xs[c2], xs[r]);
+                        if (!UNIT) xs[r] *= rd[r];
+                    }
+                } else {
+#pragma unroll
+                    for (int r = 7; r >= 0; --r) {
+#pragma unroll
+                        for (int c2 = r + 1; c2 < 8; ++c2) xs[r] = fma(-(double)dg[r][c2], xs[c2], xs[r]);
+                        if (!UNIT) xs[r] *= rd[r];
+                    }
+                }
+                double u = y[k], uo = yo[k];
+#pragma unroll
+                for (int c2 = 0; c2 < 8; ++c2) {
+                    u = fma(-(double)dy[c2], xs[c2], u);
+                    if (OTHER) uo = fma(-(double)dother[c2], xs[c2], uo);
+                }
+                const bool later = LOWER ? lane >= gl + 8 : lane < gl;
+                double v = later ? u : y[k];
+#pragma unroll
+                for (int r = 0; r < 8; ++r) v = lane == gl + r ? xs[r] : v;
+                y[k] = v;
+                if (OTHER) yo[k] = uo;
+            }
+        }
+    };
+    // groups in solve order: 0..7 for a lower solve, 7..0 for an upper one
     if (LOWER) {
 #pragma unroll
-        for (int i = 0; i < 64; ++i) {  // rows 0..63: both halves are updated
-            const double da = (double)Da[i], db = (double)Db[i];
+        for (int g = 0; g < 8; ++g) group(ic<0>{}, g, std::true_type{}, ya, yb);
 #pragma unroll
-            for (int k = 0; k < NR; ++k) {
-                if (k < nrhs) {
-                    if (!UNIT && lane == i) ya[k] *= rdiag[i];
-                    const double xi = bcast_lane(ya[k], i);
-                    if (lane > i) ya[k] -= da * xi;
-                    yb[k] -= db * xi;
-                }
-            }
-        }
-#pragma unroll
-        for (int li = 0; li < 64; ++li) {
-            const double db = (double)Db[64 + li];
-#pragma unroll
-            for (int k = 0; k < NR; ++k) {
-                if (k < nrhs) {
-                    if (!UNIT && lane == li) yb[k] *= rdiag[64 + li];
-                    const double xi = bcast_lane(yb[k], li);
-                    if (lane > li) yb[k] -= db * xi;
-                }
-            }
-        }
+        for (int g = 0; g < 8; ++g) group(ic<64>{}, g, std::false_type{}, yb, ya);
     } else {
 #pragma unroll
-        for (int li = 63; li >= 0; --li) {  // rows 127..64: both halves are updated
-            const double da = (double)Da[64 + li], db = (double)Db[64 + li];
+        for (int g = 7; g >= 0; --g) group(ic<64>{}, g, std::true_type{}, yb, ya);
 #pragma unroll
-            for (int k = 0; k < NR; ++k) {
-                if (k < nrhs) {
-                    if (!UNIT && lane == li) yb[k] *= rdiag[64 + li];
-                    const double xi = bcast_lane(yb[k], li);
-                    if (lane < li) yb[k] -= db * xi;
-                    ya[k] -= da * xi;
-                }
-            }
-        }
-#pragma unroll
-        for (int i = 63; i >= 0; --i) {
-            const double da = (double)Da[i];
-#pragma unroll
-            for (int k = 0; k < NR; ++k) {
-                if (k < nrhs) {
-                    if (!UNIT && lane == i) ya[k] *= rdiag[i];
-                    const double xi = bcast_lane(ya[k], i);
-                    if (lane < i) ya[k] -= da * xi;
-                }
-            }
-        }
+        for (int g = 7; g >= 0; --g) group(ic<0>{}, g, std::false_type{}, ya, yb);
     }
 #pragma unroll
     for (int k = 0; k < NR; ++k) {

@@ -357,7 +357,7 @@ int factor_panel(hipStream_t q, Workspace* ws, Arena& a, int K, int KE, int* pai
 }
 
 // A22[:, c0:c1) -= L21 U12[:, c0:c1) for the outer panel [K, KE): FP32 matrix cores, or bf16
-// operands (staged once per outer panel, c0 == KE) with FP32 accumulation.
+// operands with FP32 accumulation (L21 staged by the call with c0 == KE, U12 per call).
 int trailing_update(hipStream_t s, Arena& a, bool bf16, int K, int KE, int c0, int c1) {
     const int np = (int)a.np;
     const size_t ld = a.np;
@@ -366,11 +366,10 @@ int trailing_update(hipStream_t s, Arena& a, bool bf16, int K, int KE, int c0, i
     auto at = [&](int i, int j) { return F + (size_t)i + (size_t)j * ld; };
     if (c1 <= c0) return AL_OK;
     if (bf16) {
-        if (c0 == KE) {
-            AL_CHECK(bf16_rows(s, m2, kb, kb, at(KE, K), ld, a.At));
-            AL_CHECK(bf16_cols(s, kb, kb, m2, at(K, KE), ld, a.Bt));
-        }
-        return gemm_bf16(s, m2, c1 - c0, kb, -1.f, a.At, a.Bt + (size_t)(c0 - KE) * kb, 1.f, at(KE, c0), ld);
+        if (c0 == KE) AL_CHECK(bf16_rows(s, m2, kb, kb, at(KE, K), ld, a.At));
+        __bf16* Bt = a.Bt + (size_t)(c0 - KE) * kb;
+        AL_CHECK(bf16_cols(s, kb, kb, c1 - c0, at(K, c0), ld, Bt));
+        return gemm_bf16(s, m2, c1 - c0, kb, -1.f, a.At, Bt, 1.f, at(KE, c0), ld);
     }
     return gemm_f32(s, false, false, m2, c1 - c0, kb, -1.f, at(KE, K), ld, at(K, c0), ld, 1.f, at(KE, c0), ld);
 }
@@ -406,31 +405,45 @@ int getrf_lowp(hipStream_t s, Workspace* ws, Arena& a, bool bf16) {
     AL_HIP(hipStreamWaitEvent(side, ws->ev_main, 0));
     AL_CHECK(factor_panel(side, ws, a, 0, np < nbo ? np : nbo, a.pairs_side));
     AL_HIP(hipEventRecord(ws->ev_side, side));
+    // the outer panel [K, KE)'s interchanges on columns [c0, c1) and, when n1 > 0, on [c2, c2 + n1)
+    auto swaps = [&](int K, int KE, int c0, int n0, int c2, int n1) -> int {
+        for (int J = K; J < KE; J += 256) {
+            const int jb = (KE - J) < 256 ? (KE - J) : 256;
+            AL_CHECK(swap_block_f32(s, ws, a.ipiv, J, jb, at(0, c0), n0, n1 > 0 ? at(0, c2) : nullptr, n1, ld, info));
+        }
+        return AL_OK;
+    };
+    // U12 := L11^{-1} A12 on columns [c0, c1), 64 rows at a time
+    auto u12 = [&](int K, int KE, int c0, int c1) -> int {
+        if (c1 <= c0) return AL_OK;
+        for (int J = K; J < KE; J += PB) {
+            AL_CHECK(trsm_ll64(s, a, J, c0, c1));
+            if (J + PB < KE)
+                AL_CHECK(gemm_f32(s, false, false, KE - J - PB, c1 - c0, PB, -1.f, at(J + PB, J), ld, at(J, c0), ld, 1.f,
+                                  at(J + PB, c0), ld));
+        }
+        return AL_OK;
+    };
     for (int K = 0; K < np; K += nbo) {
         const int kb = (np - K) < nbo ? (np - K) : nbo;
         const int KE = K + kb;
         AL_HIP(hipStreamWaitEvent(s, ws->ev_side, 0));  // the outer panel [K, KE) is factored
-        // its interchanges on the columns left and right of it (at most 256 at a time)
-        for (int J = K; J < KE; J += 256) {
-            const int jb = (KE - J) < 256 ? (KE - J) : 256;
-            AL_CHECK(swap_block_f32(s, ws, a.ipiv, J, jb, at(0, 0), K, at(0, KE), np - KE, ld, info));
+        if (KE >= np) {
+            AL_CHECK(swaps(K, KE, 0, K, 0, 0));
+            break;
         }
-        if (KE >= np) break;
-        // U12 := L11^{-1} A12, 64 rows at a time
-        for (int J = K; J < KE; J += PB) {
-            AL_CHECK(trsm_ll64(s, a, J, KE, np));
-            if (J + PB < KE)
-                AL_CHECK(gemm_f32(s, false, false, KE - J - PB, np - KE, PB, -1.f, at(J + PB, J), ld, at(J, KE), ld, 1.f,
-                                  at(J + PB, KE), ld));
-        }
-        // trailing update: the next outer panel's columns, then the side stream factors that
-        // panel while the main stream updates the rest
+        // the next outer panel's columns first (interchanges, U12, update), so the side stream
+        // can start factoring it; then the other columns
         const int KE2 = (np - KE) < nbo ? np : KE + nbo;
+        AL_CHECK(swaps(K, KE, KE, KE2 - KE, 0, 0));
+        AL_CHECK(u12(K, KE, KE, KE2));
         AL_CHECK(trailing_update(s, a, bf16, K, KE, KE, KE2));
         AL_HIP(hipEventRecord(ws->ev_main, s));
         AL_HIP(hipStreamWaitEvent(side, ws->ev_main, 0));
         AL_CHECK(factor_panel(side, ws, a, KE, KE2, a.pairs_side));
         AL_HIP(hipEventRecord(ws->ev_side, side));
+        AL_CHECK(swaps(K, KE, 0, K, KE2, np - KE2));
+        AL_CHECK(u12(K, KE, KE2, np));
         AL_CHECK(trailing_update(s, a, bf16, K, KE, KE2, np));
     }
     return perm_from_ipiv(s, a.ipiv, a.np, a.perm);

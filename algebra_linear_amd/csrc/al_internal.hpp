@@ -157,6 +157,10 @@ int swap_block_f64(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, siz
 // that may run concurrently on two streams need buffers of their own.
 int swap_block_f32(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, size_t jb, float* A0, size_t n0,
                    float* A1, size_t n1, size_t lda, const int* info, int* pairs = nullptr);
+// Applies (destination, source) row pairs already composed (as tslu_panel writes them) to the
+// columns [0, n0) of A0 and [0, n1) of A1; nothing when *info != 0.
+int swap_apply_f32(hipStream_t s, const int* pairs, float* A0, size_t n0, float* A1, size_t n1, size_t lda,
+                   const int* info);
 int perm_init(hipStream_t s, double* perm, size_t n);                          // perm[i] = i
 int perm_from_ipiv(hipStream_t s, const int* ipiv, size_t n, double* perm);    // compose all pivots
 int gather_rows(hipStream_t s, const double* B, size_t ldb, size_t n, size_t nrhs, const double* perm, double* W);
@@ -180,7 +184,7 @@ int panel_f32(hipStream_t s, Workspace* ws, float* P, size_t lda, size_t row0, s
 // (one per tree node, left at zero by every panel).
 size_t tslu_scratch_bytes(size_t m);
 int tslu_panel(hipStream_t s, float* F, size_t ld, size_t J, size_t m, int* ipiv, int* info, void* scratch,
-               unsigned* counters);
+               unsigned* counters, int* pairs = nullptr);
 int tslu_finish(hipStream_t s, float* F, size_t ld, size_t J, size_t m, const void* scratch);
 
 // Solve with a getrf factorization: B (n x nrhs) := A^{-1} B.  perm (nullable) is getrf's row

@@ -344,8 +344,8 @@ int factor_panel(hipStream_t q, Workspace* ws, Arena& a, int K, int KE, int* pai
     float* F = a.F;
     auto at = [&](int i, int j) { return F + (size_t)i + (size_t)j * ld; };
     for (int J = K; J < KE; J += PB) {
-        AL_CHECK(tslu_panel(q, F, ld, J, np - J, a.ipiv, a.minfo, a.tslu, a.counters));
-        AL_CHECK(swap_block_f32(q, ws, a.ipiv, J, PB, at(0, K), KE - K, nullptr, 0, ld, a.minfo, pairs));
+        AL_CHECK(tslu_panel(q, F, ld, J, np - J, a.ipiv, a.minfo, a.tslu, a.counters, pairs));
+        AL_CHECK(swap_apply_f32(q, pairs, at(0, K), KE - K, nullptr, 0, ld, a.minfo));
         AL_CHECK(tslu_finish(q, F, ld, J, np - J, a.tslu));
         if (J + PB < KE) {
             AL_CHECK(trsm_ll64(q, a, J, J + PB, KE));

@@ -173,6 +173,15 @@ int swap_block_f32(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, siz
     return swap_block_t<float>(s, ws, ipiv, k0, jb, A0, n0, A1, n1, lda, nullptr, info, pairs);
 }
 
+int swap_apply_f32(hipStream_t s, const int* pairs, float* A0, size_t n0, float* A1, size_t n1, size_t lda,
+                   const int* info) {
+    if (n0 + n1 == 0) return AL_OK;
+    hipLaunchKernelGGL(apply_pairs<float>, dim3(ceil_div(n0 + n1, 4)), dim3(256), 0, s, A0, (int)n0, A1, (int)n1, lda,
+                       (double*)nullptr, pairs, info);
+    AL_HIP(hipGetLastError());
+    return AL_OK;
+}
+
 int perm_init(hipStream_t s, double* perm, size_t n) {
     if (n == 0) return AL_OK;
     hipLaunchKernelGGL(perm_iota, dim3(ceil_div(n, 256)), dim3(256), 0, s, perm, n);

@@ -195,7 +195,8 @@ __device__ __forceinline__ void gepp_rows(float (&x)[W], bool active, int row, i
 __global__ __launch_bounds__(256, 1) void tslu_k(const float* __restrict__ F, size_t ld, int J, int m, Tree tree,
                                                  float* __restrict__ cand, int* __restrict__ cidx,
                                                  unsigned* __restrict__ counters, float* __restrict__ D,
-                                                 int* __restrict__ ipiv, int* __restrict__ info) {
+                                                 int* __restrict__ ipiv, int* __restrict__ info,
+                                                 int* __restrict__ pairs) {
     __shared__ TsluLds L;
     const int t = threadIdx.x, lane = t & 63, q = t >> 6;
 
@@ -279,7 +280,10 @@ __global__ __launch_bounds__(256, 1) void tslu_k(const float* __restrict__ F, si
     const bool sane = __all(my_winner >= J && my_winner < J + m);
     if (!sane) {  // cannot happen with a correct elimination; never hand out a wild row index
         ipiv[J + lane] = J + lane;
-        if (lane == 0) atomicCAS(info, 0, -1);
+        if (lane == 0) {
+            atomicCAS(info, 0, -1);
+            if (pairs) pairs[0] = 0;
+        }
         return;
     }
     int occ = J + lane, where = J + lane, my_p = J + lane;
@@ -296,6 +300,28 @@ __global__ __launch_bounds__(256, 1) void tslu_k(const float* __restrict__ F, si
         }
     }
     ipiv[J + lane] = my_p;
+    if (pairs) {
+        // the interchanges composed into (destination, source) row pairs, sources being the
+        // original rows: position J + lane now holds original row occ; original row J + lane,
+        // if it left the block, now sits at where (below it).  These are exactly the positions
+        // the sequence changes, top block first.
+        const bool top = occ != J + lane;
+        const bool down = where >= J + W;
+        const unsigned long long bt = __ballot(top), bd = __ballot(down);
+        const unsigned long long below = (1ull << lane) - 1ull;
+        const int nt = __popcll(bt);
+        if (top) {
+            const int r = __popcll(bt & below);
+            pairs[1 + 2 * r] = J + lane;
+            pairs[2 + 2 * r] = occ;
+        }
+        if (down) {
+            const int r = nt + __popcll(bd & below);
+            pairs[1 + 2 * r] = where;
+            pairs[2 + 2 * r] = J + lane;
+        }
+        if (lane == 0) pairs[0] = nt + __popcll(bd);
+    }
     const bool zero = !(L.piv[lane] != 0.f);
     const unsigned long long zb = __ballot(zero);
     if (lane == 0 && zb != 0ull) atomicCAS(info, 0, J + (__ffsll((long long)zb) - 1) + 1);
@@ -360,7 +386,7 @@ size_t tslu_scratch_bytes(size_t m) {
 }
 
 int tslu_panel(hipStream_t s, float* F, size_t ld, size_t J, size_t m, int* ipiv, int* info, void* scratch,
-               unsigned* counters) {
+               unsigned* counters, int* pairs) {
     if (m < (size_t)W || m % W != 0 || m > (size_t)LEAF * 4096) {
         set_error("tslu: panel height must be a positive multiple of 64");
         return AL_ERR_ARG;
@@ -376,7 +402,7 @@ int tslu_panel(hipStream_t s, float* F, size_t ld, size_t J, size_t m, int* ipiv
     float* D = (float*)(cidx + (size_t)nodes * W);
     const int ticket = prof_begin(s);
     hipLaunchKernelGGL(tslu_k, dim3((unsigned)tr.count[0]), dim3(256), 0, s, F, ld, (int)J, (int)m, tr, cand, cidx,
-                       counters, D, ipiv, info);
+                       counters, D, ipiv, info, pairs);
     AL_HIP(hipGetLastError());
     if (ticket >= 0) {
         const double dm = (double)m, b = W;

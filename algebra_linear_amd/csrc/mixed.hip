@@ -442,7 +442,10 @@ int getrf_lowp(hipStream_t s, Workspace* ws, Arena& a, bool bf16) {
         AL_HIP(hipStreamWaitEvent(side, ws->ev_main, 0));
         AL_CHECK(factor_panel(side, ws, a, KE, KE2, a.pairs_side));
         AL_HIP(hipEventRecord(ws->ev_side, side));
-        AL_CHECK(swaps(K, KE, 0, K, KE2, np - KE2));
+        if (KE - K <= 256)  // one pivot block: its pairs are still in ws->pairs from the first call
+            AL_CHECK(swap_apply_f32(s, ws->pairs, at(0, 0), K, at(0, KE2), np - KE2, ld, info));
+        else
+            AL_CHECK(swaps(K, KE, 0, K, KE2, np - KE2));
         AL_CHECK(u12(K, KE, KE2, np));
         AL_CHECK(trailing_update(s, a, bf16, K, KE, KE2, np));
     }

@@ -19,19 +19,22 @@ constexpr int PAIRS_MAX = 2 * JB_MAX;
 // Composes the pivots ipiv[k0..k0+jb) into (destination, source) row pairs: out[0] = count,
 // out[1 + 2p] = destination row, out[2 + 2p] = source row.  Every row the block touches --
 // the rows k0..k0+jb-1 and the pivot rows below them -- is traced backwards through the swaps
-// independently (one thread per row, the swaps broadcast from LDS): walking the sequence in
-// reverse from a destination row yields the original row whose value ends there.
+// independently (one thread per row, the swaps broadcast from LDS four at a time): walking the
+// sequence in reverse from a destination row yields the original row whose value ends there.
+// A pivot row is kept only at the first step that touches it.  The loops have fixed trip
+// counts and no early exits, and the pairs are written in thread order (top rows ascending,
+// then pivot rows in step order), so the list is the same on every run.
 __global__ __launch_bounds__(512) void piv_pairs(const int* __restrict__ ipiv, int k0, int jb, int* __restrict__ out,
                                                  const int* __restrict__ info) {
-    __shared__ int piv[JB_MAX];
-    __shared__ int cnt;
+    __shared__ __align__(16) int piv[JB_MAX];
+    __shared__ int wcount[8];
     const int t = threadIdx.x;
     if (info && *info != 0) {
         if (t == 0) out[0] = 0;
         return;
     }
-    if (t < jb) piv[t] = ipiv[k0 + t];
-    if (t == 0) cnt = 0;
+    const int jb4 = (jb + 3) & ~3;
+    if (t < jb4) piv[t] = t < jb ? ipiv[k0 + t] : k0 + t;  // padding: a row swapped with itself
     __syncthreads();
     const int hi = k0 + jb;
     int dst = -1;
@@ -41,28 +44,40 @@ __global__ __launch_bounds__(512) void piv_pairs(const int* __restrict__ ipiv, i
         const int s0 = t - jb;
         const int p = piv[s0];
         if (p >= hi) {
-            dst = p;
-            for (int s = 0; s < s0; ++s)
-                if (piv[s] == p) {  // not the first step that touched this row
-                    dst = -1;
-                    break;
-                }
+            bool dup = false;
+            for (int s = 0; s < jb4; s += 4) {
+                const int4 q = *(const int4*)&piv[s];
+                dup = dup || (s < s0 && q.x == p) || (s + 1 < s0 && q.y == p) || (s + 2 < s0 && q.z == p) ||
+                      (s + 3 < s0 && q.w == p);
+            }
+            dst = dup ? -1 : p;
         }
     }
     int x = dst;
-    if (dst >= 0) {
-        for (int s = jb - 1; s >= 0; --s) {
-            const int a = k0 + s, b = piv[s];
-            x = (x == a) ? b : (x == b) ? a : x;
-        }
+    for (int s = jb4 - 4; s >= 0; s -= 4) {
+        const int4 q = *(const int4*)&piv[s];
+        x = (x == k0 + s + 3) ? q.w : (x == q.w) ? k0 + s + 3 : x;
+        x = (x == k0 + s + 2) ? q.z : (x == q.z) ? k0 + s + 2 : x;
+        x = (x == k0 + s + 1) ? q.y : (x == q.y) ? k0 + s + 1 : x;
+        x = (x == k0 + s) ? q.x : (x == q.x) ? k0 + s : x;
     }
-    if (dst >= 0 && x != dst) {
-        const int at = atomicAdd(&cnt, 1);
-        out[1 + 2 * at] = dst;
-        out[2 + 2 * at] = x;
-    }
+    const bool emit = dst >= 0 && x != dst;
+    const unsigned long long bal = __ballot(emit);
+    const int lane = t & 63, wave = t >> 6;
+    if (lane == 0) wcount[wave] = __popcll(bal);
     __syncthreads();
-    if (t == 0) out[0] = cnt;
+    int base = 0;
+    for (int w = 0; w < wave; ++w) base += wcount[w];
+    if (emit) {
+        const int r = base + __popcll(bal & ((1ull << lane) - 1ull));
+        out[1 + 2 * r] = dst;
+        out[2 + 2 * r] = x;
+    }
+    if (t == 0) {
+        int total = 0;
+        for (int w = 0; w < 8; ++w) total += wcount[w];
+        out[0] = total;
+    }
 }
 
 // One wave per column: columns [0, n0) of A0, then [0, n1) of A1, then (if perm) the vector

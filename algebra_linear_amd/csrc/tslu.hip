@@ -218,9 +218,11 @@ __global__ __launch_bounds__(256, 1) void tslu_k(const float* __restrict__ F, si
         gepp_rows(x, active, row, step, L);
         if (level == tree.levels - 1) {
             // root: L11 \ U11 in pivot order (winners and pivots are in LDS already)
-            if (step >= 0)
+            if (step >= 0) {  // row-major: row `step` of L11 \ U11 is 64 consecutive floats
+                float4* dst = (float4*)(D + step * W);
 #pragma unroll
-                for (int c = 0; c < W; ++c) D[step + c * W] = x[c];
+                for (int v = 0; v < W / 4; ++v) dst[v] = float4{x[4 * v], x[4 * v + 1], x[4 * v + 2], x[4 * v + 3]};
+            }
             break;
         }
         // nominate: original values and row index of the rows chosen as pivots
@@ -328,37 +330,45 @@ __global__ __launch_bounds__(256, 1) void tslu_k(const float* __restrict__ F, si
 }
 
 // The rest of the panel once its rows are interchanged: rows J..J+63 receive L11 \ U11 from
-// D; every row below solves x U11 = a (one thread per row, U11 broadcast from LDS).
+// D (row-major); every row below solves x U11 = a, one thread per row.  U11 is the same for
+// every thread: it comes through scalar loads as FMA operands, and each pivot is applied as a
+// multiplication by its reciprocal (v_rcp_f32, 1 ulp), so the 64-step chain has no LDS round
+// trips and no division sequences.
 __global__ __launch_bounds__(256) void tslu_l21_k(float* __restrict__ F, size_t ld, int J, int m,
                                                   const float* __restrict__ D) {
-    __shared__ float U[W][W + 1];
     const int t = threadIdx.x;
     if (blockIdx.x == 0) {
         for (int idx = t; idx < W * W; idx += 256) {
             const int i = idx & 63, c = idx >> 6;
-            F[(size_t)(J + i) + (size_t)(J + c) * ld] = D[i + c * W];
+            F[(size_t)(J + i) + (size_t)(J + c) * ld] = D[i * W + c];
         }
         return;
     }
-    for (int idx = t; idx < W * W; idx += 256) {
-        const int i = idx & 63, c = idx >> 6;
-        U[i][c] = c >= i ? D[i + c * W] : 0.f;
-    }
-    __syncthreads();
     const int r = W + (blockIdx.x - 1) * 256 + t;
     if (r >= m) return;
-    float* rowp = F + (size_t)(J + r);
+    float* const row0 = F + (size_t)(J + r) + (size_t)J * ld;  // the row's entry in column J
     float x[W];
+    {
+        const float* p = row0;
 #pragma unroll
-    for (int c = 0; c < W; ++c) x[c] = rowp[(size_t)(J + c) * ld];
-#pragma unroll
-    for (int j = 0; j < W; ++j) {
-        x[j] = x[j] / U[j][j];
-#pragma unroll
-        for (int c = j + 1; c < W; ++c) x[c] = fmaf(-x[j], U[j][c], x[c]);
+        for (int c = 0; c < W; ++c) {
+            x[c] = *p;
+            p += ld;
+        }
     }
 #pragma unroll
-    for (int c = 0; c < W; ++c) rowp[(size_t)(J + c) * ld] = x[c];
+    for (int j = 0; j < W; ++j) {
+        const float* uj = D + j * W;  // row j of U11
+        x[j] *= __builtin_amdgcn_rcpf(uj[j]);
+#pragma unroll
+        for (int c = j + 1; c < W; ++c) x[c] = fmaf(-x[j], uj[c], x[c]);
+    }
+    float* p = row0;
+#pragma unroll
+    for (int c = 0; c < W; ++c) {
+        *p = x[c];
+        p += ld;
+    }
 }
 
 Tree make_tree(int m) {

@@ -256,6 +256,7 @@ struct Arena {
     void* tslu = nullptr;     // tournament panel scratch (nominees, L11 \ U11)
     unsigned* counters = nullptr;  // its arrival counters (zero between panels)
     int* pairs_side = nullptr;     // row-interchange pairs of the panel stream's swaps
+    int* pairs_outer[2] = {nullptr, nullptr};  // an outer panel's pairs, alternating by panel
 };
 
 constexpr int NORM_A = 0, NORM_B = 1, NORM_R = 17, NORM_X = 33, NORM_WORDS = 64;
@@ -308,12 +309,14 @@ int make_arena(hipStream_t s, Workspace* ws, size_t n, size_t np, size_t nr, siz
     const size_t oT = off;
     if (lowp) off += align256(tslu_scratch_bytes(np));
     const size_t oS = off;
-    if (lowp) off += align256((1 + 2 * 512) * sizeof(int));
+    if (lowp) off += 3 * align256((1 + 2 * 512) * sizeof(int));
     AL_CHECK(grow(&ws->mbuf, &ws->mbuf_bytes, off));
     char* base = (char*)ws->mbuf;
     AL_CHECK(fill_u32(s, (unsigned*)base, 1024, 0u));
     a->tslu = base + oT;
     a->pairs_side = lowp ? (int*)(base + oS) : nullptr;
+    a->pairs_outer[0] = lowp ? (int*)(base + oS + align256((1 + 2 * 512) * sizeof(int))) : nullptr;
+    a->pairs_outer[1] = lowp ? (int*)(base + oS + 2 * align256((1 + 2 * 512) * sizeof(int))) : nullptr;
     a->counters = (unsigned*)base;
     a->F = (float*)(base + oF);
     a->ipiv = (int*)(base + oV);
@@ -384,11 +387,14 @@ int ensure_side_stream(Workspace* ws) {
 // Blocked right-looking LU with partial pivoting of the np x np FP32 matrix a.F
 // (np % 64 == 0); pivots as global row indices in a.ipiv, status in a.minfo[0].
 //
-// Look-ahead on two streams: the next outer panel's columns are updated first, and while the
-// main stream applies the rest of the trailing update the side stream factors that panel (its
-// tournaments, interchanges inside the panel, L21 and inner updates touch only the panel's
-// columns, which the rest of the update does not read or write).  The main stream waits for
-// the panel before applying its interchanges to the other columns.
+// Look-ahead on two streams.  The side stream carries the chain that bounds the run: it
+// factors an outer panel, then prepares the next panel's columns (the panel's interchanges,
+// U12 and the product with L21 on those columns only) and factors that panel, and so on.  The
+// main stream updates every other column behind it: once a panel is factored it applies the
+// panel's interchanges to the columns left of it and right of the next panel, solves U12 and
+// applies the trailing product there.  The next panel's preparation waits for that update of
+// the previous panel (its columns are among those updated); nothing else crosses streams,
+// because the two streams touch disjoint columns.
 int getrf_lowp(hipStream_t s, Workspace* ws, Arena& a, bool bf16) {
     const int np = (int)a.np;
     const int nbo = bf16 ? 512 : 256;
@@ -401,53 +407,57 @@ int getrf_lowp(hipStream_t s, Workspace* ws, Arena& a, bool bf16) {
     AL_CHECK(fill_i32(s, info, 1, 0));
     AL_CHECK(fill_u32(s, ws->abort_flag, 1, 0u));
     AL_CHECK(iota_i32(s, a.ipiv, a.np));
-    AL_HIP(hipEventRecord(ws->ev_main, s));
+    AL_HIP(hipEventRecord(ws->ev_main, s));  // also "the previous update is done" for the first panel
     AL_HIP(hipStreamWaitEvent(side, ws->ev_main, 0));
-    AL_CHECK(factor_panel(side, ws, a, 0, np < nbo ? np : nbo, a.pairs_side));
-    AL_HIP(hipEventRecord(ws->ev_side, side));
-    // the outer panel [K, KE)'s interchanges on columns [c0, c1) and, when n1 > 0, on [c2, c2 + n1)
-    auto swaps = [&](int K, int KE, int c0, int n0, int c2, int n1) -> int {
+    // the outer panel [K, KE)'s interchanges on columns [c0, c0 + n0) and [c2, c2 + n1); with
+    // `pairs` (one pivot block) the composed pairs are kept there
+    auto swaps = [&](hipStream_t q, int K, int KE, int c0, int n0, int c2, int n1, int* pairs) -> int {
         for (int J = K; J < KE; J += 256) {
             const int jb = (KE - J) < 256 ? (KE - J) : 256;
-            AL_CHECK(swap_block_f32(s, ws, a.ipiv, J, jb, at(0, c0), n0, n1 > 0 ? at(0, c2) : nullptr, n1, ld, info));
+            AL_CHECK(swap_block_f32(q, ws, a.ipiv, J, jb, at(0, c0), n0, n1 > 0 ? at(0, c2) : nullptr, n1, ld, info,
+                                    pairs));
         }
         return AL_OK;
     };
     // U12 := L11^{-1} A12 on columns [c0, c1), 64 rows at a time
-    auto u12 = [&](int K, int KE, int c0, int c1) -> int {
+    auto u12 = [&](hipStream_t q, int K, int KE, int c0, int c1) -> int {
         if (c1 <= c0) return AL_OK;
         for (int J = K; J < KE; J += PB) {
-            AL_CHECK(trsm_ll64(s, a, J, c0, c1));
+            AL_CHECK(trsm_ll64(q, a, J, c0, c1));
             if (J + PB < KE)
-                AL_CHECK(gemm_f32(s, false, false, KE - J - PB, c1 - c0, PB, -1.f, at(J + PB, J), ld, at(J, c0), ld, 1.f,
+                AL_CHECK(gemm_f32(q, false, false, KE - J - PB, c1 - c0, PB, -1.f, at(J + PB, J), ld, at(J, c0), ld, 1.f,
                                   at(J + PB, c0), ld));
         }
         return AL_OK;
     };
+    AL_CHECK(factor_panel(side, ws, a, 0, np < nbo ? np : nbo, a.pairs_side));
     for (int K = 0; K < np; K += nbo) {
         const int kb = (np - K) < nbo ? (np - K) : nbo;
         const int KE = K + kb;
-        AL_HIP(hipStreamWaitEvent(s, ws->ev_side, 0));  // the outer panel [K, KE) is factored
-        if (KE >= np) {
-            AL_CHECK(swaps(K, KE, 0, K, 0, 0));
+        if (KE >= np) {  // the last panel: only the columns left of it remain
+            AL_HIP(hipEventRecord(ws->ev_side, side));
+            AL_HIP(hipStreamWaitEvent(s, ws->ev_side, 0));
+            AL_CHECK(swaps(s, K, KE, 0, K, 0, 0, nullptr));
             break;
         }
-        // the next outer panel's columns first (interchanges, U12, update), so the side stream
-        // can start factoring it; then the other columns
         const int KE2 = (np - KE) < nbo ? np : KE + nbo;
-        AL_CHECK(swaps(K, KE, KE, KE2 - KE, 0, 0));
-        AL_CHECK(u12(K, KE, KE, KE2));
-        AL_CHECK(trailing_update(s, a, bf16, K, KE, KE, KE2));
-        AL_HIP(hipEventRecord(ws->ev_main, s));
+        int* pairs = kb <= 256 ? a.pairs_outer[(K / nbo) & 1] : nullptr;
+        // side: the next panel's columns, after the main stream's last update of them
         AL_HIP(hipStreamWaitEvent(side, ws->ev_main, 0));
-        AL_CHECK(factor_panel(side, ws, a, KE, KE2, a.pairs_side));
+        AL_CHECK(swaps(side, K, KE, KE, KE2 - KE, 0, 0, pairs));
+        AL_CHECK(u12(side, K, KE, KE, KE2));
+        AL_CHECK(trailing_update(side, a, bf16, K, KE, KE, KE2));
         AL_HIP(hipEventRecord(ws->ev_side, side));
-        if (KE - K <= 256)  // one pivot block: its pairs are still in ws->pairs from the first call
-            AL_CHECK(swap_apply_f32(s, ws->pairs, at(0, 0), K, at(0, KE2), np - KE2, ld, info));
+        AL_CHECK(factor_panel(side, ws, a, KE, KE2, a.pairs_side));
+        // main: every other column
+        AL_HIP(hipStreamWaitEvent(s, ws->ev_side, 0));
+        if (pairs)
+            AL_CHECK(swap_apply_f32(s, pairs, at(0, 0), K, at(0, KE2), np - KE2, ld, info));
         else
-            AL_CHECK(swaps(K, KE, 0, K, KE2, np - KE2));
-        AL_CHECK(u12(K, KE, KE2, np));
+            AL_CHECK(swaps(s, K, KE, 0, K, KE2, np - KE2, nullptr));
+        AL_CHECK(u12(s, K, KE, KE2, np));
         AL_CHECK(trailing_update(s, a, bf16, K, KE, KE2, np));
+        AL_HIP(hipEventRecord(ws->ev_main, s));
     }
     return perm_from_ipiv(s, a.ipiv, a.np, a.perm);
 }

@@ -112,9 +112,15 @@ int ws_reserve_flags(Workspace* ws, size_t entries);
 // One-launch blocked triangular solve of op = lower/upper, unit or not, with nrhs <= 16
 // right-hand sides in X (n x nrhs, FP64); the matrix may be FP64 or FP32.
 bool trsv_fits(size_t nrhs);
+// Dinv (FP32 matrices, nrhs <= 4): the 128 x 128 diagonal blocks' inverses from
+// trsv_diag_inverse, applied as products instead of substitutions; may be null.
 template <typename TA>
 int trsv_solve(hipStream_t s, Workspace* ws, bool lower, bool unit, size_t n, size_t nrhs, const TA* A, size_t lda,
-               double* X, size_t ldx);
+               double* X, size_t ldx, const double* Dinv = nullptr);
+// FP64 inverses of the 128-row diagonal blocks of the unit-lower and the upper triangle of an
+// FP32 LU factor (n x n in A): trsv_diag_inverse_len(n) doubles each.
+size_t trsv_diag_inverse_len(size_t n);
+int trsv_diag_inverse(hipStream_t s, size_t n, const float* A, size_t lda, double* Linv, double* Uinv);
 
 // Grows the workspace's perm (n entries) and staging buffer (nw doubles).
 int ws_reserve(Workspace* ws, size_t n, size_t nw);

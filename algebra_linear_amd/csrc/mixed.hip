@@ -257,6 +257,7 @@ struct Arena {
     unsigned* counters = nullptr;  // its arrival counters (zero between panels)
     int* pairs_side = nullptr;     // row-interchange pairs of the panel stream's swaps
     int* pairs_outer[2] = {nullptr, nullptr};  // an outer panel's pairs, alternating by panel
+    double* dinv = nullptr;   // FP64 inverses of the factor's 128-row diagonal blocks: L's, then U's
 };
 
 constexpr int NORM_A = 0, NORM_B = 1, NORM_R = 17, NORM_X = 33, NORM_WORDS = 64;
@@ -280,7 +281,7 @@ int grow(void** p, size_t* have, size_t want) {
 }
 
 int make_arena(hipStream_t s, Workspace* ws, size_t n, size_t np, size_t nr, size_t nbo, bool lowp, bool bf16,
-               Arena* a) {
+               bool refine, Arena* a) {
     a->n = n;
     a->np = np;
     a->nr = nr;
@@ -310,6 +311,8 @@ int make_arena(hipStream_t s, Workspace* ws, size_t n, size_t np, size_t nr, siz
     if (lowp) off += align256(tslu_scratch_bytes(np));
     const size_t oS = off;
     if (lowp) off += 3 * align256((1 + 2 * 512) * sizeof(int));
+    const size_t oD = off;
+    if (refine) off += 2 * align256(trsv_diag_inverse_len(np) * sizeof(double));
     AL_CHECK(grow(&ws->mbuf, &ws->mbuf_bytes, off));
     char* base = (char*)ws->mbuf;
     AL_CHECK(fill_u32(s, (unsigned*)base, 1024, 0u));
@@ -317,6 +320,7 @@ int make_arena(hipStream_t s, Workspace* ws, size_t n, size_t np, size_t nr, siz
     a->pairs_side = lowp ? (int*)(base + oS) : nullptr;
     a->pairs_outer[0] = lowp ? (int*)(base + oS + align256((1 + 2 * 512) * sizeof(int))) : nullptr;
     a->pairs_outer[1] = lowp ? (int*)(base + oS + 2 * align256((1 + 2 * 512) * sizeof(int))) : nullptr;
+    a->dinv = refine ? (double*)(base + oD) : nullptr;
     a->counters = (unsigned*)base;
     a->F = (float*)(base + oF);
     a->ipiv = (int*)(base + oV);
@@ -571,7 +575,8 @@ int gesv_mixed(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const double
         return AL_ERR_ARG;
     }
     Arena a;
-    AL_CHECK(make_arena(s, ws, n, np, nr_max, bf16 ? 512 : 256, factor != AL_FACTOR_FP64 && !reuse, bf16, &a));
+    AL_CHECK(make_arena(s, ws, n, np, nr_max, bf16 ? 512 : 256, factor != AL_FACTOR_FP64 && !reuse, bf16,
+                        factor != AL_FACTOR_FP64, &a));
     if (reuse) {
         a.F = reuse->F;
         a.perm = reuse->perm;
@@ -619,6 +624,8 @@ int gesv_mixed(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const double
         }
         stats.factor_breakdown = minfo;
         if (minfo == 0) {
+            // the correction solves apply the factor's 128-row diagonal blocks as FP64 inverses
+            AL_CHECK(trsv_diag_inverse(s, np, a.F, np, a.dinv, a.dinv + trsv_diag_inverse_len(np)));
             berr_low = 0.0;
             ok_low = true;
             for (size_t c0 = 0; c0 < nrhs; c0 += NR_MAX) {
@@ -639,8 +646,9 @@ int gesv_mixed(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const double
                     hipLaunchKernelGGL(gather_perm_k, dim3(ceil_div(np * nr, 256)), dim3(256), 0, s, (int)n, (int)np,
                                        (int)nr, a.perm, a.R, n, a.Y);
                     AL_HIP(hipGetLastError());
-                    AL_CHECK(trsv_solve<float>(s, ws, true, true, np, nr, a.F, np, a.Y, np));
-                    AL_CHECK(trsv_solve<float>(s, ws, false, false, np, nr, a.F, np, a.Y, np));
+                    AL_CHECK(trsv_solve<float>(s, ws, true, true, np, nr, a.F, np, a.Y, np, a.dinv));
+                    AL_CHECK(trsv_solve<float>(s, ws, false, false, np, nr, a.F, np, a.Y, np,
+                                               a.dinv + trsv_diag_inverse_len(np)));
                     hipLaunchKernelGGL(add_k, dim3(ceil_div(n * nr, 256)), dim3(256), 0, s, (int)n, (int)np, (int)nr, a.Y,
                                        Xc, ldx, it > 1 ? 1 : 0);
                     AL_HIP(hipGetLastError());

@@ -179,32 +179,53 @@ constexpr int NR2_MAX = 8;
 template <int V>
 using ic = std::integral_constant<int, V>;
 
-template <int NR>
+constexpr int DINV_ROW = TB2 + 2;  // doubles per row of a staged diagonal-block inverse
+constexpr int NR_INV_MAX = 4;      // right-hand sides of the inverse-based solve (LDS bound)
+
+template <bool INV, int NR>
 constexpr size_t trsv128_lds_bytes() {
-    return (size_t)TB2 * (TB2 + 1) * sizeof(float) + (size_t)4 * TB2 * NR * sizeof(double) + TB2 * sizeof(double) + 16;
+    return INV ? (size_t)TB2 * DINV_ROW * sizeof(double) + (size_t)4 * TB2 * NR * sizeof(double) +
+                     (size_t)TB2 * NR * sizeof(double) + 16
+               : (size_t)TB2 * (TB2 + 1) * sizeof(float) + (size_t)4 * TB2 * NR * sizeof(double) +
+                     TB2 * sizeof(double) + 16;
 }
 
-template <bool LOWER, bool UNIT, int NR>
+// INV: the block's diagonal triangle is applied as its precomputed FP64 inverse Dinv (one
+// 128 x 128 column-major block per row block), staged into LDS while the block waits; the
+// diagonal step of the critical path is then a matrix-vector product over all four waves
+// instead of a 128-step substitution.
+template <bool LOWER, bool UNIT, int NR, bool INV>
 __global__ __launch_bounds__(256) void trsv128_f32(int n, int nrhs, const float* __restrict__ A, size_t lda,
                                                    double* __restrict__ X, size_t ldx, unsigned* __restrict__ flags,
-                                                   unsigned gen, unsigned* __restrict__ abort_flag, long long timeout) {
+                                                   unsigned gen, unsigned* __restrict__ abort_flag, long long timeout,
+                                                   const double* __restrict__ Dinv) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     double* red = (double*)lds_raw;                       // [4][TB2][NR]
-    double* rdiag = red + 4 * TB2 * NR;                   // [TB2]
-    float* D = (float*)(rdiag + TB2);                     // [TB2][TB2 + 1], strictly triangular
-    int* ok_lds = (int*)(D + TB2 * (TB2 + 1));
+    double* rdiag = red + 4 * TB2 * NR;                   // [TB2] (substitution)
+    double* ys = rdiag;                                   // [TB2][NR] (inverse)
+    float* D = (float*)(rdiag + TB2);                     // [TB2][TB2 + 1], strictly triangular (substitution)
+    double* Di = ys + TB2 * NR;                           // [TB2][DINV_ROW], row-major (inverse)
+    int* ok_lds = INV ? (int*)(Di + TB2 * DINV_ROW) : (int*)(D + TB2 * (TB2 + 1));
     const int nblk = (n + TB2 - 1) / TB2;
     const int b = LOWER ? (int)blockIdx.x : nblk - 1 - (int)blockIdx.x;
     const int r0 = b * TB2;
     const int nb = (n - r0) < TB2 ? (n - r0) : TB2;
     const int t = threadIdx.x, lane = t & 63, part = t >> 6;
 
-    for (int idx = t; idx < TB2 * TB2; idx += 256) {
-        const int i = idx & (TB2 - 1), k = idx >> 7;
-        const bool tri = LOWER ? (k < i) : (k > i);
-        D[i * (TB2 + 1) + k] = (i < nb && k < nb && tri) ? A[(size_t)(r0 + i) + (size_t)(r0 + k) * lda] : 0.f;
+    if (INV) {
+        const double* src = Dinv + (size_t)b * TB2 * TB2;  // column-major: src[k * 128 + i] = Dinv[i][k]
+        for (int idx = t; idx < TB2 * TB2; idx += 256) {
+            const int i = idx & (TB2 - 1), k = idx >> 7;
+            Di[i * DINV_ROW + k] = src[idx];
+        }
+    } else {
+        for (int idx = t; idx < TB2 * TB2; idx += 256) {
+            const int i = idx & (TB2 - 1), k = idx >> 7;
+            const bool tri = LOWER ? (k < i) : (k > i);
+            D[i * (TB2 + 1) + k] = (i < nb && k < nb && tri) ? A[(size_t)(r0 + i) + (size_t)(r0 + k) * lda] : 0.f;
+        }
+        if (!UNIT && t < TB2) rdiag[t] = t < nb ? 1.0 / (double)A[(size_t)(r0 + t) + (size_t)(r0 + t) * lda] : 0.0;
     }
-    if (!UNIT && t < TB2) rdiag[t] = t < nb ? 1.0 / (double)A[(size_t)(r0 + t) + (size_t)(r0 + t) * lda] : 0.0;
 
     double acc0[NR], acc1[NR];
 #pragma unroll
@@ -213,6 +234,13 @@ __global__ __launch_bounds__(256) void trsv128_f32(int n, int nrhs, const float*
     const int ndep = LOWER ? b : nblk - 1 - b;
     const int ra = r0 + lane, rb = r0 + 64 + lane;
     const bool oka = ra < n, okb = rb < n;
+    // this block's right-hand side, read now rather than on the critical path
+    double ba[NR], bb[NR];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        ba[k] = (part == 0 && k < nrhs && oka) ? X[(size_t)ra + (size_t)k * ldx] : 0.0;
+        bb[k] = (part == 0 && k < nrhs && okb) ? X[(size_t)rb + (size_t)k * ldx] : 0.0;
+    }
     // rows past n read row 0 and columns past n (only in an upper solve's partial last block)
     // column 0, and are replaced by zeros: the loads are branch-free and walk the columns with a
     // running pointer
@@ -300,18 +328,62 @@ __global__ __launch_bounds__(256) void trsv128_f32(int n, int nrhs, const float*
         red[(part * TB2 + 64 + lane) * NR + k] = acc1[k];
     }
     __syncthreads();
-    if (part != 0) return;
+    if (!INV && part != 0) return;
     double ya[NR], yb[NR];
+    if (part == 0) {
 #pragma unroll
-    for (int k = 0; k < NR; ++k) {
-        const double sa = ((red[lane * NR + k] + red[(TB2 + lane) * NR + k]) + red[(2 * TB2 + lane) * NR + k]) +
-                          red[(3 * TB2 + lane) * NR + k];
-        const double sb = ((red[(64 + lane) * NR + k] + red[(TB2 + 64 + lane) * NR + k]) +
-                           red[(2 * TB2 + 64 + lane) * NR + k]) +
-                          red[(3 * TB2 + 64 + lane) * NR + k];
-        ya[k] = (k < nrhs && oka) ? X[(size_t)ra + (size_t)k * ldx] - sa : 0.0;
-        yb[k] = (k < nrhs && okb) ? X[(size_t)rb + (size_t)k * ldx] - sb : 0.0;
+        for (int k = 0; k < NR; ++k) {
+            const double sa = ((red[lane * NR + k] + red[(TB2 + lane) * NR + k]) + red[(2 * TB2 + lane) * NR + k]) +
+                              red[(3 * TB2 + lane) * NR + k];
+            const double sb = ((red[(64 + lane) * NR + k] + red[(TB2 + 64 + lane) * NR + k]) +
+                               red[(2 * TB2 + 64 + lane) * NR + k]) +
+                              red[(3 * TB2 + 64 + lane) * NR + k];
+            ya[k] = (k < nrhs && oka) ? ba[k] - sa : 0.0;
+            yb[k] = (k < nrhs && okb) ? bb[k] - sb : 0.0;
+        }
     }
+    if (INV) {
+        // x = Dinv y: wave 0 publishes y, every thread sums its two rows over its 32 columns,
+        // wave 0 adds the four partial sums in a fixed order
+        if (part == 0) {
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                ys[lane * NR + k] = ya[k];
+                ys[(64 + lane) * NR + k] = yb[k];
+            }
+        }
+        __syncthreads();
+        double pa[NR], pb[NR];
+#pragma unroll
+        for (int k = 0; k < NR; ++k) pa[k] = pb[k] = 0.0;
+        const double* Ra = Di + lane * DINV_ROW + 32 * part;
+        const double* Rb = Di + (64 + lane) * DINV_ROW + 32 * part;
+#pragma unroll
+        for (int kk = 0; kk < 32; ++kk) {
+            const double da = Ra[kk], db = Rb[kk];
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                const double yv = ys[(32 * part + kk) * NR + k];
+                pa[k] = fma(da, yv, pa[k]);
+                pb[k] = fma(db, yv, pb[k]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            red[(part * TB2 + lane) * NR + k] = pa[k];
+            red[(part * TB2 + 64 + lane) * NR + k] = pb[k];
+        }
+        __syncthreads();
+        if (part != 0) return;
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            ya[k] = ((red[lane * NR + k] + red[(TB2 + lane) * NR + k]) + red[(2 * TB2 + lane) * NR + k]) +
+                    red[(3 * TB2 + lane) * NR + k];
+            yb[k] = ((red[(64 + lane) * NR + k] + red[(TB2 + 64 + lane) * NR + k]) +
+                     red[(2 * TB2 + 64 + lane) * NR + k]) +
+                    red[(3 * TB2 + 64 + lane) * NR + k];
+        }
+    } else {
     // The diagonal block in groups of 8 rows, in solve order.  For a group every lane reads the
     // 8 current values by readlane and solves the 8 x 8 triangle itself (a dependency chain of 8
     // short steps instead of 8 lane-to-lane broadcasts), then updates the rows it owns that come
@@ -391,6 +463,7 @@ __global__ __launch_bounds__(256) void trsv128_f32(int n, int nrhs, const float*
 #pragma unroll
         for (int g = 7; g >= 0; --g) group(ic<0>{}, g, std::false_type{}, ya, yb);
     }
+    }
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
         if (k < nrhs) {
@@ -401,18 +474,124 @@ __global__ __launch_bounds__(256) void trsv128_f32(int n, int nrhs, const float*
     if (lane == 0) __hip_atomic_store(&flags[b * 16], gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <bool LOWER, bool UNIT, int NR>
-int launch128(hipStream_t s, unsigned nblk, int n, int nrhs, const float* A, size_t lda, double* X, size_t ldx,
-              unsigned* flags, unsigned gen, unsigned* abort_flag, long long timeout) {
+// Inverses of the 128 x 128 diagonal blocks of an LU factor held in one FP32 matrix: Linv of
+// the unit lower triangle (blockIdx.y == 0) and Uinv of the upper triangle (blockIdx.y == 1),
+// in FP64 from the FP32 entries, column-major 128 x 128 per block (identity / zero past the
+// matrix edge).  Column j of a block's inverse is solved by thread j (rows 0..63) and thread
+// 128 + j (rows 64..127): the half solved first publishes its values through LDS, the other
+// half subtracts the off-diagonal block times them and solves its own triangle.
+constexpr int DI_ROW = TB2 + 4;   // floats per staged column of the block
+constexpr int DI_ZROW = 66;       // doubles per published half column
+constexpr size_t DIAG_INV_LDS = (size_t)TB2 * DI_ROW * sizeof(float) + (size_t)TB2 * DI_ZROW * sizeof(double) +
+                                TB2 * sizeof(double);
+
+__global__ __launch_bounds__(256) void diag_inv_k(int n, const float* __restrict__ A, size_t lda,
+                                                  double* __restrict__ Linv, double* __restrict__ Uinv) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    float* S = (float*)lds_raw;                 // S[k * DI_ROW + i] = A[r0 + i][r0 + k]
+    double* Z = (double*)(S + TB2 * DI_ROW);    // Z[j * DI_ZROW + i]: the first-solved half of column j
+    double* rd = Z + TB2 * DI_ZROW;             // reciprocals of U's diagonal
+    const int b = blockIdx.x;
+    const bool upper = blockIdx.y == 1;
+    const int r0 = b * TB2;
+    const int nb = (n - r0) < TB2 ? (n - r0) : TB2;
+    const int t = threadIdx.x;
+    for (int idx = t; idx < TB2 * TB2; idx += 256) {
+        const int i = idx & (TB2 - 1), k = idx >> 7;
+        S[k * DI_ROW + i] = (i < nb && k < nb) ? A[(size_t)(r0 + i) + (size_t)(r0 + k) * lda] : 0.f;
+    }
+    if (t < TB2) rd[t] = (upper && t < nb) ? 1.0 / (double)A[(size_t)(r0 + t) + (size_t)(r0 + t) * lda] : 0.0;
+    __syncthreads();
+    const int j = t & (TB2 - 1);
+    const int h = __builtin_amdgcn_readfirstlane(t >> 7);  // which half of the column, uniform per wave
+    double z[64];
+#pragma unroll
+    for (int i = 0; i < 64; ++i) z[i] = (64 * h + i == j) ? 1.0 : 0.0;
+    if (!upper) {
+        if (h == 0) {  // rows 0..63: L11 z = e_j
+#pragma unroll
+            for (int k = 0; k < 64; ++k) {
+                const double zk = z[k];
+#pragma unroll
+                for (int i = k + 1; i < 64; ++i) z[i] = fma(-(double)S[k * DI_ROW + i], zk, z[i]);
+            }
+#pragma unroll
+            for (int i = 0; i < 64; ++i) Z[j * DI_ZROW + i] = z[i];
+        }
+        __syncthreads();
+        if (h == 1) {  // rows 64..127: L22 z = e_j - L21 z1
+#pragma unroll
+            for (int k = 0; k < 64; ++k) {
+                const double zk = Z[j * DI_ZROW + k];
+#pragma unroll
+                for (int i = 0; i < 64; ++i) z[i] = fma(-(double)S[k * DI_ROW + 64 + i], zk, z[i]);
+            }
+#pragma unroll
+            for (int k = 0; k < 64; ++k) {
+                const double zk = z[k];
+#pragma unroll
+                for (int i = k + 1; i < 64; ++i) z[i] = fma(-(double)S[(64 + k) * DI_ROW + 64 + i], zk, z[i]);
+            }
+        }
+    } else {
+        if (h == 1) {  // rows 64..127: U22 z = e_j, backward
+#pragma unroll
+            for (int k = 63; k >= 0; --k) {
+                z[k] *= rd[64 + k];
+                const double zk = z[k];
+#pragma unroll
+                for (int i = 0; i < k; ++i) z[i] = fma(-(double)S[(64 + k) * DI_ROW + 64 + i], zk, z[i]);
+            }
+#pragma unroll
+            for (int i = 0; i < 64; ++i) Z[j * DI_ZROW + i] = z[i];
+        }
+        __syncthreads();
+        if (h == 0) {  // rows 0..63: U11 z = e_j - U12 z2
+#pragma unroll
+            for (int k = 0; k < 64; ++k) {
+                const double zk = Z[j * DI_ZROW + k];
+#pragma unroll
+                for (int i = 0; i < 64; ++i) z[i] = fma(-(double)S[(64 + k) * DI_ROW + i], zk, z[i]);
+            }
+#pragma unroll
+            for (int k = 63; k >= 0; --k) {
+                z[k] *= rd[k];
+                const double zk = z[k];
+#pragma unroll
+                for (int i = 0; i < k; ++i) z[i] = fma(-(double)S[k * DI_ROW + i], zk, z[i]);
+            }
+        }
+    }
+    double* out = (upper ? Uinv : Linv) + (size_t)b * TB2 * TB2 + (size_t)j * TB2 + 64 * h;
+#pragma unroll
+    for (int i = 0; i < 64; ++i) out[i] = z[i];
+}
+
+template <bool LOWER, bool UNIT, int NR, bool INV>
+int launch128_t(hipStream_t s, unsigned nblk, int n, int nrhs, const float* A, size_t lda, double* X, size_t ldx,
+                unsigned* flags, unsigned gen, unsigned* abort_flag, long long timeout, const double* Dinv) {
     static bool attr_set = false;  // benign race: every thread sets the same value
-    auto fn = trsv128_f32<LOWER, UNIT, NR>;
-    constexpr size_t bytes = trsv128_lds_bytes<NR>();
+    auto fn = trsv128_f32<LOWER, UNIT, NR, INV>;
+    constexpr size_t bytes = trsv128_lds_bytes<INV, NR>();
     if (!attr_set) {
         AL_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
         attr_set = true;
     }
-    hipLaunchKernelGGL(fn, dim3(nblk), dim3(256), bytes, s, n, nrhs, A, lda, X, ldx, flags, gen, abort_flag, timeout);
+    hipLaunchKernelGGL(fn, dim3(nblk), dim3(256), bytes, s, n, nrhs, A, lda, X, ldx, flags, gen, abort_flag, timeout,
+                       Dinv);
     return AL_OK;
+}
+
+template <bool LOWER, bool UNIT, int NR>
+int launch128(hipStream_t s, unsigned nblk, int n, int nrhs, const float* A, size_t lda, double* X, size_t ldx,
+              unsigned* flags, unsigned gen, unsigned* abort_flag, long long timeout, const double* Dinv) {
+    if constexpr (NR <= NR_INV_MAX) {
+        if (Dinv)
+            return launch128_t<LOWER, UNIT, NR, true>(s, nblk, n, nrhs, A, lda, X, ldx, flags, gen, abort_flag, timeout,
+                                                      Dinv);
+    }
+    return launch128_t<LOWER, UNIT, NR, false>(s, nblk, n, nrhs, A, lda, X, ldx, flags, gen, abort_flag, timeout,
+                                               nullptr);
 }
 
 template <typename TA, bool LOWER, bool UNIT, int NR>
@@ -423,7 +602,8 @@ void launch_nr(hipStream_t s, unsigned nblk, int n, int nrhs, const TA* A, size_
 }
 
 template <typename TA, bool LOWER, bool UNIT>
-int launch_trsv(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const TA* A, size_t lda, double* X, size_t ldx) {
+int launch_trsv(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const TA* A, size_t lda, double* X, size_t ldx,
+                const double* Dinv) {
     const size_t nblk = (n + TB - 1) / TB;
     AL_CHECK(ws_reserve_flags(ws, nblk * 16));
     if (ws->trsv_gen > 0xF0000000u) {  // keep generations unique: restart from a cleared array
@@ -439,16 +619,16 @@ int launch_trsv(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const TA* A
         const float* Af = (const float*)A;
         if (nrhs <= 1)
             AL_CHECK((launch128<LOWER, UNIT, 1>(s, g2, (int)n, (int)nrhs, Af, lda, X, ldx, ws->trsv_flags, gen,
-                                                ws->abort_flag, timeout)));
+                                                ws->abort_flag, timeout, Dinv)));
         else if (nrhs <= 2)
             AL_CHECK((launch128<LOWER, UNIT, 2>(s, g2, (int)n, (int)nrhs, Af, lda, X, ldx, ws->trsv_flags, gen,
-                                                ws->abort_flag, timeout)));
+                                                ws->abort_flag, timeout, Dinv)));
         else if (nrhs <= 4)
             AL_CHECK((launch128<LOWER, UNIT, 4>(s, g2, (int)n, (int)nrhs, Af, lda, X, ldx, ws->trsv_flags, gen,
-                                                ws->abort_flag, timeout)));
+                                                ws->abort_flag, timeout, Dinv)));
         else
             AL_CHECK((launch128<LOWER, UNIT, 8>(s, g2, (int)n, (int)nrhs, Af, lda, X, ldx, ws->trsv_flags, gen,
-                                                ws->abort_flag, timeout)));
+                                                ws->abort_flag, timeout, Dinv)));
     } else if (nrhs <= 1)
         launch_nr<TA, LOWER, UNIT, 1>(s, g, (int)n, (int)nrhs, A, lda, X, ldx, ws->trsv_flags, gen, ws->abort_flag, timeout);
     else if (nrhs <= 2)
@@ -473,21 +653,37 @@ bool trsv_fits(size_t nrhs) { return nrhs >= 1 && nrhs <= (size_t)NRHS_MAX; }
 
 template <typename TA>
 int trsv_solve(hipStream_t s, Workspace* ws, bool lower, bool unit, size_t n, size_t nrhs, const TA* A, size_t lda,
-               double* X, size_t ldx) {
+               double* X, size_t ldx, const double* Dinv) {
     if (n == 0 || nrhs == 0) return AL_OK;
     if (!trsv_fits(nrhs) || n > INT32_MAX) {
         set_error("trsv: more right-hand sides than the one-launch solve takes");
         return AL_ERR_ARG;
     }
-    if (lower) return unit ? launch_trsv<TA, true, true>(s, ws, n, nrhs, A, lda, X, ldx)
-                           : launch_trsv<TA, true, false>(s, ws, n, nrhs, A, lda, X, ldx);
-    return unit ? launch_trsv<TA, false, true>(s, ws, n, nrhs, A, lda, X, ldx)
-                : launch_trsv<TA, false, false>(s, ws, n, nrhs, A, lda, X, ldx);
+    if (lower) return unit ? launch_trsv<TA, true, true>(s, ws, n, nrhs, A, lda, X, ldx, Dinv)
+                           : launch_trsv<TA, true, false>(s, ws, n, nrhs, A, lda, X, ldx, Dinv);
+    return unit ? launch_trsv<TA, false, true>(s, ws, n, nrhs, A, lda, X, ldx, Dinv)
+                : launch_trsv<TA, false, false>(s, ws, n, nrhs, A, lda, X, ldx, Dinv);
 }
 
 template int trsv_solve<double>(hipStream_t, Workspace*, bool, bool, size_t, size_t, const double*, size_t, double*,
-                                size_t);
+                                size_t, const double*);
 template int trsv_solve<float>(hipStream_t, Workspace*, bool, bool, size_t, size_t, const float*, size_t, double*,
-                               size_t);
+                               size_t, const double*);
+
+size_t trsv_diag_inverse_len(size_t n) { return (n + TB2 - 1) / TB2 * (size_t)TB2 * TB2; }
+
+int trsv_diag_inverse(hipStream_t s, size_t n, const float* A, size_t lda, double* Linv, double* Uinv) {
+    if (n == 0) return AL_OK;
+    static bool attr_set = false;  // benign race: every thread sets the same value
+    if (!attr_set) {
+        AL_HIP(hipFuncSetAttribute((const void*)diag_inv_k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)DIAG_INV_LDS));
+        attr_set = true;
+    }
+    const unsigned nblk = (unsigned)((n + TB2 - 1) / TB2);
+    hipLaunchKernelGGL(diag_inv_k, dim3(nblk, 2), dim3(256), DIAG_INV_LDS, s, (int)n, A, lda, Linv, Uinv);
+    AL_HIP(hipGetLastError());
+    return AL_OK;
+}
 
 }  // namespace al

@@ -208,16 +208,12 @@ __global__ void copy2d_k(int m, int n, const double* __restrict__ S, size_t lds,
 // ---------------------------------------------------------------------------------------
 
 // X := L^{-1} X for X = F[J:J+64, c0:c1) and L the unit lower triangle of F[J:J+64, J:J+64].
-// One thread per column (64 values in registers), L broadcast from LDS.
-__global__ __launch_bounds__(256) void trsm_ll64_k(float* __restrict__ F, size_t ld, int J, int c0, int c1) {
-    __shared__ float Lc[PB][PB + 1];  // Lc[k][i] = L[i][k]
-    const int t = threadIdx.x;
-    for (int idx = t; idx < PB * PB; idx += 256) {
-        const int i = idx & 63, k = idx >> 6;
-        Lc[k][i] = (i > k) ? F[(size_t)(J + i) + (size_t)(J + k) * ld] : 0.f;
-    }
-    __syncthreads();
-    const int c = c0 + blockIdx.x * 256 + t;
+// One thread per column (64 values in registers).  L is the same for every thread: it is
+// passed as a separate read-only pointer (the block does not overlap X) so its entries arrive
+// through scalar loads as FMA operands, with no LDS round trip on the dependency chain.
+__global__ __launch_bounds__(256) void trsm_ll64_k(float* __restrict__ F, size_t ld, int J, int c0, int c1,
+                                                   const float* __restrict__ L) {
+    const int c = c0 + blockIdx.x * 256 + threadIdx.x;
     if (c >= c1) return;
     float* col = F + (size_t)J + (size_t)c * ld;
     float x[PB];
@@ -230,9 +226,11 @@ __global__ __launch_bounds__(256) void trsm_ll64_k(float* __restrict__ F, size_t
         x[4 * v + 3] = q.w;
     }
 #pragma unroll
-    for (int i = 0; i < PB; ++i)
+    for (int i = 0; i < PB; ++i) {
+        const float* li = L + (size_t)i * ld;  // column i of the block
 #pragma unroll
-        for (int k = i + 1; k < PB; ++k) x[k] = fmaf(-Lc[i][k], x[i], x[k]);
+        for (int k = i + 1; k < PB; ++k) x[k] = fmaf(-li[k], x[i], x[k]);
+    }
 #pragma unroll
     for (int v = 0; v < PB / 4; ++v) *(float4*)(col + 4 * v) = float4{x[4 * v], x[4 * v + 1], x[4 * v + 2], x[4 * v + 3]};
 }
@@ -337,7 +335,8 @@ int make_arena(hipStream_t s, Workspace* ws, size_t n, size_t np, size_t nr, siz
 
 int trsm_ll64(hipStream_t s, Arena& a, int J, int c0, int c1) {
     if (c1 <= c0) return AL_OK;
-    hipLaunchKernelGGL(trsm_ll64_k, dim3(ceil_div((size_t)(c1 - c0), 256)), dim3(256), 0, s, a.F, a.np, J, c0, c1);
+    hipLaunchKernelGGL(trsm_ll64_k, dim3(ceil_div((size_t)(c1 - c0), 256)), dim3(256), 0, s, a.F, a.np, J, c0, c1,
+                       (const float*)(a.F + (size_t)J + (size_t)J * a.np));
     AL_HIP(hipGetLastError());
     return AL_OK;
 }

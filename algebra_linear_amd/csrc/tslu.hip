@@ -48,18 +48,19 @@ struct TsluLds {
     int last;
 };
 
-// Maximum over the wave of values >= -1, on the VALU: prefix maxima inside each row of 16
-// lanes (DPP row_shr 1, 2, 4, 8), then row 0 into row 1 and row 2 into row 3 (row_bcast:15),
-// then rows 0-1 into rows 2-3 (row_bcast:31); lane 63 ends with the maximum.
-__device__ __forceinline__ float wave_max_dpp(float v) {
-    const int id = __float_as_int(-2.f);
-    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(id, __float_as_int(v), 0x111, 0xf, 0xf, false)));
-    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(id, __float_as_int(v), 0x112, 0xf, 0xf, false)));
-    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(id, __float_as_int(v), 0x114, 0xf, 0xf, false)));
-    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(id, __float_as_int(v), 0x118, 0xf, 0xf, false)));
-    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(id, __float_as_int(v), 0x142, 0xa, 0xf, false)));
-    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(id, __float_as_int(v), 0x143, 0xc, 0xf, false)));
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+// Maximum over the wave of the int images of floats that are >= 0 or -1 (non-negative floats
+// order like their bit patterns; -1.0f is negative as an int), on the VALU: prefix maxima inside
+// each row of 16 lanes (DPP row_shr 1, 2, 4, 8), then row 0 into row 1 and row 2 into row 3
+// (row_bcast:15), then rows 0-1 into rows 2-3 (row_bcast:31); lane 63 ends with the maximum.
+// A lane without a source keeps its own value, so no identity element is needed.
+__device__ __forceinline__ int wave_max_dpp(int v) {
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x111, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x112, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x114, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x118, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xa, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xc, 0xf, false));
+    return __builtin_amdgcn_readlane(v, 63);
 }
 
 __device__ __forceinline__ float lane_value(float v, int lane) {
@@ -78,9 +79,11 @@ __device__ __forceinline__ void narrow_step(float (&x)[W], bool& active, int row
                                             int wave) {
     constexpr int b = J & 1;
     constexpr int NV = C0 + KB - J;  // block columns from J on
-    const float m = !active ? -1.f : (x[J] != x[J] ? INFINITY : fabsf(x[J]));
-    const float wm = wave_max_dpp(m);
-    const unsigned long long bal = __ballot(m == wm);  // never empty: some lane holds the maximum
+    const float ax = fabsf(x[J]);
+    const int m = active ? (ax == ax ? __float_as_int(ax) : 0x7f800000) : __float_as_int(-1.f);
+    const int wmi = wave_max_dpp(m);
+    const float wm = __int_as_float(wmi);
+    const unsigned long long bal = __ballot(m == wmi);  // never empty: some lane holds the maximum
     const int wl = __ffsll((long long)bal) - 1;
     if (lane == wl) {
         L.mag[b][wave] = wm;

@@ -3,16 +3,18 @@
 // src/lapack/mod.rs:309-323); this is the GPU extension beside it, with the same result
 // contract measured as normwise backward error.
 //
-// Pipeline (all on one stream):
+// Pipeline:
 //   |A|_inf and |B_j|_inf   row sums / column maxima, FP64
 //   F = fl32(A)             A rounded to FP32, padded to a multiple of 64 with |A|_inf/n * I
-//   P F = L U               blocked right-looking LU with partial pivoting in FP32: 64-wide
-//                           cooperative panels, 64-row U solves, FP32 MFMA updates inside a
-//                           256/512-wide outer block, and the outer trailing update on FP32
-//                           or bf16 matrix cores (FP32 accumulation)
+//   P F = L U               blocked right-looking LU in FP32 with tournament-pivoted 64-column
+//                           panels, 64-row U solves and FP32 MFMA updates inside a 256/512-wide
+//                           outer block, the outer trailing update on FP32 or bf16 matrix cores
+//                           (FP32 accumulation); the panel chain runs one outer panel ahead on
+//                           a second stream
 //   refinement              r = B - A X (FP64, A streamed once per step, deterministic
-//                           blocked sums), d = U^{-1} L^{-1} P r with the FP32 factors and
-//                           FP64 vectors, X += d
+//                           blocked sums), d = U^{-1} L^{-1} P r with the FP32 factors (their
+//                           128-row diagonal blocks applied as FP64 inverses) and FP64
+//                           vectors, X += d
 //   fallback                FP64 pivoted LU (getrf_f64/getrs_f64) on a copy of A
 //
 // Partial pivoting is not optional here: without it (even after a random butterfly

@@ -9,12 +9,15 @@ fixed seed.  A step is one complete solve of A X = B on operands already residen
       (trailing updates on FP32 or bf16 matrix cores), FP64 iterative refinement to an FP64
       backward error, FP64 fallback if refinement fails; A is left untouched;
   fp64 (reported beside it): al_dev_dgetrf + al_dev_dgetrs -- the drop-in FP64 path with the
-      crate's partial pivoting (A restored from a pristine copy inside the timed region).
+      crate's partial pivoting (A restored from a pristine copy inside the timed region);
+  vendor (context): torch.linalg.solve in FP64 on the same device-resident system (rocSOLVER
+      getrf + getrs), the platform's own dense solve.
 
 Throughput is the HPL / HPL-MxP convention: (2/3 n^3 + 2 n^2 nrhs) FP64-equivalent flops per
 second of wall time of a step.  Prints ONE JSON line; extra objects:
   forms        -- every form measured: ms per step, TFLOP/s, refinement steps, fallback,
-                  backward error (on the host, FP64) -- and the mixed/fp64 speedup;
+                  backward error (on the host, FP64) -- and the mixed/fp64 and mixed/vendor
+                  speedups;
   roofline     -- the GEMM doing the O(n^3) work of the headline form (its algorithmic flops
                   / its launches' summed duration, from HIP events around every launch on the
                   stream it runs on, during the timed steps);
@@ -312,7 +315,8 @@ def main() -> int:
     ap.add_argument("--nrhs", type=int, default=1)
     ap.add_argument("--kappa", type=float, default=1e2)
     ap.add_argument("--form", choices=["fp32", "bf16"], default="fp32")
-    ap.add_argument("--also", default="bf16,fp64", help="other forms measured beside the headline")
+    ap.add_argument("--also", default="bf16,fp64,vendor",
+                    help="other forms measured beside the headline (vendor: torch.linalg.solve, rocSOLVER FP64)")
     ap.add_argument("--cpu-sample-n", type=int, default=1536)
     ap.add_argument("--batch", type=int, default=20000, help="systems in the batched leg (0: skip)")
     ap.add_argument("--served", type=int, default=120, help="requests per rate in the served leg (0: skip)")
@@ -377,6 +381,11 @@ def main() -> int:
                 st = lib.al_dev_dgetrs(n, nrhs, A.data_ptr(), n, ipiv.data_ptr(), X.data_ptr(), n, sp)
                 if st != 0:
                     raise RuntimeError(f"getrs: {st} {_lib.last_error()}")
+        elif form == "vendor":
+            # the platform's own FP64 dense solve (rocSOLVER getrf + getrs through torch), same
+            # device-resident system, for context beside this library's FP64 path
+            def step():
+                X.copy_(torch.linalg.solve(At.T, B0t.T).T)
         else:
             code = lapack.FACTORS[form]
 
@@ -416,7 +425,7 @@ def main() -> int:
                "p50_ms": round(float(np.percentile(times, 50)), 3),
                "p99_ms": round(float(np.percentile(times, 99)), 3),
                "backward_error": host_berr(X), "_kernels": kern, "_times": times}
-        if form != "fp64":
+        if form not in ("fp64", "vendor"):
             out.update({"refinement_steps": stats.iterations, "fell_back": stats.fell_back,
                         "backward_error_low_precision": stats.backward_error_low})
         return out
@@ -465,6 +474,8 @@ def main() -> int:
     }
     if "fp64" in forms:
         out["speedup_vs_fp64"] = round(forms["fp64"]["ms_per_step"] / head["ms_per_step"], 3)
+    if "vendor" in forms:
+        out["speedup_vs_vendor_fp64"] = round(forms["vendor"]["ms_per_step"] / head["ms_per_step"], 3)
     del At, B0t
     torch.cuda.empty_cache()
     if args.batch > 0:

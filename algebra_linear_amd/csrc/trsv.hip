@@ -659,6 +659,15 @@ int trsv_solve(hipStream_t s, Workspace* ws, bool lower, bool unit, size_t n, si
         set_error("trsv: more right-hand sides than the one-launch solve takes");
         return AL_ERR_ARG;
     }
+    if (Dinv && std::is_same<TA, float>::value && nrhs > (size_t)NR_INV_MAX) {
+        // the inverse-based solve holds at most NR_INV_MAX right-hand sides: a launch per group
+        // (each launch is bound by its hop chain, not by the number of right-hand sides)
+        for (size_t c0 = 0; c0 < nrhs; c0 += NR_INV_MAX) {
+            const size_t nr = (nrhs - c0) < (size_t)NR_INV_MAX ? (nrhs - c0) : (size_t)NR_INV_MAX;
+            AL_CHECK(trsv_solve<TA>(s, ws, lower, unit, n, nr, A, lda, X + c0 * ldx, ldx, Dinv));
+        }
+        return AL_OK;
+    }
     if (lower) return unit ? launch_trsv<TA, true, true>(s, ws, n, nrhs, A, lda, X, ldx, Dinv)
                            : launch_trsv<TA, true, false>(s, ws, n, nrhs, A, lda, X, ldx, Dinv);
     return unit ? launch_trsv<TA, false, true>(s, ws, n, nrhs, A, lda, X, ldx, Dinv)

@@ -306,6 +306,90 @@ def run_served(per_rate: int, lib, _lib) -> dict:
     return res
 
 
+def run_sweep(ns, kappas, nrhss, factors, lib, _lib, lapack, stream, sp) -> dict:
+    """The mixed-precision configuration's grid: every (n, kappa, nrhs, factor), one warm-up and
+    one timed solve each, beside the FP64 drop-in path per (n, nrhs); backward errors on the
+    device in FP64 (torch), the same normwise formula as the headline's host check."""
+    import numpy as np
+    import torch
+    dev = torch.device("cuda:0")
+    cases = []
+    fp64 = {}
+    for n in ns:
+        g = torch.Generator(device=dev)
+        g.manual_seed(1988 + n)
+
+        def haar():
+            q, r = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device=dev, generator=g))
+            return q * torch.sign(torch.diagonal(r))
+
+        U, V = haar(), haar()
+        for kappa in kappas:
+            sig = kappa ** (-torch.arange(n, dtype=torch.float64, device=dev) / max(n - 1, 1))
+            At = ((U * sig) @ V.T).T.contiguous()   # column-major A
+            anorm = At.abs().sum(dim=0).max()  # |A|_inf: At[j, i] = A[i, j], so A's row sums are At's column sums
+            for nrhs in nrhss:
+                B0t = torch.randn((nrhs, n), dtype=torch.float64, device=dev, generator=g)
+                X = torch.zeros_like(B0t)
+
+                def berr() -> float:
+                    r = B0t.T - At.T @ X.T
+                    return float((r.abs().amax(dim=0) / (anorm * X.T.abs().amax(dim=0) + B0t.T.abs().amax(dim=0))).max())
+
+                def timed(step) -> float:
+                    step()
+                    torch.cuda.synchronize()
+                    e0 = torch.cuda.Event(enable_timing=True)
+                    e1 = torch.cuda.Event(enable_timing=True)
+                    e0.record(stream)
+                    step()
+                    e1.record(stream)
+                    e1.synchronize()
+                    return e0.elapsed_time(e1)
+
+                for factor in factors:
+                    stats = _lib.IrStats()
+                    opt = lapack.ir_options()
+                    code = lapack.FACTORS[factor]
+
+                    def step():
+                        st = lib.al_dev_dgesv_mixed(n, nrhs, At.data_ptr(), n, B0t.data_ptr(), n, X.data_ptr(), n, code,
+                                                    ctypes.byref(opt), ctypes.byref(stats), sp)
+                        if st not in (0, 3):
+                            raise RuntimeError(f"dgesv_mixed: {st} {_lib.last_error()}")
+
+                    ms = timed(step)
+                    cases.append({"n": n, "kappa": kappa, "nrhs": nrhs, "factor": factor, "ms": round(ms, 3),
+                                  "tflops": round(hpl_flops(n, nrhs) / (ms * 1e-3) / 1e12, 3),
+                                  "refinement_steps": stats.iterations, "fell_back": stats.fell_back,
+                                  "backward_error": berr()})
+                if (n, nrhs) not in fp64:
+                    A = torch.empty_like(At)
+                    ipiv = torch.empty(n, dtype=torch.int32, device=dev)
+                    info = torch.zeros(1, dtype=torch.int32, device=dev)
+
+                    def step64():
+                        A.copy_(At)
+                        X.copy_(B0t)
+                        lib.al_dev_dgetrf(n, n, A.data_ptr(), n, ipiv.data_ptr(), info.data_ptr(), sp)
+                        lib.al_dev_dgetrs(n, nrhs, A.data_ptr(), n, ipiv.data_ptr(), X.data_ptr(), n, sp)
+
+                    ms = timed(step64)
+                    fp64[(n, nrhs)] = {"n": n, "nrhs": nrhs, "ms": round(ms, 3), "kappa": kappa,
+                                       "backward_error": berr()}
+                    del A
+                del B0t, X
+            del At
+            torch.cuda.empty_cache()
+        del U, V
+        torch.cuda.empty_cache()
+    for c in cases:
+        c["speedup_vs_fp64"] = round(fp64[(c["n"], c["nrhs"])]["ms"] / c["ms"], 3)
+    return {"cases": cases, "fp64": list(fp64.values()),
+            "note": "one timed solve per case after one warm-up; kappa 1e8 and above is beyond FP32 refinement "
+                    "(kappa * 2^-24 > 1), so those cases end in the FP64 fallback by design"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -321,6 +405,8 @@ def main() -> int:
     ap.add_argument("--batch", type=int, default=20000, help="systems in the batched leg (0: skip)")
     ap.add_argument("--served", type=int, default=120, help="requests per rate in the served leg (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sweep", default="4096,16384",
+                    help="orders of the kappa x nrhs x factor grid of the mixed configuration (empty: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one MI355X; --gpus must be 1 (multi-GPU sharding is "
@@ -478,6 +564,9 @@ def main() -> int:
         out["speedup_vs_vendor_fp64"] = round(forms["vendor"]["ms_per_step"] / head["ms_per_step"], 3)
     del At, B0t
     torch.cuda.empty_cache()
+    if args.sweep:
+        out["sweep"] = run_sweep([int(v) for v in args.sweep.split(",")], [1e2, 1e5, 1e8, 1e11], [1, 16],
+                                 ["fp32", "bf16"], lib, _lib, lapack, stream, sp)
     if args.batch > 0:
         out["batched"] = run_batched(args.batch, args.steps, args.warmup, lib, _lib, stream, sp)
     if args.served > 0:

@@ -35,6 +35,10 @@ second of wall time of a step.  Prints ONE JSON line; extra objects:
                   continuous-batching server in FP64 and in FP32 + FP64 refinement, latency
                   (arrival to result on the host) p50/p99 and solves/s per rate, and the same
                   request stream solved one drop-in call at a time;
+  dropin_fp64  -- the plain FP64 drop-in configuration: dgemm nn/tn/nt/tt at 1024, 4096 and
+                  8192, dgetrf at 8192, dgetrf_rect on 16384 x 2048, dpotrf at 8192;
+  sweep        -- the mixed configuration's grid: n in {4096, 16384}, kappa in {1e2, 1e5, 1e8,
+                  1e11}, nrhs in {1, 16}, FP32 and bf16 factors, beside the FP64 path;
   cpu_baseline -- the CPU oracle (a C restatement of the crate's unblocked dgesv) timed on a
                   bounded sample on one host core.
 """
@@ -390,6 +394,78 @@ def run_sweep(ns, kappas, nrhss, factors, lib, _lib, lapack, stream, sp) -> dict
                     "(kappa * 2^-24 > 1), so those cases end in the FP64 fallback by design"}
 
 
+def run_dropin(lib, stream, sp) -> dict:
+    """The plain FP64 drop-in configuration: dgemm nn/tn/nt/tt at m = n = k in {1024, 4096, 8192}
+    with uniform(-1, 1) operands, dgetrf of an order-8192 uniform matrix, dgetrf_rect of a
+    16384 x 2048 panel and dpotrf of an order-8192 SPD matrix (G G^T + n I), all device-resident;
+    one warm-up and two timed calls each (the factorizations restore their input inside the
+    timed region, as the FP64 solve does)."""
+    import torch
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(7)
+
+    def timed(step, reps=2) -> float:
+        step()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            step()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    out = {"dgemm": []}
+    for n in (1024, 4096, 8192):
+        A = torch.rand(n * n, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+        B = torch.rand(n * n, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+        C = torch.zeros(n * n, dtype=torch.float64, device=dev)
+        for ta, tb, name in ((0, 0, "nn"), (1, 0, "tn"), (0, 1, "nt"), (1, 1, "tt")):
+            def step():
+                st = lib.al_dev_dgemm(ta, tb, n, n, n, 1.0, A.data_ptr(), n, B.data_ptr(), n, 0.0, C.data_ptr(), n, sp)
+                if st != 0:
+                    raise RuntimeError(f"dgemm {st}")
+            ms = timed(step)
+            out["dgemm"].append({"n": n, "op": name, "ms": round(ms, 3), "tflops": round(2.0 * n ** 3 / (ms * 1e-3) / 1e12, 2)})
+        del A, B, C
+    ipiv = torch.empty(16384, dtype=torch.int32, device=dev)
+    info = torch.zeros(1, dtype=torch.int32, device=dev)
+    for m, n, name in ((8192, 8192, "dgetrf"), (16384, 2048, "dgetrf_rect")):
+        A0 = torch.rand(m * n, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+        A = torch.empty_like(A0)
+
+        def step():
+            A.copy_(A0)
+            st = lib.al_dev_dgetrf(m, n, A.data_ptr(), m, ipiv.data_ptr(), info.data_ptr(), sp)
+            if st != 0:
+                raise RuntimeError(f"{name} {st}")
+        ms = timed(step)
+        k = min(m, n)
+        fl = 2.0 * (m * n * k - (m + n) * k * k / 2.0 + k ** 3 / 3.0)  # LAWN 41: 2/3 n^3 when square
+        out[name] = {"m": m, "n": n, "ms": round(ms, 3), "tflops": round(fl / (ms * 1e-3) / 1e12, 2),
+                     "status": int(info.item())}
+        del A0, A
+    n = 8192
+    G = torch.rand(n, n, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+    S0 = (G @ G.T + n * torch.eye(n, dtype=torch.float64, device=dev)).contiguous().reshape(-1)
+    del G
+    S = torch.empty_like(S0)
+
+    def step():
+        S.copy_(S0)
+        st = lib.al_dev_dpotrf(n, S.data_ptr(), n, info.data_ptr(), sp)
+        if st != 0:
+            raise RuntimeError(f"dpotrf {st}")
+    ms = timed(step)
+    out["dpotrf"] = {"n": n, "ms": round(ms, 3), "tflops": round(n ** 3 / 3.0 / (ms * 1e-3) / 1e12, 2),
+                     "status": int(info.item())}
+    del S0, S
+    torch.cuda.empty_cache()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -405,6 +481,7 @@ def main() -> int:
     ap.add_argument("--batch", type=int, default=20000, help="systems in the batched leg (0: skip)")
     ap.add_argument("--served", type=int, default=120, help="requests per rate in the served leg (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the plain FP64 drop-in leg")
     ap.add_argument("--sweep", default="4096,16384",
                     help="orders of the kappa x nrhs x factor grid of the mixed configuration (empty: skip)")
     args = ap.parse_args()
@@ -564,6 +641,8 @@ def main() -> int:
         out["speedup_vs_vendor_fp64"] = round(forms["vendor"]["ms_per_step"] / head["ms_per_step"], 3)
     del At, B0t
     torch.cuda.empty_cache()
+    if not args.no_dropin:
+        out["dropin_fp64"] = run_dropin(lib, stream, sp)
     if args.sweep:
         out["sweep"] = run_sweep([int(v) for v in args.sweep.split(",")], [1e2, 1e5, 1e8, 1e11], [1, 16],
                                  ["fp32", "bf16"], lib, _lib, lapack, stream, sp)

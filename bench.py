@@ -56,6 +56,13 @@ sys.path.insert(0, ROOT)
 
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}   # MI355X dense matrix-core peaks
 CLASSES = ["sgemm_nn", "bgemm", "dgemm_nn", "tslu_tree", "panel_getf2", "trsv", "residual_gemv"]
+# HBM bytes per launch of the trailing-update kernel, from rocprofv3 FETCH_SIZE and WRITE_SIZE
+# over one FP32-form solve at n = 16384 (uniform matrix; the launch shapes do not depend on the
+# values), each counter scaled by what it reads for a copy kernel of known size (FETCH x 2.0,
+# WRITE x 1.0); the command and the per-kernel table are in
+# profiles/sgemm_nn_hbm_traffic_n16384.md.  Reported only for the workload it was measured on.
+MEASURED_TRAFFIC = {("fp32", 16384): {"bytes_per_launch": 87.497e6, "fetch_bytes": 52.849e6, "write_bytes": 34.648e6,
+                                      "source": "profiles/sgemm_nn_hbm_traffic_n16384.md"}}
 
 
 def hpl_flops(n: int, nrhs: int) -> float:
@@ -482,6 +489,9 @@ def main() -> int:
     ap.add_argument("--served", type=int, default=120, help="requests per rate in the served leg (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-dropin", action="store_true", help="skip the plain FP64 drop-in leg")
+    ap.add_argument("--gen", choices=["haar", "uniform"], default="haar",
+                    help="matrix of the headline solve: U diag(sigma) V^T (default) or uniform(-1, 1) entries "
+                         "(for counter-collection runs, where the QR-based generator is not available)")
     ap.add_argument("--sweep", default="4096,16384",
                     help="orders of the kappa x nrhs x factor grid of the mixed configuration (empty: skip)")
     args = ap.parse_args()
@@ -505,12 +515,15 @@ def main() -> int:
         q, r = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device=dev, generator=g))
         return q * torch.sign(torch.diagonal(r))
 
-    U = haar()
-    V = haar()
-    sig = args.kappa ** (-torch.arange(n, dtype=torch.float64, device=dev) / max(n - 1, 1))
-    # column-major storage of A: a row-major tensor holding A^T
-    At = ((U * sig) @ V.T).T.contiguous()
-    del U, V
+    if args.gen == "uniform":
+        At = torch.rand((n, n), dtype=torch.float64, device=dev, generator=g) * 2 - 1
+    else:
+        U = haar()
+        V = haar()
+        sig = args.kappa ** (-torch.arange(n, dtype=torch.float64, device=dev) / max(n - 1, 1))
+        # column-major storage of A: a row-major tensor holding A^T
+        At = ((U * sig) @ V.T).T.contiguous()
+        del U, V
     B0t = torch.randn((nrhs, n), dtype=torch.float64, device=dev, generator=g)   # B^T, row-major
     torch.cuda.empty_cache()
     stream = torch.cuda.Stream(device=dev)  # default priority; everything below is ordered on it
@@ -604,6 +617,8 @@ def main() -> int:
     peak = PEAK_TFLOPS["bf16" if args.form == "bf16" else "f32"]
     avg_ms = gk["total_ms"] / max(gk["launches"], 1)
     achieved = gk["flops"] / max(gk["launches"], 1) / (avg_ms * 1e-3) / 1e12
+    alg_bytes = gk.get("bytes", 0.0) / max(gk["launches"], 1)
+    meas = MEASURED_TRAFFIC.get((args.form, n))
     steps = max(args.steps, 1)
     out = {
         "metric": "dgesv TFLOP/s (HPL-MxP FP64-equivalent flops) at FP64 backward error, mixed-IR vs FP64 LU",
@@ -629,7 +644,11 @@ def main() -> int:
         "forms": {f: {k: v for k, v in d.items() if not k.startswith("_")} for f, d in forms.items()},
         "roofline": {"kernel": f"{gemm_class} (trailing updates of the {args.form} factorization)", "bound": "mfma",
                      "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-                     "frac": round(achieved / peak, 4), "traffic": None,
+                     "frac": round(achieved / peak, 4),
+                     "traffic": round(meas["bytes_per_launch"]) if meas else None,
+                     "traffic_unit": "bytes per launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
+                     "traffic_source": meas["source"] if meas else None,
+                     "algorithmic_bytes_per_launch": round(alg_bytes),
                      "launches_per_step": gk["launches"] // steps, "avg_launch_ms": round(avg_ms, 4),
                      "share_of_step": round(gk["total_ms"] / (head["ms_per_step"] * steps), 4)},
         "kernels": {c: {"ms_per_step": round(v["total_ms"] / steps, 3), "launches_per_step": v["launches"] // steps}

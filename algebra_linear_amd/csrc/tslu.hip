@@ -7,7 +7,9 @@
 // in registers (thread = row) and nominates the 64 rows it pivoted on.  Leaves are merged four
 // at a time: the last workgroup of a group to finish (an atomic arrival counter, no waiting)
 // runs the same elimination on the original values of the 4 x 64 nominees and nominates 64 of
-// them, up to a single root.  The root's elimination of its nominees is the LU of the panel's
+// them, up to a single root.  Every level is a chain of 64 latency-bound steps; a 16384-row
+// panel has four.  Wider workgroups (512 threads: leaves of 512, 8-way merges, three levels)
+// make each step slower by more than the level they save.  The root's elimination of its nominees is the LU of the panel's
 // pivot rows (L11 \ U11); it also writes the pivots as the usual sequence of row interchanges.
 // The caller then applies the interchanges and solves L21 U11 = A21 for the other rows.
 //
@@ -22,8 +24,10 @@ namespace al {
 namespace {
 
 constexpr int W = 64;        // panel width
-constexpr int LEAF = 256;    // rows per leaf (one per thread)
-constexpr int ARITY = 4;     // children per merge (4 x 64 nominees = 256 rows)
+constexpr int NT = 256;      // threads per workgroup
+constexpr int NW = NT / 64;  // waves per workgroup
+constexpr int LEAF = NT;     // rows per leaf (one per thread)
+constexpr int ARITY = NT / W;  // children per merge (4 x 64 nominees = 256 rows)
 constexpr int MAX_LEVELS = 8;
 
 typedef unsigned long long u64;
@@ -38,9 +42,9 @@ constexpr int KB = 8;        // columns per elimination block
 
 // LDS of one panel workgroup.
 struct TsluLds {
-    float mag[2][4];          // per wave: its best candidate's |x| ..
-    int crow[2][4];           // .. its row ..
-    float cval[2][4][KB];     // .. and that row's values in the current block
+    float mag[2][NW];         // per wave: its best candidate's |x| ..
+    int crow[2][NW];          // .. its row ..
+    float cval[2][NW][KB];    // .. and that row's values in the current block
     __attribute__((aligned(16))) float ur[KB][W];  // the block's pivot rows right of the block (U12 after the solve)
     float l11[KB][KB];        // the block's multipliers among its pivot rows
     float piv[W];             // pivots in elimination order
@@ -95,7 +99,7 @@ __device__ __forceinline__ void narrow_step(float (&x)[W], bool& active, int row
     int best = 0;
     float bm = L.mag[b][0];
 #pragma unroll
-    for (int w = 1; w < 4; ++w) {
+    for (int w = 1; w < NW; ++w) {
         const float mw = L.mag[b][w];
         best = mw > bm ? w : best;
         bm = mw > bm ? mw : bm;
@@ -182,7 +186,7 @@ __device__ __forceinline__ void elim_all(std::integer_sequence<int, Bs...>, floa
     (elim_block<Bs * KB>(std::make_integer_sequence<int, KB>{}, x, active, row, step, L, lane, wave), ...);
 }
 
-// Gaussian elimination with partial pivoting of up to 256 rows (one per thread, 64 columns in
+// Gaussian elimination with partial pivoting of up to NT rows (one per thread, 64 columns in
 // registers), blocked by 8 columns: inside a block each step updates only the block's
 // columns; the rest of the row is updated once per block with a rank-8 product.  On return
 // `step` is the step at which this thread's row was chosen (-1 if never); chosen rows hold
@@ -195,7 +199,7 @@ __device__ __forceinline__ void gepp_rows(float (&x)[W], bool active, int row, i
 
 // One launch per panel.  Leaves are workgroups 0..count[0]-1; node n of level l stores its 64
 // nominees' original rows at cand[node][s][0..64) and their global row indices at cidx[node][s].
-__global__ __launch_bounds__(256, 1) void tslu_k(const float* __restrict__ F, size_t ld, int J, int m, Tree tree,
+__global__ __launch_bounds__(NT, 1) void tslu_k(const float* __restrict__ F, size_t ld, int J, int m, Tree tree,
                                                  float* __restrict__ cand, int* __restrict__ cidx,
                                                  unsigned* __restrict__ counters, float* __restrict__ D,
                                                  int* __restrict__ ipiv, int* __restrict__ info,
@@ -203,7 +207,7 @@ __global__ __launch_bounds__(256, 1) void tslu_k(const float* __restrict__ F, si
     __shared__ TsluLds L;
     const int t = threadIdx.x, lane = t & 63, q = t >> 6;
 
-    // leaf: thread t holds row J + 256 b + t
+    // leaf: thread t holds row J + LEAF b + t
     int node = blockIdx.x;
     int level = 0;
     float x[W];
@@ -414,12 +418,12 @@ int tslu_panel(hipStream_t s, float* F, size_t ld, size_t J, size_t m, int* ipiv
     int* cidx = (int*)(cand + (size_t)nodes * W * W);
     float* D = (float*)(cidx + (size_t)nodes * W);
     const int ticket = prof_begin(s);
-    hipLaunchKernelGGL(tslu_k, dim3((unsigned)tr.count[0]), dim3(256), 0, s, F, ld, (int)J, (int)m, tr, cand, cidx,
+    hipLaunchKernelGGL(tslu_k, dim3((unsigned)tr.count[0]), dim3(NT), 0, s, F, ld, (int)J, (int)m, tr, cand, cidx,
                        counters, D, ipiv, info, pairs);
     AL_HIP(hipGetLastError());
     if (ticket >= 0) {
         const double dm = (double)m, b = W;
-        prof_end(ticket, s, "tslu_tree", (dm + (double)(nodes - tr.count[0]) * 256.0) * b * b, 4.0 * dm * b);
+        prof_end(ticket, s, "tslu_tree", (dm + (double)(nodes - tr.count[0]) * (double)LEAF) * b * b, 4.0 * dm * b);
     }
     return AL_OK;
 }

@@ -20,6 +20,7 @@
 // Partial pivoting is not optional here: without it (even after a random butterfly
 // transform) the LU of a dense random matrix has |L||U| / |A| of 1e4-1e5 at n = 1024, enough
 // to stall FP32 refinement at n = 16384 and to make a bf16 update useless.
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -209,32 +210,46 @@ __global__ void copy2d_k(int m, int n, const double* __restrict__ S, size_t lds,
 // LU with partial pivoting, FP32
 // ---------------------------------------------------------------------------------------
 
-// X := L^{-1} X for X = F[J:J+64, c0:c1) and L the unit lower triangle of F[J:J+64, J:J+64].
-// One thread per column (64 values in registers).  L is the same for every thread: it is
-// passed as a separate read-only pointer (the block does not overlap X) so its entries arrive
-// through scalar loads as FMA operands, with no LDS round trip on the dependency chain.
+// X := L^{-1} X for X = F[J:J+64, c0:c1) and L the unit lower triangle of F[J:J+64, J:J+64]
+// (the block does not overlap X).  One wave solves TR_G columns at a time, lane i holding
+// row i of each: the lane keeps row i of L in registers (64 coalesced loads per wave, once),
+// and step j takes x_j of every column from lane j with v_readlane, so the 63 dependent
+// steps wait on neither memory nor LDS.  The FMAs are those of column-oriented forward
+// substitution, in the same order.  Waves stride over the column groups.
+constexpr int TR_G = 4;        // columns per wave and pass
+constexpr int TR_MAX_WG = 512;  // two workgroups per CU; wider blocks loop
 __global__ __launch_bounds__(256) void trsm_ll64_k(float* __restrict__ F, size_t ld, int J, int c0, int c1,
                                                    const float* __restrict__ L) {
-    const int c = c0 + blockIdx.x * 256 + threadIdx.x;
-    if (c >= c1) return;
-    float* col = F + (size_t)J + (size_t)c * ld;
-    float x[PB];
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int nwaves = gridDim.x * 4;
+    const int ncols = c1 - c0;
+    float lrow[PB];
 #pragma unroll
-    for (int v = 0; v < PB / 4; ++v) {
-        const float4 q = *(const float4*)(col + 4 * v);
-        x[4 * v] = q.x;
-        x[4 * v + 1] = q.y;
-        x[4 * v + 2] = q.z;
-        x[4 * v + 3] = q.w;
+    for (int j = 0; j < PB; ++j) lrow[j] = L[(size_t)lane + (size_t)j * ld];  // L(lane, j)
+    for (int g = wave * TR_G; g < ncols; g += nwaves * TR_G) {
+        float x[TR_G];
+        float* col[TR_G];
+#pragma unroll
+        for (int q = 0; q < TR_G; ++q) {
+            const int c = c0 + min(g + q, ncols - 1);  // a short last group repeats its last column, unstored
+            col[q] = F + (size_t)J + (size_t)c * ld;
+            x[q] = col[q][lane];
+        }
+#pragma unroll
+        for (int j = 0; j < PB - 1; ++j) {
+            const bool below = lane > j;
+#pragma unroll
+            for (int q = 0; q < TR_G; ++q) {
+                const float xj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x[q]), j));
+                const float v = fmaf(-lrow[j], xj, x[q]);
+                x[q] = below ? v : x[q];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < TR_G; ++q)
+            if (g + q < ncols) col[q][lane] = x[q];
     }
-#pragma unroll
-    for (int i = 0; i < PB; ++i) {
-        const float* li = L + (size_t)i * ld;  // column i of the block
-#pragma unroll
-        for (int k = i + 1; k < PB; ++k) x[k] = fmaf(-li[k], x[i], x[k]);
-    }
-#pragma unroll
-    for (int v = 0; v < PB / 4; ++v) *(float4*)(col + 4 * v) = float4{x[4 * v], x[4 * v + 1], x[4 * v + 2], x[4 * v + 3]};
 }
 
 // ---------------------------------------------------------------------------------------
@@ -337,7 +352,9 @@ int make_arena(hipStream_t s, Workspace* ws, size_t n, size_t np, size_t nr, siz
 
 int trsm_ll64(hipStream_t s, Arena& a, int J, int c0, int c1) {
     if (c1 <= c0) return AL_OK;
-    hipLaunchKernelGGL(trsm_ll64_k, dim3(ceil_div((size_t)(c1 - c0), 256)), dim3(256), 0, s, a.F, a.np, J, c0, c1,
+    const size_t groups = ceil_div((size_t)(c1 - c0), (size_t)TR_G);
+    const size_t wgs = std::min<size_t>(ceil_div(groups, (size_t)4), (size_t)TR_MAX_WG);
+    hipLaunchKernelGGL(trsm_ll64_k, dim3((unsigned)wgs), dim3(256), 0, s, a.F, a.np, J, c0, c1,
                        (const float*)(a.F + (size_t)J + (size_t)J * a.np));
     AL_HIP(hipGetLastError());
     return AL_OK;

@@ -338,9 +338,10 @@ __global__ __launch_bounds__(NT, 1) void tslu_k(const float* __restrict__ F, siz
 
 // The rest of the panel once its rows are interchanged: rows J..J+63 receive L11 \ U11 from
 // D (row-major); every row below solves x U11 = a, one thread per row.  U11 is the same for
-// every thread: it comes through scalar loads as FMA operands, and each pivot is applied as a
-// multiplication by its reciprocal (v_rcp_f32, 1 ulp), so the 64-step chain has no LDS round
-// trips and no division sequences.
+// every thread: each workgroup stages it in LDS once, and the 64-step chain reads it as
+// broadcast LDS loads (all lanes one address), which the compiler issues well ahead of the
+// FMAs that use them.  Each pivot is applied as a multiplication by its reciprocal
+// (v_rcp_f32, 1 ulp).
 __global__ __launch_bounds__(256) void tslu_l21_k(float* __restrict__ F, size_t ld, int J, int m,
                                                   const float* __restrict__ D) {
     const int t = threadIdx.x;
@@ -351,6 +352,9 @@ __global__ __launch_bounds__(256) void tslu_l21_k(float* __restrict__ F, size_t 
         }
         return;
     }
+    __shared__ __align__(16) float U[W * W];
+    for (int idx = t; idx < W * W / 4; idx += 256) ((float4*)U)[idx] = ((const float4*)D)[idx];
+    __syncthreads();
     const int r = W + (blockIdx.x - 1) * 256 + t;
     if (r >= m) return;
     float* const row0 = F + (size_t)(J + r) + (size_t)J * ld;  // the row's entry in column J
@@ -365,7 +369,7 @@ __global__ __launch_bounds__(256) void tslu_l21_k(float* __restrict__ F, size_t 
     }
 #pragma unroll
     for (int j = 0; j < W; ++j) {
-        const float* uj = D + j * W;  // row j of U11
+        const float* uj = U + j * W;  // row j of U11
         x[j] *= __builtin_amdgcn_rcpf(uj[j]);
 #pragma unroll
         for (int c = j + 1; c < W; ++c) x[c] = fmaf(-x[j], uj[c], x[c]);

@@ -16,6 +16,7 @@
 // The pivots differ from partial pivoting's (the factorization is still P A = L U with |L| <=
 // a small bound in practice); the FP64 drop-in path keeps the crate's partial pivoting.
 #include <cfloat>
+#include <climits>
 #include <utility>
 
 #include "al_internal.hpp"
@@ -40,11 +41,18 @@ struct Tree {
 
 constexpr int KB = 8;        // columns per elimination block
 
+// One wave's candidate of a step: its row's values in the current block and its row index,
+// 64 bytes so that the winner's record is found with a shift.
+struct TsluCand {
+    float v[KB];
+    int row;
+    int pad[7];
+};
+
 // LDS of one panel workgroup.
 struct TsluLds {
     float mag[2][NW];         // per wave: its best candidate's |x| ..
-    int crow[2][NW];          // .. its row ..
-    float cval[2][NW][KB];    // .. and that row's values in the current block
+    TsluCand cand[2][NW];     // .. and that candidate
     __attribute__((aligned(16))) float ur[KB][W];  // the block's pivot rows right of the block (U12 after the solve)
     float l11[KB][KB];        // the block's multipliers among its pivot rows
     float piv[W];             // pivots in elimination order
@@ -56,14 +64,16 @@ struct TsluLds {
 // order like their bit patterns; -1.0f is negative as an int), on the VALU: prefix maxima inside
 // each row of 16 lanes (DPP row_shr 1, 2, 4, 8), then row 0 into row 1 and row 2 into row 3
 // (row_bcast:15), then rows 0-1 into rows 2-3 (row_bcast:31); lane 63 ends with the maximum.
-// A lane without a source keeps its own value, so no identity element is needed.
+// A lane without a source reads INT_MIN, the identity of max, which lets the compiler fold
+// each stage into one v_max_i32 with a DPP operand.
 __device__ __forceinline__ int wave_max_dpp(int v) {
-    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x111, 0xf, 0xf, false));
-    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x112, 0xf, 0xf, false));
-    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x114, 0xf, 0xf, false));
-    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x118, 0xf, 0xf, false));
-    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xa, 0xf, false));
-    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xc, 0xf, false));
+    constexpr int lo = INT_MIN;
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, 0x111, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, 0x112, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, 0x114, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, 0x118, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, 0x142, 0xa, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, 0x143, 0xc, 0xf, false));
     return __builtin_amdgcn_readlane(v, 63);
 }
 
@@ -83,32 +93,35 @@ __device__ __forceinline__ void narrow_step(float (&x)[W], bool& active, int row
                                             int wave) {
     constexpr int b = J & 1;
     constexpr int NV = C0 + KB - J;  // block columns from J on
-    const float ax = fabsf(x[J]);
-    const int m = active ? (ax == ax ? __float_as_int(ax) : 0x7f800000) : __float_as_int(-1.f);
+    const float ax = fminf(fabsf(x[J]), __builtin_inff());  // minNum: a NaN becomes +inf
+    const int m = active ? __float_as_int(ax) : __float_as_int(-1.f);
     const int wmi = wave_max_dpp(m);
     const float wm = __int_as_float(wmi);
     const unsigned long long bal = __ballot(m == wmi);  // never empty: some lane holds the maximum
     const int wl = __ffsll((long long)bal) - 1;
     if (lane == wl) {
         L.mag[b][wave] = wm;
-        L.crow[b][wave] = row;
+        L.cand[b][wave].row = row;
 #pragma unroll
-        for (int c = 0; c < NV; ++c) L.cval[b][wave][c] = x[J + c];
+        for (int c = 0; c < NV; ++c) L.cand[b][wave].v[c] = x[J + c];
     }
     __syncthreads();
-    int best = 0;
-    float bm = L.mag[b][0];
+    // the first wave holding the largest magnitude (compared as ints, like the DPP maximum)
+    int mk[NW];
 #pragma unroll
-    for (int w = 1; w < NW; ++w) {
-        const float mw = L.mag[b][w];
-        best = mw > bm ? w : best;
-        bm = mw > bm ? mw : bm;
-    }
-    const int prow = L.crow[b][best];
-    const float piv = L.cval[b][best][0];
+    for (int w = 0; w < NW; ++w) mk[w] = __float_as_int(L.mag[b][w]);
+    int mx = mk[0];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) mx = max(mx, mk[w]);
+    int best = NW - 1;
+#pragma unroll
+    for (int w = NW - 2; w >= 0; --w) best = mk[w] == mx ? w : best;
+    const TsluCand& cb = L.cand[b][best];
+    const int prow = cb.row;
+    const float piv = cb.v[0];
     float u[NV];
 #pragma unroll
-    for (int c = 1; c < NV; ++c) u[c] = L.cval[b][best][c];
+    for (int c = 1; c < NV; ++c) u[c] = cb.v[c];
     const float r = __builtin_amdgcn_rcpf(piv);
     const bool is_piv = active && row == prow;
     const bool upd = active && !is_piv;

@@ -51,7 +51,7 @@ struct TsluCand {
 
 // LDS of one panel workgroup.
 struct TsluLds {
-    float mag[2][NW];         // per wave: its best candidate's |x| ..
+    int mag[2][NW];           // per wave: its best candidate's key ..
     TsluCand cand[2][NW];     // .. and that candidate
     __attribute__((aligned(16))) float ur[KB][W];  // the block's pivot rows right of the block (U12 after the solve)
     float l11[KB][KB];        // the block's multipliers among its pivot rows
@@ -81,41 +81,39 @@ __device__ __forceinline__ float lane_value(float v, int lane) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-// Step J of the elimination, inside the block of columns [C0, C0 + KB): every wave finds its
-// first maximum of |x[J]| over its active rows (DPP max, then the lowest lane holding it) and
-// that lane publishes it with its row's block values; after one barrier every thread takes the
-// first maximum over the waves (ties to the lower wave, i.e. the lower row), and the active rows
-// eliminate column J inside the block.  A NaN counts as the largest magnitude.  The step is
-// one dependency chain, so it is kept free of branches: the multiplier is x[J] times the
-// pivot's reciprocal (v_rcp_f32, 1 ulp) and rows that do not update select their old values.
+// Step J of the elimination, inside the block of columns [C0, C0 + KB).  Each candidate row's
+// key is |x[J]|'s bit pattern with its low 8 bits replaced by the row's wave and lane
+// (inverted, so that on equal magnitudes the lower wave and lane win): non-negative floats
+// order like their bit patterns, so one integer maximum over the wave (DPP) names the wave's
+// winner, and one over the four waves' winners names the step's pivot row.  The comparison
+// ignores the low 8 of 23 mantissa bits: the pivot is within 2^-15 of the largest magnitude
+// (the pivot value itself is exact).  A NaN counts as the largest magnitude; rows no longer
+// candidates have negative keys.  The winning lane of each wave publishes its row's block
+// values; after one barrier every thread reads the winning wave's record and the active rows
+// eliminate column J inside the block.  The step is one dependency chain, so it is kept free
+// of branches: the multiplier is x[J] times the pivot's reciprocal (v_rcp_f32, 1 ulp) and
+// rows that do not update select their old values.
+static_assert(NW <= 4, "two bits of the pivot key name the wave");
 template <int C0, int J>
 __device__ __forceinline__ void narrow_step(float (&x)[W], bool& active, int row, int& step, TsluLds& L, int lane,
                                             int wave) {
     constexpr int b = J & 1;
     constexpr int NV = C0 + KB - J;  // block columns from J on
+    const int tag = ((NW - 1 - wave) << 6) | (63 - lane);
     const float ax = fminf(fabsf(x[J]), __builtin_inff());  // minNum: a NaN becomes +inf
-    const int m = active ? __float_as_int(ax) : __float_as_int(-1.f);
-    const int wmi = wave_max_dpp(m);
-    const float wm = __int_as_float(wmi);
-    const unsigned long long bal = __ballot(m == wmi);  // never empty: some lane holds the maximum
-    const int wl = __ffsll((long long)bal) - 1;
-    if (lane == wl) {
-        L.mag[b][wave] = wm;
+    const int key = ((active ? __float_as_int(ax) : __float_as_int(-1.f)) & ~0xFF) | tag;
+    const int wmi = wave_max_dpp(key);
+    if (key == wmi) {  // exactly one lane: keys are distinct
+        L.mag[b][wave] = wmi;
         L.cand[b][wave].row = row;
 #pragma unroll
         for (int c = 0; c < NV; ++c) L.cand[b][wave].v[c] = x[J + c];
     }
     __syncthreads();
-    // the first wave holding the largest magnitude (compared as ints, like the DPP maximum)
-    int mk[NW];
+    int mx = L.mag[b][0];
 #pragma unroll
-    for (int w = 0; w < NW; ++w) mk[w] = __float_as_int(L.mag[b][w]);
-    int mx = mk[0];
-#pragma unroll
-    for (int w = 1; w < NW; ++w) mx = max(mx, mk[w]);
-    int best = NW - 1;
-#pragma unroll
-    for (int w = NW - 2; w >= 0; --w) best = mk[w] == mx ? w : best;
+    for (int w = 1; w < NW; ++w) mx = max(mx, L.mag[b][w]);
+    const int best = NW - 1 - ((mx >> 6) & 3);
     const TsluCand& cb = L.cand[b][best];
     const int prow = cb.row;
     const float piv = cb.v[0];

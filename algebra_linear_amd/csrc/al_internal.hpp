@@ -185,13 +185,14 @@ int panel_f32(hipStream_t s, Workspace* ws, float* P, size_t lda, size_t row0, s
 
 // Tournament-pivoted 64-column panel of an FP32 matrix F (rows J..J+m, columns J..J+64):
 // tslu_panel chooses the pivots (ipiv[J..J+64), global indices) and the LU of the pivot rows;
-// after the interchanges are applied to the panel's columns, tslu_finish writes L11 \ U11 and
-// solves the rows below.  scratch: tslu_scratch_bytes(m) bytes; counters: zero-filled words
-// (one per tree node, left at zero by every panel).
+// after the interchanges are applied to the panel's columns, tslu_finish writes L11 \ U11,
+// solves the rows below and, in the same launch, U12 := L11^{-1} A12 on columns [u_c0, u_c1) of
+// rows J..J+64 (none when u_c1 <= u_c0).  scratch: tslu_scratch_bytes(m) bytes; counters:
+// zero-filled words (one per tree node, left at zero by every panel).
 size_t tslu_scratch_bytes(size_t m);
 int tslu_panel(hipStream_t s, float* F, size_t ld, size_t J, size_t m, int* ipiv, int* info, void* scratch,
                unsigned* counters, int* pairs = nullptr);
-int tslu_finish(hipStream_t s, float* F, size_t ld, size_t J, size_t m, const void* scratch);
+int tslu_finish(hipStream_t s, float* F, size_t ld, size_t J, size_t m, const void* scratch, int u_c0, int u_c1);
 
 // Solve with a getrf factorization: B (n x nrhs) := A^{-1} B.  perm (nullable) is getrf's row
 // origin vector; without it the pivots are composed first.

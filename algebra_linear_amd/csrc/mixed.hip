@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "al_internal.hpp"
+#include "solve64.hpp"
 
 namespace al {
 namespace {
@@ -211,45 +212,11 @@ __global__ void copy2d_k(int m, int n, const double* __restrict__ S, size_t lds,
 // ---------------------------------------------------------------------------------------
 
 // X := L^{-1} X for X = F[J:J+64, c0:c1) and L the unit lower triangle of F[J:J+64, J:J+64]
-// (the block does not overlap X).  One wave solves TR_G columns at a time, lane i holding
-// row i of each: the lane keeps row i of L in registers (64 coalesced loads per wave, once),
-// and step j takes x_j of every column from lane j with v_readlane, so the 63 dependent
-// steps wait on neither memory nor LDS.  The FMAs are those of column-oriented forward
-// substitution, in the same order.  Waves stride over the column groups.
-constexpr int TR_G = 4;        // columns per wave and pass
+// (the block does not overlap X); the substitution itself is solve_l64_cols.
 constexpr int TR_MAX_WG = 512;  // two workgroups per CU; wider blocks loop
 __global__ __launch_bounds__(256) void trsm_ll64_k(float* __restrict__ F, size_t ld, int J, int c0, int c1,
                                                    const float* __restrict__ L) {
-    const int lane = threadIdx.x & 63;
-    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int nwaves = gridDim.x * 4;
-    const int ncols = c1 - c0;
-    float lrow[PB];
-#pragma unroll
-    for (int j = 0; j < PB; ++j) lrow[j] = L[(size_t)lane + (size_t)j * ld];  // L(lane, j)
-    for (int g = wave * TR_G; g < ncols; g += nwaves * TR_G) {
-        float x[TR_G];
-        float* col[TR_G];
-#pragma unroll
-        for (int q = 0; q < TR_G; ++q) {
-            const int c = c0 + min(g + q, ncols - 1);  // a short last group repeats its last column, unstored
-            col[q] = F + (size_t)J + (size_t)c * ld;
-            x[q] = col[q][lane];
-        }
-#pragma unroll
-        for (int j = 0; j < PB - 1; ++j) {
-            const bool below = lane > j;
-#pragma unroll
-            for (int q = 0; q < TR_G; ++q) {
-                const float xj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x[q]), j));
-                const float v = fmaf(-lrow[j], xj, x[q]);
-                x[q] = below ? v : x[q];
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < TR_G; ++q)
-            if (g + q < ncols) col[q][lane] = x[q];
-    }
+    solve_l64_cols<false>(F, ld, J, c0, c1, L, ld, blockIdx.x * 4 + (threadIdx.x >> 6), gridDim.x * 4);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -352,7 +319,7 @@ int make_arena(hipStream_t s, Workspace* ws, size_t n, size_t np, size_t nr, siz
 
 int trsm_ll64(hipStream_t s, Arena& a, int J, int c0, int c1) {
     if (c1 <= c0) return AL_OK;
-    const size_t groups = ceil_div((size_t)(c1 - c0), (size_t)TR_G);
+    const size_t groups = ceil_div((size_t)(c1 - c0), (size_t)S64_G);
     const size_t wgs = std::min<size_t>(ceil_div(groups, (size_t)4), (size_t)TR_MAX_WG);
     hipLaunchKernelGGL(trsm_ll64_k, dim3((unsigned)wgs), dim3(256), 0, s, a.F, a.np, J, c0, c1,
                        (const float*)(a.F + (size_t)J + (size_t)J * a.np));
@@ -371,9 +338,8 @@ int factor_panel(hipStream_t q, Workspace* ws, Arena& a, int K, int KE, int* pai
     for (int J = K; J < KE; J += PB) {
         AL_CHECK(tslu_panel(q, F, ld, J, np - J, a.ipiv, a.minfo, a.tslu, a.counters, pairs));
         AL_CHECK(swap_apply_f32(q, pairs, at(0, K), KE - K, nullptr, 0, ld, a.minfo));
-        AL_CHECK(tslu_finish(q, F, ld, J, np - J, a.tslu));
+        AL_CHECK(tslu_finish(q, F, ld, J, np - J, a.tslu, J + PB, KE));  // L21, and U12 on [J + 64, KE)
         if (J + PB < KE) {
-            AL_CHECK(trsm_ll64(q, a, J, J + PB, KE));
             AL_CHECK(gemm_f32(q, false, false, np - J - PB, KE - J - PB, PB, -1.f, at(J + PB, J), ld, at(J, J + PB), ld,
                               1.f, at(J + PB, J + PB), ld));
         }

@@ -20,6 +20,7 @@
 #include <utility>
 
 #include "al_internal.hpp"
+#include "solve64.hpp"
 
 namespace al {
 namespace {
@@ -352,15 +353,20 @@ __global__ __launch_bounds__(NT, 1) void tslu_k(const float* __restrict__ F, siz
 // every thread: each workgroup stages it in LDS once, and the 64-step chain reads it as
 // broadcast LDS loads (all lanes one address), which the compiler issues well ahead of the
 // FMAs that use them.  Each pivot is applied as a multiplication by its reciprocal
-// (v_rcp_f32, 1 ulp).
+// (v_rcp_f32, 1 ulp).  The last nu workgroups instead solve U12 := L11^{-1} A12 on columns
+// [u_c0, u_c1) of rows J..J+63, with L11 from D: nothing here depends on the other parts.
 __global__ __launch_bounds__(256) void tslu_l21_k(float* __restrict__ F, size_t ld, int J, int m,
-                                                  const float* __restrict__ D) {
+                                                  const float* __restrict__ D, int nl, int u_c0, int u_c1, int nu) {
     const int t = threadIdx.x;
     if (blockIdx.x == 0) {
         for (int idx = t; idx < W * W; idx += 256) {
             const int i = idx & 63, c = idx >> 6;
             F[(size_t)(J + i) + (size_t)(J + c) * ld] = D[i * W + c];
         }
+        return;
+    }
+    if ((int)blockIdx.x > nl) {
+        solve_l64_cols<true>(F, ld, J, u_c0, u_c1, D, W, ((int)blockIdx.x - nl - 1) * 4 + (t >> 6), nu * 4);
         return;
     }
     __shared__ __align__(16) float U[W * W];
@@ -443,13 +449,16 @@ int tslu_panel(hipStream_t s, float* F, size_t ld, size_t J, size_t m, int* ipiv
     return AL_OK;
 }
 
-int tslu_finish(hipStream_t s, float* F, size_t ld, size_t J, size_t m, const void* scratch) {
+int tslu_finish(hipStream_t s, float* F, size_t ld, size_t J, size_t m, const void* scratch, int u_c0, int u_c1) {
     const Tree tr = make_tree((int)m);
     const int nodes = tr.offset[tr.levels - 1] + 1;
     const float* D = (const float*)((const char*)scratch + (size_t)nodes * W * W * sizeof(float) +
                                     (size_t)nodes * W * sizeof(int));
-    const unsigned grid = 1u + (unsigned)((m - W + 255) / 256);
-    hipLaunchKernelGGL(tslu_l21_k, dim3(grid), dim3(256), 0, s, F, ld, (int)J, (int)m, D);
+    const int nl = (int)((m - W + 255) / 256);
+    const int groups = u_c1 > u_c0 ? (u_c1 - u_c0 + S64_G - 1) / S64_G : 0;
+    const int nu = (groups + 3) / 4;
+    hipLaunchKernelGGL(tslu_l21_k, dim3((unsigned)(1 + nl + nu)), dim3(256), 0, s, F, ld, (int)J, (int)m, D, nl, u_c0,
+                       u_c1, nu);
     AL_HIP(hipGetLastError());
     return AL_OK;
 }

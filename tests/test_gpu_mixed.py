@@ -175,6 +175,22 @@ def test_results_are_reproducible():
     assert s1 == s2
 
 
+def test_deep_tournament_tree_converges_and_is_reproducible():
+    # n = 4160: the first 64-column panel has 4160 rows, i.e. 17 leaves of 256 and four tree
+    # levels (17 -> 5 -> 2 -> 1), so nominees pass between workgroups (and XCDs) three times;
+    # a nominee read before it is visible would pick wrong pivot rows (no convergence) or
+    # change the result from run to run
+    n = 4160
+    a = uniform(n * n)
+    b = uniform(n, [7, 11, 13, 17])
+    x1, s1 = solve(a, b, n, 1, "fp32")
+    x2, s2 = solve(a, b, n, 1, "fp32")
+    assert s1["fell_back"] == 0 and s1["converged"] == 1 and s1["factor_breakdown"] == 0, s1
+    assert O.backward_error(a, x1, b, n) <= TOL
+    np.testing.assert_array_equal(x1, x2)
+    assert s1 == s2
+
+
 def test_device_api_on_a_torch_stream_with_padded_leading_dimensions():
     torch = pytest.importorskip("torch")
     import ctypes

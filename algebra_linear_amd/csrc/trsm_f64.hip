@@ -2,9 +2,12 @@
 //
 // blas::dtrsm (src/blas.rs:1384-1438) solves one vector at a time with a scalar
 // substitution.  Here a triangular solve is blocked in 64-wide diagonal blocks: each
-// diagonal block is solved by `trsm_diag` (substitution, one thread per right-hand-side
-// vector), and the off-diagonal coupling is an FP64 MFMA GEMM, so almost all flops of a
-// large solve run on the matrix cores.
+// diagonal block is solved by substitution (`trsm_fwd_wave` for lower triangles, one wave per
+// four right-hand-side vectors; `trsm_diag` for upper ones, one thread per vector), and the
+// off-diagonal coupling is an FP64 MFMA GEMM, so almost all flops of a large solve run on the
+// matrix cores.
+#include <algorithm>
+
 #include "al_internal.hpp"
 
 namespace al {
@@ -64,6 +67,63 @@ __global__ __launch_bounds__(256) void trsm_diag(int nb, int nvec, const double*
         if (i < nb) xp[(size_t)i * x_es] = x[i];
 }
 
+__device__ __forceinline__ double lane_d(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | lo);
+}
+
+// The forward case of trsm_diag (M lower triangular) with the same arithmetic, organized for
+// latency: one wave solves four vectors at a time, lane i holding element i of each.  Lane i
+// keeps row i of M left of the diagonal in registers (loaded once per wave; nothing on or above
+// the diagonal is read, except the diagonal itself when not unit); step j finishes x_j in lane j
+// (the division by M[j][j] of the fully accumulated value, as the row sweep does) and hands it
+// to every lane with v_readlane, and the lanes below subtract M[i][j] x_j.  Each x_i thus sees
+// exactly the row sweep's FMAs, in the same order (j ascending), with no LDS on the chain.
+// Waves stride over the groups of four vectors.
+template <bool UNIT>
+__global__ __launch_bounds__(256) void trsm_fwd_wave(int nb, int nvec, const double* __restrict__ T, size_t t_rs,
+                                                     size_t t_cs, double* __restrict__ X, size_t x_es, size_t x_vs,
+                                                     const int* __restrict__ info) {
+    if (info && *info != 0) return;
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6), nwaves = gridDim.x * 4;
+    const bool in = lane < nb;
+    double trow[DB];
+#pragma unroll
+    for (int j = 0; j < DB; ++j) trow[j] = (in && j < lane) ? T[(size_t)lane * t_rs + (size_t)j * t_cs] : 0.0;
+    const double dg = (!UNIT && in) ? T[(size_t)lane * (t_rs + t_cs)] : 1.0;
+    for (int g = wave * 4; g < nvec; g += nwaves * 4) {
+        double x[4];
+        double* xp[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            xp[q] = X + (size_t)min(g + q, nvec - 1) * x_vs;  // a short last group repeats, unstored
+            x[q] = in ? xp[q][(size_t)lane * x_es] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < DB; ++j) {
+            const double dj = UNIT ? 1.0 : lane_d(dg, j);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                double xj = lane_d(x[q], j);
+                if (!UNIT) {
+                    xj = xj / dj;
+                    x[q] = lane == j ? xj : x[q];
+                }
+                const double v = fma(-trow[j], xj, x[q]);
+                x[q] = lane > j ? v : x[q];
+            }
+        }
+        if (in) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (g + q < nvec) xp[q][(size_t)lane * x_es] = x[q];
+        }
+    }
+}
+
 __global__ void scale_block(int m, int n, double alpha, double* B, size_t ldb, const int* info) {
     if (info && *info != 0) return;
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -96,6 +156,17 @@ inline const double* opT_block(const double* T, size_t ldt, bool trans, size_t r
 int diag_solve(hipStream_t s, int nb, int nvec, const double* T, size_t t_rs, size_t t_cs, double* X, size_t x_es,
                size_t x_vs, bool forward, bool unit, const int* info) {
     if (nvec <= 0 || nb <= 0) return AL_OK;
+    if (forward) {
+        const unsigned wgs = (unsigned)std::min<size_t>(ceil_div((size_t)nvec, (size_t)16), (size_t)512);
+        if (unit)
+            hipLaunchKernelGGL(trsm_fwd_wave<true>, dim3(wgs), dim3(256), 0, s, nb, nvec, T, t_rs, t_cs, X, x_es, x_vs,
+                               info);
+        else
+            hipLaunchKernelGGL(trsm_fwd_wave<false>, dim3(wgs), dim3(256), 0, s, nb, nvec, T, t_rs, t_cs, X, x_es,
+                               x_vs, info);
+        AL_HIP(hipGetLastError());
+        return AL_OK;
+    }
     hipLaunchKernelGGL(trsm_diag, dim3(ceil_div((size_t)nvec, 256)), dim3(256), 0, s, nb, nvec, T, t_rs, t_cs, X,
                        x_es, x_vs, forward ? 1 : 0, unit ? 1 : 0, info);
     AL_HIP(hipGetLastError());

@@ -139,10 +139,13 @@ __device__ __forceinline__ void narrow_step(float (&x)[W], bool& active, int row
     }
 }
 
-// After the KB steps of block [C0, C0 + KB): the pivot rows' entries right of the block become
-// U12 = L11^{-1} A12 (one thread per column), every other active row subtracts its 8
-// multipliers times U12, and the pivot rows take their U12 rows.
-// No barrier is needed before the pivot rows write ur and l11: every thread last read ur in
+// After the KB steps of block [C0, C0 + KB): the rows right of the block take the block's
+// rank-8 update.  The pivot rows publish their entries right of the block (A12) and their
+// multipliers among each other (L11); after one barrier every candidate row turns its 8
+// multipliers l into w = l L11^{-1} (28 FMAs against broadcast LDS values) and subtracts w A12,
+// which is l U12 with U12 = L11^{-1} A12 never formed.  A pivot row k takes w = e_k - e_k L11^{-1},
+// which turns its own A12 row into row k of U12.
+// No barrier is needed before the pivot rows write ur and l11: every thread last read them in
 // the previous block's update, and each narrow step since then ends at a barrier.
 template <int C0>
 __device__ __forceinline__ void block_update(float (&x)[W], bool active, int step, TsluLds& L) {
@@ -157,30 +160,24 @@ __device__ __forceinline__ void block_update(float (&x)[W], bool active, int ste
         for (int i = 0; i < KB; ++i) L.l11[k][i] = i < k ? x[C0 + i] : 0.f;
     }
     __syncthreads();
-    if ((int)threadIdx.x < REST) {
-        const int c = threadIdx.x;
-        float u[KB];
+    if (piv_here || active) {
+        // v L11 = a by back substitution, a = e_k for a pivot row, its multipliers otherwise
+        float v[KB];
 #pragma unroll
-        for (int kk = 0; kk < KB; ++kk) {
-            float v = L.ur[kk][c];
+        for (int i = KB - 1; i >= 0; --i) {
+            float t = piv_here ? (i == k ? 1.f : 0.f) : x[C0 + i];
 #pragma unroll
-            for (int i = 0; i < kk; ++i) v = fmaf(-L.l11[kk][i], u[i], v);
-            u[kk] = v;
+            for (int j = i + 1; j < KB; ++j) t = fmaf(-v[j], L.l11[j][i], t);
+            v[i] = t;
         }
 #pragma unroll
-        for (int kk = 0; kk < KB; ++kk) L.ur[kk][c] = u[kk];
-    }
-    __syncthreads();
-    if (piv_here) {
-#pragma unroll
-        for (int c = 0; c < REST; ++c) x[C0 + KB + c] = L.ur[k][c];
-    } else if (active) {
+        for (int i = 0; i < KB; ++i) v[i] = piv_here ? (i == k ? 1.f : 0.f) - v[i] : v[i];
 #pragma unroll
         for (int c = 0; c < REST; ++c) {
-            float v = x[C0 + KB + c];
+            float t = x[C0 + KB + c];
 #pragma unroll
-            for (int kk = 0; kk < KB; ++kk) v = fmaf(-x[C0 + kk], L.ur[kk][c], v);
-            x[C0 + KB + c] = v;
+            for (int i = 0; i < KB; ++i) t = fmaf(-v[i], L.ur[i][c], t);
+            x[C0 + KB + c] = t;
         }
     }
 }

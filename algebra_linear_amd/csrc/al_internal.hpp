@@ -167,6 +167,10 @@ int swap_block_f32(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, siz
 // columns [0, n0) of A0 and [0, n1) of A1; nothing when *info != 0.
 int swap_apply_f32(hipStream_t s, const int* pairs, float* A0, size_t n0, float* A1, size_t n1, size_t lda,
                    const int* info);
+// Applies nl pivot blocks' pair lists (each from one 64-pivot tournament, at pairs + l * stride)
+// in order to columns [0, n0) of A0 and [0, n1) of A1.
+int swap_apply_multi_f32(hipStream_t s, const int* pairs, size_t stride, int nl, float* A0, size_t n0, float* A1,
+                         size_t n1, size_t lda, const int* info);
 int perm_init(hipStream_t s, double* perm, size_t n);                          // perm[i] = i
 int perm_from_ipiv(hipStream_t s, const int* ipiv, size_t n, double* perm);    // compose all pivots
 int gather_rows(hipStream_t s, const double* B, size_t ldb, size_t n, size_t nrhs, const double* perm, double* W);

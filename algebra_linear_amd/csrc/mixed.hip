@@ -33,6 +33,8 @@ namespace al {
 namespace {
 
 constexpr int PB = 64;          // panel width and padding granule
+constexpr int PAIR_LISTS = 8;     // 64-pivot blocks per outer panel, at most (bf16: 512 columns)
+constexpr int PAIR_STRIDE = 320;  // ints per pair list: a count and up to 128 (destination, source) pairs
 constexpr int GV_ROWS = 256;    // residual: rows per workgroup
 constexpr int GV_COLS = 256;    // residual: columns per partial sum
 constexpr int NR_MAX = 16;      // right-hand sides per refinement pass
@@ -237,8 +239,8 @@ struct Arena {
     __bf16* Bt = nullptr;
     void* tslu = nullptr;     // tournament panel scratch (nominees, L11 \ U11)
     unsigned* counters = nullptr;  // its arrival counters (zero between panels)
-    int* pairs_side = nullptr;     // row-interchange pairs of the panel stream's swaps
-    int* pairs_outer[2] = {nullptr, nullptr};  // an outer panel's pairs, alternating by panel
+    int* pairs_sub = nullptr;      // [2][PAIR_LISTS] row-interchange pair lists, one per 64-pivot
+                                   // block of an outer panel, alternating by outer panel
     double* dinv = nullptr;   // FP64 inverses of the factor's 128-row diagonal blocks: L's, then U's
 };
 
@@ -292,16 +294,14 @@ int make_arena(hipStream_t s, Workspace* ws, size_t n, size_t np, size_t nr, siz
     const size_t oT = off;
     if (lowp) off += align256(tslu_scratch_bytes(np));
     const size_t oS = off;
-    if (lowp) off += 3 * align256((1 + 2 * 512) * sizeof(int));
+    if (lowp) off += align256(2 * PAIR_LISTS * PAIR_STRIDE * sizeof(int));
     const size_t oD = off;
     if (refine) off += 2 * align256(trsv_diag_inverse_len(np) * sizeof(double));
     AL_CHECK(grow(&ws->mbuf, &ws->mbuf_bytes, off));
     char* base = (char*)ws->mbuf;
     AL_CHECK(fill_u32(s, (unsigned*)base, 1024, 0u));
     a->tslu = base + oT;
-    a->pairs_side = lowp ? (int*)(base + oS) : nullptr;
-    a->pairs_outer[0] = lowp ? (int*)(base + oS + align256((1 + 2 * 512) * sizeof(int))) : nullptr;
-    a->pairs_outer[1] = lowp ? (int*)(base + oS + 2 * align256((1 + 2 * 512) * sizeof(int))) : nullptr;
+    a->pairs_sub = lowp ? (int*)(base + oS) : nullptr;
     a->dinv = refine ? (double*)(base + oD) : nullptr;
     a->counters = (unsigned*)base;
     a->F = (float*)(base + oF);
@@ -330,12 +330,13 @@ int trsm_ll64(hipStream_t s, Arena& a, int J, int c0, int c1) {
 // One outer panel of the FP32 LU, columns [K, KE) and rows K..np, 64 columns at a time:
 // tournament pivots, the interchanges applied to the outer panel's columns, L21 of the 64
 // columns, then the rest of the outer panel updated (64-row U solve and a k = 64 product).
-int factor_panel(hipStream_t q, Workspace* ws, Arena& a, int K, int KE, int* pairs) {
+int factor_panel(hipStream_t q, Workspace* ws, Arena& a, int K, int KE, int* lists) {
     const int np = (int)a.np;
     const size_t ld = a.np;
     float* F = a.F;
     auto at = [&](int i, int j) { return F + (size_t)i + (size_t)j * ld; };
     for (int J = K; J < KE; J += PB) {
+        int* pairs = lists + (size_t)((J - K) / PB) * PAIR_STRIDE;  // this block's interchanges
         AL_CHECK(tslu_panel(q, F, ld, J, np - J, a.ipiv, a.minfo, a.tslu, a.counters, pairs));
         AL_CHECK(swap_apply_f32(q, pairs, at(0, K), KE - K, nullptr, 0, ld, a.minfo));
         AL_CHECK(tslu_finish(q, F, ld, J, np - J, a.tslu, J + PB, KE));  // L21, and U12 on [J + 64, KE)
@@ -397,15 +398,12 @@ int getrf_lowp(hipStream_t s, Workspace* ws, Arena& a, bool bf16) {
     AL_CHECK(iota_i32(s, a.ipiv, a.np));
     AL_HIP(hipEventRecord(ws->ev_main, s));  // also "the previous update is done" for the first panel
     AL_HIP(hipStreamWaitEvent(side, ws->ev_main, 0));
-    // the outer panel [K, KE)'s interchanges on columns [c0, c0 + n0) and [c2, c2 + n1); with
-    // `pairs` (one pivot block) the composed pairs are kept there
-    auto swaps = [&](hipStream_t q, int K, int KE, int c0, int n0, int c2, int n1, int* pairs) -> int {
-        for (int J = K; J < KE; J += 256) {
-            const int jb = (KE - J) < 256 ? (KE - J) : 256;
-            AL_CHECK(swap_block_f32(q, ws, a.ipiv, J, jb, at(0, c0), n0, n1 > 0 ? at(0, c2) : nullptr, n1, ld, info,
-                                    pairs));
-        }
-        return AL_OK;
+    // the outer panel [K, KE)'s interchanges (its 64-pivot blocks' pair lists, in order) on
+    // columns [c0, c0 + n0) and [c2, c2 + n1)
+    auto lists = [&](int K) { return a.pairs_sub + (size_t)((K / nbo) & 1) * PAIR_LISTS * PAIR_STRIDE; };
+    auto swaps = [&](hipStream_t q, int K, int KE, int c0, int n0, int c2, int n1) -> int {
+        return swap_apply_multi_f32(q, lists(K), PAIR_STRIDE, (KE - K) / PB, at(0, c0), n0,
+                                    n1 > 0 ? at(0, c2) : nullptr, n1, ld, info);
     };
     // U12 := L11^{-1} A12 on columns [c0, c1), 64 rows at a time
     auto u12 = [&](hipStream_t q, int K, int KE, int c0, int c1) -> int {
@@ -418,31 +416,27 @@ int getrf_lowp(hipStream_t s, Workspace* ws, Arena& a, bool bf16) {
         }
         return AL_OK;
     };
-    AL_CHECK(factor_panel(side, ws, a, 0, np < nbo ? np : nbo, a.pairs_side));
+    AL_CHECK(factor_panel(side, ws, a, 0, np < nbo ? np : nbo, lists(0)));
     for (int K = 0; K < np; K += nbo) {
         const int kb = (np - K) < nbo ? (np - K) : nbo;
         const int KE = K + kb;
         if (KE >= np) {  // the last panel: only the columns left of it remain
             AL_HIP(hipEventRecord(ws->ev_side, side));
             AL_HIP(hipStreamWaitEvent(s, ws->ev_side, 0));
-            AL_CHECK(swaps(s, K, KE, 0, K, 0, 0, nullptr));
+            AL_CHECK(swaps(s, K, KE, 0, K, 0, 0));
             break;
         }
         const int KE2 = (np - KE) < nbo ? np : KE + nbo;
-        int* pairs = kb <= 256 ? a.pairs_outer[(K / nbo) & 1] : nullptr;
         // side: the next panel's columns, after the main stream's last update of them
         AL_HIP(hipStreamWaitEvent(side, ws->ev_main, 0));
-        AL_CHECK(swaps(side, K, KE, KE, KE2 - KE, 0, 0, pairs));
+        AL_CHECK(swaps(side, K, KE, KE, KE2 - KE, 0, 0));
         AL_CHECK(u12(side, K, KE, KE, KE2));
         AL_CHECK(trailing_update(side, a, bf16, K, KE, KE, KE2));
         AL_HIP(hipEventRecord(ws->ev_side, side));
-        AL_CHECK(factor_panel(side, ws, a, KE, KE2, a.pairs_side));
+        AL_CHECK(factor_panel(side, ws, a, KE, KE2, lists(KE)));  // the other list set
         // main: every other column
         AL_HIP(hipStreamWaitEvent(s, ws->ev_side, 0));
-        if (pairs)
-            AL_CHECK(swap_apply_f32(s, pairs, at(0, 0), K, at(0, KE2), np - KE2, ld, info));
-        else
-            AL_CHECK(swaps(s, K, KE, 0, K, KE2, np - KE2, nullptr));
+        AL_CHECK(swaps(s, K, KE, 0, K, KE2, np - KE2));
         AL_CHECK(u12(s, K, KE, KE2, np));
         AL_CHECK(trailing_update(s, a, bf16, K, KE, KE2, np));
         AL_HIP(hipEventRecord(ws->ev_main, s));

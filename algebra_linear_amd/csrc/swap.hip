@@ -117,6 +117,51 @@ __global__ __launch_bounds__(256) void apply_pairs(T* __restrict__ A0, int n0, T
     }
 }
 
+// Several pivot blocks' pair lists applied one after the other, as their swap sequences are:
+// lists l = 0..nl-1 at pairs + l * stride, each in piv_pairs' format with at most
+// SUB_PAIRS_MAX pairs (one 64-pivot block).  One wave per column, so the lists need no
+// synchronization between them: a wave finishes a list's reads before its writes, and a list's
+// writes before the next list's reads.
+constexpr int SUB_PAIRS_MAX = 128;
+constexpr int MULTI_MAX = 8;
+__global__ __launch_bounds__(256) void apply_pairs_multi(float* __restrict__ A0, int n0, float* __restrict__ A1,
+                                                         int n1, size_t lda, const int* __restrict__ pairs,
+                                                         size_t stride, int nl, const int* __restrict__ info) {
+    __shared__ int pr[MULTI_MAX][1 + 2 * SUB_PAIRS_MAX];
+    if (info && *info != 0) return;
+    const int t = threadIdx.x;
+    for (int l = 0; l < nl; ++l) {
+        const int* p = pairs + (size_t)l * stride;
+        const int cnt = min(p[0], SUB_PAIRS_MAX);
+        if (t == 0) pr[l][0] = cnt;
+        for (int i = t; i < 2 * cnt; i += 256) pr[l][1 + i] = p[1 + i];
+    }
+    __syncthreads();
+    const int lane = t & 63;
+    const int c = blockIdx.x * 4 + (t >> 6);
+    float* col;
+    if (c < n0)
+        col = A0 + (size_t)c * lda;
+    else if (c < n0 + n1)
+        col = A1 + (size_t)(c - n0) * lda;
+    else
+        return;
+    for (int l = 0; l < nl; ++l) {
+        const int cnt = pr[l][0];
+        float v[SUB_PAIRS_MAX / 64];
+#pragma unroll
+        for (int q = 0; q < SUB_PAIRS_MAX / 64; ++q) {
+            const int i = lane + 64 * q;
+            v[q] = i < cnt ? col[pr[l][2 + 2 * i]] : 0.f;
+        }
+#pragma unroll
+        for (int q = 0; q < SUB_PAIRS_MAX / 64; ++q) {
+            const int i = lane + 64 * q;
+            if (i < cnt) col[pr[l][1 + 2 * i]] = v[q];
+        }
+    }
+}
+
 __global__ void perm_iota(double* perm, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) perm[i] = (double)i;
@@ -193,6 +238,19 @@ int swap_apply_f32(hipStream_t s, const int* pairs, float* A0, size_t n0, float*
     if (n0 + n1 == 0) return AL_OK;
     hipLaunchKernelGGL(apply_pairs<float>, dim3(ceil_div(n0 + n1, 4)), dim3(256), 0, s, A0, (int)n0, A1, (int)n1, lda,
                        (double*)nullptr, pairs, info);
+    AL_HIP(hipGetLastError());
+    return AL_OK;
+}
+
+int swap_apply_multi_f32(hipStream_t s, const int* pairs, size_t stride, int nl, float* A0, size_t n0, float* A1,
+                         size_t n1, size_t lda, const int* info) {
+    if (n0 + n1 == 0 || nl == 0) return AL_OK;
+    if (nl > MULTI_MAX) {
+        set_error("swap_apply_multi: more than 8 pivot blocks");
+        return AL_ERR_ARG;
+    }
+    hipLaunchKernelGGL(apply_pairs_multi, dim3(ceil_div(n0 + n1, 4)), dim3(256), 0, s, A0, (int)n0, A1, (int)n1, lda,
+                       pairs, stride, nl, info);
     AL_HIP(hipGetLastError());
     return AL_OK;
 }

@@ -27,7 +27,6 @@
 #include <vector>
 
 #include "al_internal.hpp"
-#include "solve64.hpp"
 
 namespace al {
 namespace {
@@ -213,12 +212,75 @@ __global__ void copy2d_k(int m, int n, const double* __restrict__ S, size_t lds,
 // LU with partial pivoting, FP32
 // ---------------------------------------------------------------------------------------
 
-// X := L^{-1} X for X = F[J:J+64, c0:c1) and L the unit lower triangle of F[J:J+64, J:J+64]
-// (the block does not overlap X); the substitution itself is solve_l64_cols.
-constexpr int TR_MAX_WG = 512;  // two workgroups per CU; wider blocks loop
-__global__ __launch_bounds__(256) void trsm_ll64_k(float* __restrict__ F, size_t ld, int J, int c0, int c1,
-                                                   const float* __restrict__ L) {
-    solve_l64_cols<false>(F, ld, J, c0, c1, L, ld, blockIdx.x * 4 + (threadIdx.x >> 6), gridDim.x * 4);
+// Pieces of u12_k on four columns at once (their element `lane` of one 64-row block in x):
+// x -= L_ba x_a with L_ba's row `lane` at Lp (stride ld between its columns), and the forward
+// solve with a unit lower diagonal block.  Kept out of line so that each holds one 64-register
+// tile of L at a time.
+__device__ __attribute__((noinline)) float4 u12_off(const float* __restrict__ Lp, size_t ld, float4 x, float4 xa) {
+    float lr[64];
+#pragma unroll
+    for (int j = 0; j < 64; ++j) lr[j] = Lp[(size_t)j * ld];
+    float xv[4] = {x.x, x.y, x.z, x.w};
+    const float av[4] = {xa.x, xa.y, xa.z, xa.w};
+#pragma unroll
+    for (int j = 0; j < 64; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            xv[q] = fmaf(-lr[j], __int_as_float(__builtin_amdgcn_readlane(__float_as_int(av[q]), j)), xv[q]);
+    return float4{xv[0], xv[1], xv[2], xv[3]};
+}
+
+__device__ __attribute__((noinline)) float4 u12_diag(const float* __restrict__ Lp, size_t ld, float4 x, int lane) {
+    float lr[64];
+#pragma unroll
+    for (int j = 0; j < 64; ++j) lr[j] = Lp[(size_t)j * ld];
+    float xv[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+    for (int j = 0; j < 63; ++j) {
+        const bool below = lane > j;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float xj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xv[q]), j));
+            const float v = fmaf(-lr[j], xj, xv[q]);
+            xv[q] = below ? v : xv[q];
+        }
+    }
+    return float4{xv[0], xv[1], xv[2], xv[3]};
+}
+
+// U12 := L11^{-1} A12 for the outer panel's NB * 64 rows at K and columns [c0, c1), L11 the
+// unit lower triangle of F[K:K+64NB, K:K+64NB], in one launch.  One wave solves four columns at
+// a time, block b's element i of each in lane i: block b first subtracts L(b, a) x_a for every
+// a < b (the 64 x 64 block's row i streamed into lane i's registers, x_a's elements handed out
+// by v_readlane), then solves with its diagonal block as solve_l64_cols does.
+template <int NB>
+__global__ __launch_bounds__(256) void u12_k(float* __restrict__ F, size_t ld, int K, int c0, int c1) {
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6), nwaves = gridDim.x * 4;
+    const int ncols = c1 - c0;
+    const float* Lrow = F + (size_t)(K + lane) + (size_t)K * ld;  // L11(lane, 0)
+    for (int g = wave * 4; g < ncols; g += nwaves * 4) {
+        float* col[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) col[q] = F + (size_t)K + (size_t)(c0 + min(g + q, ncols - 1)) * ld;  // a short last group repeats
+        float4 x[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+            x[b] = float4{col[0][64 * b + lane], col[1][64 * b + lane], col[2][64 * b + lane], col[3][64 * b + lane]};
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+#pragma unroll
+            for (int a = 0; a < b; ++a) x[b] = u12_off(Lrow + 64 * b + (size_t)(64 * a) * ld, ld, x[b], x[a]);
+            x[b] = u12_diag(Lrow + 64 * b + (size_t)(64 * b) * ld, ld, x[b], lane);
+        }
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const float xv[4] = {x[b].x, x[b].y, x[b].z, x[b].w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (g + q < ncols) col[q][64 * b + lane] = xv[q];
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -317,15 +379,6 @@ int make_arena(hipStream_t s, Workspace* ws, size_t n, size_t np, size_t nr, siz
     return AL_OK;
 }
 
-int trsm_ll64(hipStream_t s, Arena& a, int J, int c0, int c1) {
-    if (c1 <= c0) return AL_OK;
-    const size_t groups = ceil_div((size_t)(c1 - c0), (size_t)S64_G);
-    const size_t wgs = std::min<size_t>(ceil_div(groups, (size_t)4), (size_t)TR_MAX_WG);
-    hipLaunchKernelGGL(trsm_ll64_k, dim3((unsigned)wgs), dim3(256), 0, s, a.F, a.np, J, c0, c1,
-                       (const float*)(a.F + (size_t)J + (size_t)J * a.np));
-    AL_HIP(hipGetLastError());
-    return AL_OK;
-}
 
 // One outer panel of the FP32 LU, columns [K, KE) and rows K..np, 64 columns at a time:
 // tournament pivots, the interchanges applied to the outer panel's columns, L21 of the 64
@@ -405,15 +458,15 @@ int getrf_lowp(hipStream_t s, Workspace* ws, Arena& a, bool bf16) {
         return swap_apply_multi_f32(q, lists(K), PAIR_STRIDE, (KE - K) / PB, at(0, c0), n0,
                                     n1 > 0 ? at(0, c2) : nullptr, n1, ld, info);
     };
-    // U12 := L11^{-1} A12 on columns [c0, c1), 64 rows at a time
-    auto u12 = [&](hipStream_t q, int K, int KE, int c0, int c1) -> int {
+    // U12 := L11^{-1} A12 on columns [c0, c1) of a full outer panel [K, K + nbo), one launch
+    auto u12 = [&](hipStream_t q, int K, int c0, int c1) -> int {
         if (c1 <= c0) return AL_OK;
-        for (int J = K; J < KE; J += PB) {
-            AL_CHECK(trsm_ll64(q, a, J, c0, c1));
-            if (J + PB < KE)
-                AL_CHECK(gemm_f32(q, false, false, KE - J - PB, c1 - c0, PB, -1.f, at(J + PB, J), ld, at(J, c0), ld, 1.f,
-                                  at(J + PB, c0), ld));
-        }
+        const unsigned wgs = (unsigned)std::min<size_t>(ceil_div((size_t)(c1 - c0), (size_t)16), (size_t)512);
+        if (bf16)
+            hipLaunchKernelGGL(u12_k<8>, dim3(wgs), dim3(256), 0, q, F, ld, K, c0, c1);
+        else
+            hipLaunchKernelGGL(u12_k<4>, dim3(wgs), dim3(256), 0, q, F, ld, K, c0, c1);
+        AL_HIP(hipGetLastError());
         return AL_OK;
     };
     AL_CHECK(factor_panel(side, ws, a, 0, np < nbo ? np : nbo, lists(0)));
@@ -430,14 +483,14 @@ int getrf_lowp(hipStream_t s, Workspace* ws, Arena& a, bool bf16) {
         // side: the next panel's columns, after the main stream's last update of them
         AL_HIP(hipStreamWaitEvent(side, ws->ev_main, 0));
         AL_CHECK(swaps(side, K, KE, KE, KE2 - KE, 0, 0));
-        AL_CHECK(u12(side, K, KE, KE, KE2));
+        AL_CHECK(u12(side, K, KE, KE2));  // kb == nbo: only the last panel is shorter
         AL_CHECK(trailing_update(side, a, bf16, K, KE, KE, KE2));
         AL_HIP(hipEventRecord(ws->ev_side, side));
         AL_CHECK(factor_panel(side, ws, a, KE, KE2, lists(KE)));  // the other list set
         // main: every other column
         AL_HIP(hipStreamWaitEvent(s, ws->ev_side, 0));
         AL_CHECK(swaps(s, K, KE, 0, K, KE2, np - KE2));
-        AL_CHECK(u12(s, K, KE, KE2, np));
+        AL_CHECK(u12(s, K, KE2, np));
         AL_CHECK(trailing_update(s, a, bf16, K, KE, KE2, np));
         AL_HIP(hipEventRecord(ws->ev_main, s));
     }

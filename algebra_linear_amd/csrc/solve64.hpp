@@ -1,6 +1,7 @@
 // Forward substitution with the unit lower triangle of a 64 x 64 block, applied to columns of a
-// column-major matrix: X := L^{-1} X for X = F[J:J+64, c0:c1).  Shared by the FP32 panel
-// kernels (L read from the factor itself, or from the tournament's row-major copy of L11 \ U11).
+// column-major matrix: X := L^{-1} X for X = F[J:J+64, c0:c1).  Used by the FP32 panel kernel
+// that finishes a tournament (L from its row-major copy of L11 \ U11); L may also be read in
+// place from a column-major matrix.
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -59,11 +59,11 @@ sys.path.insert(0, ROOT)
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}   # MI355X dense matrix-core peaks
 CLASSES = ["sgemm_nn", "bgemm", "dgemm_nn", "tslu_tree", "panel_getf2", "trsv", "residual_gemv"]
 # HBM bytes per launch of the trailing-update kernel, from rocprofv3 FETCH_SIZE and WRITE_SIZE
-# over one FP32-form solve at n = 16384 (uniform matrix; the launch shapes do not depend on the
+# over FP32-form solves at n = 16384 (uniform matrix; the launch shapes do not depend on the
 # values), each counter scaled by what it reads for a copy kernel of known size (FETCH x 2.0,
 # WRITE x 1.0); the command and the per-kernel table are in
 # profiles/sgemm_nn_hbm_traffic_n16384.md.  Reported only for the workload it was measured on.
-MEASURED_TRAFFIC = {("fp32", 16384): {"bytes_per_launch": 87.497e6, "fetch_bytes": 52.849e6, "write_bytes": 34.648e6,
+MEASURED_TRAFFIC = {("fp32", 16384): {"bytes_per_launch": 184.231e6, "fetch_bytes": 111.099e6, "write_bytes": 73.132e6,
                                       "source": "profiles/sgemm_nn_hbm_traffic_n16384.md"}}
 
 

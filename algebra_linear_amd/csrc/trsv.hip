@@ -13,6 +13,7 @@
 //
 // The matrix may be FP64 or FP32 (the low-precision factors of the mixed-precision solver);
 // right-hand sides and all accumulation are FP64.
+#include <algorithm>
 #include <type_traits>
 #include <utility>
 
@@ -207,7 +208,13 @@ __global__ __launch_bounds__(256) void trsv128_f32(int n, int nrhs, const float*
     double* Di = ys + TB2 * NR;                           // [TB2][DINV_ROW], row-major (inverse)
     int* ok_lds = INV ? (int*)(Di + TB2 * DINV_ROW) : (int*)(D + TB2 * (TB2 + 1));
     const int nblk = (n + TB2 - 1) / TB2;
-    const int b = LOWER ? (int)blockIdx.x : nblk - 1 - (int)blockIdx.x;
+    // a launch may carry several independent chains, one per group of NR right-hand sides:
+    // workgroup ids interleave them block by block, so a block still waits only on lower ids
+    const int nch = (int)gridDim.x / nblk, chunk = (int)blockIdx.x % nch, bidx = (int)blockIdx.x / nch;
+    X += (size_t)chunk * NR * ldx;
+    nrhs = min(NR, nrhs - chunk * NR);
+    flags += (size_t)chunk * nblk * 16;
+    const int b = LOWER ? bidx : nblk - 1 - bidx;
     const int r0 = b * TB2;
     const int nb = (n - r0) < TB2 ? (n - r0) : TB2;
     const int t = threadIdx.x, lane = t & 63, part = t >> 6;
@@ -567,9 +574,22 @@ __global__ __launch_bounds__(256) void diag_inv_k(int n, const float* __restrict
     for (int i = 0; i < 64; ++i) out[i] = z[i];
 }
 
+int cu_count() {
+    static int cus = 0;  // benign race: every thread computes the same value
+    if (cus == 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
+            v = 256;
+        cus = v;
+    }
+    return cus;
+}
+
 template <bool LOWER, bool UNIT, int NR, bool INV>
 int launch128_t(hipStream_t s, unsigned nblk, int n, int nrhs, const float* A, size_t lda, double* X, size_t ldx,
-                unsigned* flags, unsigned gen, unsigned* abort_flag, long long timeout, const double* Dinv) {
+                unsigned* flags, unsigned gen, unsigned* abort_flag, long long timeout, const double* Dinv,
+                unsigned nch = 1) {
     static bool attr_set = false;  // benign race: every thread sets the same value
     auto fn = trsv128_f32<LOWER, UNIT, NR, INV>;
     constexpr size_t bytes = trsv128_lds_bytes<INV, NR>();
@@ -577,8 +597,8 @@ int launch128_t(hipStream_t s, unsigned nblk, int n, int nrhs, const float* A, s
         AL_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
         attr_set = true;
     }
-    hipLaunchKernelGGL(fn, dim3(nblk), dim3(256), bytes, s, n, nrhs, A, lda, X, ldx, flags, gen, abort_flag, timeout,
-                       Dinv);
+    hipLaunchKernelGGL(fn, dim3(nblk * nch), dim3(256), bytes, s, n, nrhs, A, lda, X, ldx, flags, gen, abort_flag,
+                       timeout, Dinv);
     return AL_OK;
 }
 
@@ -601,20 +621,26 @@ void launch_nr(hipStream_t s, unsigned nblk, int n, int nrhs, const TA* A, size_
                        gen, abort_flag, timeout);
 }
 
+// nch > 1 (FP32 with Dinv only): nch chains of NR_INV_MAX right-hand sides in one launch.
 template <typename TA, bool LOWER, bool UNIT>
 int launch_trsv(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const TA* A, size_t lda, double* X, size_t ldx,
-                const double* Dinv) {
+                const double* Dinv, unsigned nch = 1) {
     const size_t nblk = (n + TB - 1) / TB;
-    AL_CHECK(ws_reserve_flags(ws, nblk * 16));
+    AL_CHECK(ws_reserve_flags(ws, nblk * 16 * nch));
     if (ws->trsv_gen > 0xF0000000u) {  // keep generations unique: restart from a cleared array
-        AL_CHECK(fill_u32(s, ws->trsv_flags, nblk * 16, 0u));
+        AL_CHECK(fill_u32(s, ws->trsv_flags, ws->trsv_flags_len, 0u));
         ws->trsv_gen = 0;
     }
     const unsigned gen = ++ws->trsv_gen;
     const long long timeout = 200000000LL;  // 2 s at 100 MHz
     const int ticket = prof_begin(s);
     const unsigned g = (unsigned)nblk;
-    if (std::is_same<TA, float>::value && nrhs <= (size_t)NR2_MAX) {
+    if (nch > 1) {
+        const unsigned g2 = (unsigned)((n + TB2 - 1) / TB2);
+        AL_CHECK((launch128_t<LOWER, UNIT, NR_INV_MAX, true>(s, g2, (int)n, (int)nrhs, (const float*)A, lda, X, ldx,
+                                                             ws->trsv_flags, gen, ws->abort_flag, timeout, Dinv,
+                                                             nch)));
+    } else if (std::is_same<TA, float>::value && nrhs <= (size_t)NR2_MAX) {
         const unsigned g2 = (unsigned)((n + TB2 - 1) / TB2);
         const float* Af = (const float*)A;
         if (nrhs <= 1)
@@ -660,11 +686,24 @@ int trsv_solve(hipStream_t s, Workspace* ws, bool lower, bool unit, size_t n, si
         return AL_ERR_ARG;
     }
     if (Dinv && std::is_same<TA, float>::value && nrhs > (size_t)NR_INV_MAX) {
-        // the inverse-based solve holds at most NR_INV_MAX right-hand sides: a launch per group
-        // (each launch is bound by its hop chain, not by the number of right-hand sides)
-        for (size_t c0 = 0; c0 < nrhs; c0 += NR_INV_MAX) {
-            const size_t nr = (nrhs - c0) < (size_t)NR_INV_MAX ? (nrhs - c0) : (size_t)NR_INV_MAX;
-            AL_CHECK(trsv_solve<TA>(s, ws, lower, unit, n, nr, A, lda, X + c0 * ldx, ldx, Dinv));
+        // the inverse-based solve holds at most NR_INV_MAX right-hand sides per chain; each chain
+        // is bound by its hop latency, so as many chains as fit on the CUs at one workgroup per CU
+        // (its LDS) run side by side in one launch
+        const size_t nblk2 = (n + TB2 - 1) / TB2;
+        const size_t fit = std::max<size_t>(1, (size_t)cu_count() / nblk2);
+        for (size_t c0 = 0; c0 < nrhs; c0 += NR_INV_MAX * fit) {
+            const size_t nr = std::min(nrhs - c0, (size_t)NR_INV_MAX * fit);
+            const unsigned nch = (unsigned)((nr + NR_INV_MAX - 1) / NR_INV_MAX);
+            if (nch == 1) {
+                AL_CHECK(trsv_solve<TA>(s, ws, lower, unit, n, nr, A, lda, X + c0 * ldx, ldx, Dinv));
+                continue;
+            }
+            if (lower)
+                AL_CHECK((unit ? launch_trsv<TA, true, true>(s, ws, n, nr, A, lda, X + c0 * ldx, ldx, Dinv, nch)
+                               : launch_trsv<TA, true, false>(s, ws, n, nr, A, lda, X + c0 * ldx, ldx, Dinv, nch)));
+            else
+                AL_CHECK((unit ? launch_trsv<TA, false, true>(s, ws, n, nr, A, lda, X + c0 * ldx, ldx, Dinv, nch)
+                               : launch_trsv<TA, false, false>(s, ws, n, nr, A, lda, X + c0 * ldx, ldx, Dinv, nch)));
         }
         return AL_OK;
     }

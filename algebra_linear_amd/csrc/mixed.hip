@@ -201,6 +201,18 @@ __global__ __launch_bounds__(256) void resid_k(int n, int nr, int nparts, const 
     }
 }
 
+// Row indices below 2^24 are exact in FP32, so the FP32 factor's interchanges can carry them
+// as one more column.
+__global__ void iota_f32_k(float* p, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = (float)i;
+}
+
+__global__ void widen_k(const float* __restrict__ src, double* __restrict__ dst, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] = (double)src[i];
+}
+
 __global__ void copy2d_k(int m, int n, const double* __restrict__ S, size_t lds, double* __restrict__ D, size_t ldd) {
     const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (idx >= (size_t)m * n) return;
@@ -292,6 +304,7 @@ struct Arena {
     float* F = nullptr;       // np x np factor
     int* ipiv = nullptr;      // np pivots of the FP32 factorization
     double* perm = nullptr;   // np row origins
+    float* rowidx = nullptr;  // np row origins as they are built: the interchanges applied to 0, 1, ..
     double* R = nullptr;      // n x nr residual
     double* Y = nullptr;      // np x nr permuted residual / correction
     double* part = nullptr;   // nparts x n x nr partial sums
@@ -339,6 +352,8 @@ int make_arena(hipStream_t s, Workspace* ws, size_t n, size_t np, size_t nr, siz
     if (lowp) off += align256(np * sizeof(int));
     const size_t oW = off;
     if (lowp) off += align256(np * sizeof(double));
+    const size_t oX = off;
+    if (lowp) off += align256(np * sizeof(float));
     const size_t oR = off;
     off += align256(n * nr * sizeof(double));
     const size_t oY = off;
@@ -369,6 +384,7 @@ int make_arena(hipStream_t s, Workspace* ws, size_t n, size_t np, size_t nr, siz
     a->F = (float*)(base + oF);
     a->ipiv = (int*)(base + oV);
     a->perm = (double*)(base + oW);
+    a->rowidx = lowp ? (float*)(base + oX) : nullptr;
     a->R = (double*)(base + oR);
     a->Y = (double*)(base + oY);
     a->part = (double*)(base + oP);
@@ -449,6 +465,8 @@ int getrf_lowp(hipStream_t s, Workspace* ws, Arena& a, bool bf16) {
     AL_CHECK(fill_i32(s, info, 1, 0));
     AL_CHECK(fill_u32(s, ws->abort_flag, 1, 0u));
     AL_CHECK(iota_i32(s, a.ipiv, a.np));
+    hipLaunchKernelGGL(iota_f32_k, dim3((unsigned)ceil_div(a.np, (size_t)256)), dim3(256), 0, s, a.rowidx, np);
+    AL_HIP(hipGetLastError());
     AL_HIP(hipEventRecord(ws->ev_main, s));  // also "the previous update is done" for the first panel
     AL_HIP(hipStreamWaitEvent(side, ws->ev_main, 0));
     // the outer panel [K, KE)'s interchanges (its 64-pivot blocks' pair lists, in order) on
@@ -477,6 +495,7 @@ int getrf_lowp(hipStream_t s, Workspace* ws, Arena& a, bool bf16) {
             AL_HIP(hipEventRecord(ws->ev_side, side));
             AL_HIP(hipStreamWaitEvent(s, ws->ev_side, 0));
             AL_CHECK(swaps(s, K, KE, 0, K, 0, 0));
+            AL_CHECK(swap_apply_multi_f32(s, lists(K), PAIR_STRIDE, (KE - K) / PB, a.rowidx, 1, nullptr, 0, ld, info));
             break;
         }
         const int KE2 = (np - KE) < nbo ? np : KE + nbo;
@@ -490,11 +509,15 @@ int getrf_lowp(hipStream_t s, Workspace* ws, Arena& a, bool bf16) {
         // main: every other column
         AL_HIP(hipStreamWaitEvent(s, ws->ev_side, 0));
         AL_CHECK(swaps(s, K, KE, 0, K, KE2, np - KE2));
+        AL_CHECK(swap_apply_multi_f32(s, lists(K), PAIR_STRIDE, (KE - K) / PB, a.rowidx, 1, nullptr, 0, ld, info));
         AL_CHECK(u12(s, K, KE2, np));
         AL_CHECK(trailing_update(s, a, bf16, K, KE, KE2, np));
         AL_HIP(hipEventRecord(ws->ev_main, s));
     }
-    return perm_from_ipiv(s, a.ipiv, a.np, a.perm);
+    // the row origins: every outer panel's interchanges were applied to 0, 1, .., np-1 in order
+    hipLaunchKernelGGL(widen_k, dim3((unsigned)ceil_div(a.np, (size_t)256)), dim3(256), 0, s, a.rowidx, a.perm, np);
+    AL_HIP(hipGetLastError());
+    return AL_OK;
 }
 
 template <int NR>

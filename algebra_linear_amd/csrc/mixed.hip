@@ -47,9 +47,10 @@ typedef unsigned long long u64;
 // F (np x np, FP32, ld np) := fl32([A 0; 0 pad I]), pad = |A|_inf / n (1 if that is zero or
 // not finite).
 __global__ __launch_bounds__(256) void to_f32_pad_k(int n, int np, const double* __restrict__ A, size_t lda,
-                                                    const u64* __restrict__ norms, float* __restrict__ F) {
-    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (idx >= (size_t)np * np) return;
+                                                    const u64* __restrict__ norms, float* __restrict__ F,
+                                                    size_t idx0, size_t idx1) {
+    const size_t idx = idx0 + (size_t)blockIdx.x * 256 + threadIdx.x;  // columns idx0 / np ..
+    if (idx >= idx1) return;
     const int i = (int)(idx % np), j = (int)(idx / np);
     float v;
     if (i < n && j < n) {
@@ -453,7 +454,19 @@ int ensure_side_stream(Workspace* ws) {
 // applies the trailing product there.  The next panel's preparation waits for that update of
 // the previous panel (its columns are among those updated); nothing else crosses streams,
 // because the two streams touch disjoint columns.
-int getrf_lowp(hipStream_t s, Workspace* ws, Arena& a, bool bf16) {
+// A (n x n, FP64) rounded into columns [c0, c1) of the padded FP32 factor a.F.
+int to_f32_cols(hipStream_t s, Arena& a, const double* A, size_t lda, size_t c0, size_t c1) {
+    if (c1 <= c0) return AL_OK;
+    const size_t i0 = c0 * a.np, i1 = c1 * a.np;
+    hipLaunchKernelGGL(to_f32_pad_k, dim3((unsigned)ceil_div(i1 - i0, (size_t)256)), dim3(256), 0, s, (int)a.n,
+                       (int)a.np, A, lda, a.norms, a.F, i0, i1);
+    AL_HIP(hipGetLastError());
+    return AL_OK;
+}
+
+// A, when given, is rounded into F here: the first outer panel's columns before the panel
+// stream starts, the rest on the main stream while that panel is factored.
+int getrf_lowp(hipStream_t s, Workspace* ws, Arena& a, bool bf16, const double* A = nullptr, size_t lda = 0) {
     const int np = (int)a.np;
     const int nbo = bf16 ? 512 : 256;
     float* F = a.F;
@@ -465,6 +478,7 @@ int getrf_lowp(hipStream_t s, Workspace* ws, Arena& a, bool bf16) {
     AL_CHECK(fill_i32(s, info, 1, 0));
     AL_CHECK(fill_u32(s, ws->abort_flag, 1, 0u));
     AL_CHECK(iota_i32(s, a.ipiv, a.np));
+    if (A) AL_CHECK(to_f32_cols(s, a, A, lda, 0, std::min<size_t>(a.np, (size_t)nbo)));
     hipLaunchKernelGGL(iota_f32_k, dim3((unsigned)ceil_div(a.np, (size_t)256)), dim3(256), 0, s, a.rowidx, np);
     AL_HIP(hipGetLastError());
     AL_HIP(hipEventRecord(ws->ev_main, s));  // also "the previous update is done" for the first panel
@@ -488,6 +502,11 @@ int getrf_lowp(hipStream_t s, Workspace* ws, Arena& a, bool bf16) {
         return AL_OK;
     };
     AL_CHECK(factor_panel(side, ws, a, 0, np < nbo ? np : nbo, lists(0)));
+    if (A && np > nbo) {
+        // the rest of F while the first panel is factored; the panel stream's next wait covers it
+        AL_CHECK(to_f32_cols(s, a, A, lda, (size_t)nbo, a.np));
+        AL_HIP(hipEventRecord(ws->ev_main, s));
+    }
     for (int K = 0; K < np; K += nbo) {
         const int kb = (np - K) < nbo ? (np - K) : nbo;
         const int KE = K + kb;
@@ -658,10 +677,7 @@ int gesv_mixed(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const double
         if (reuse) {
             minfo = reuse->breakdown;
         } else {
-            hipLaunchKernelGGL(to_f32_pad_k, dim3(ceil_div(np * np, 256)), dim3(256), 0, s, (int)n, (int)np, A, lda,
-                               a.norms, a.F);
-            AL_HIP(hipGetLastError());
-            AL_CHECK(getrf_lowp(s, ws, a, bf16));
+            AL_CHECK(getrf_lowp(s, ws, a, bf16, A, lda));  // rounds A into F as it goes
             AL_HIP(hipMemcpyAsync(&minfo, a.minfo, sizeof minfo, hipMemcpyDeviceToHost, s));
             AL_HIP(hipStreamSynchronize(s));
             if (minfo < 0) {

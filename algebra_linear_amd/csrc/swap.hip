@@ -120,8 +120,11 @@ __global__ __launch_bounds__(256) void apply_pairs(T* __restrict__ A0, int n0, T
 // Several pivot blocks' pair lists applied one after the other, as their swap sequences are:
 // lists l = 0..nl-1 at pairs + l * stride, each in piv_pairs' format with at most
 // SUB_PAIRS_MAX pairs (one 64-pivot block).  One wave per column, so the lists need no
-// synchronization between them: a wave finishes a list's reads before its writes, and a list's
-// writes before the next list's reads.
+// synchronization between workgroups: a wave issues a list's reads before its writes, and a
+// list's writes before the next list's reads.  A later list may read, in another lane, a row an
+// earlier list wrote; that relies on the hardware processing one wave's vector memory
+// instructions in order (the vector L1 is write-through).  The result was checked bit for bit
+// against applying the blocks' composed permutation instead.
 constexpr int SUB_PAIRS_MAX = 128;
 constexpr int MULTI_MAX = 8;
 __global__ __launch_bounds__(256) void apply_pairs_multi(float* __restrict__ A0, int n0, float* __restrict__ A1,

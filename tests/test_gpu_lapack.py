@@ -5,7 +5,7 @@ Bars:
     operations in the crate's order without FMA, so LU factors, pivots, determinant and the
     Cholesky factor are compared BIT FOR BIT with the oracle;
   * larger n: normwise backward error of the factorization ||PA - LU||_1/(n eps ||A||_1)
-    (DGET01) < 30 and ||A - LL^T||_1/(n eps ||A||_1) < 30 (the bar of the north star), the
+    (DGET01) < 30 and ||A - LL^T||_1/(n eps ||A||_1) < 30 (the LAPACK testing bar), the
     solve's normwise backward error ||b - Ax||_inf/(||A|| ||x|| + ||b||) <= 1e-14 (computed
     on the CPU in FP64 with compensated sums), and the pivots agree with the oracle;
   * error paths exactly as the crate's tests (src/lapack/mod.rs:497-616, :670-751).

@@ -1,7 +1,7 @@
 """GPU tests of the mixed-precision solve (low-precision randomized LU + FP64 refinement).
 
 The crate has no mixed-precision solver, so the bar is the contract of its FP64 dgesv
-measured the way the north star states it: the normwise backward error
+measured as the normwise backward error
 ||b - Ax||_inf / (||A||_inf ||x||_inf + ||b||_inf) of every returned column, recomputed on
 the CPU in FP64 with compensated sums (the oracle's or_backward_error), is <= 1e-14.
 Matrices with a prescribed condition number come from the oracle's dlatms (the crate's

@@ -1,4 +1,4 @@
-// Mixed-precision solve of A X = B: partial-pivoting LU in low precision, FP64 iterative
+// Mixed-precision solve of A X = B: pivoted LU in low precision, FP64 iterative
 // refinement, FP64 pivoted-LU fallback.  The crate solves in FP64 only (lapack::dgesv,
 // src/lapack/mod.rs:309-323); this is the GPU extension beside it, with the same result
 // contract measured as normwise backward error.
@@ -17,9 +17,10 @@
 //                           vectors, X += d
 //   fallback                FP64 pivoted LU (getrf_f64/getrs_f64) on a copy of A
 //
-// Partial pivoting is not optional here: without it (even after a random butterfly
-// transform) the LU of a dense random matrix has |L||U| / |A| of 1e4-1e5 at n = 1024, enough
-// to stall FP32 refinement at n = 16384 and to make a bf16 update useless.
+// Pivoting is not optional here: without it (even after a random butterfly transform) the LU
+// of a dense random matrix has |L||U| / |A| of 1e4-1e5 at n = 1024, enough to stall FP32
+// refinement at n = 16384 and to make a bf16 update useless.  Tournament pivoting keeps the
+// growth of partial pivoting without its per-column grid-wide exchange.
 #include <algorithm>
 #include <cfloat>
 #include <cmath>

@@ -1,4 +1,4 @@
-"""GPU tests of the mixed-precision solve (low-precision randomized LU + FP64 refinement).
+"""GPU tests of the mixed-precision solve (FP32 or bf16-update LU with tournament pivoting + FP64 refinement).
 
 The crate has no mixed-precision solver, so the bar is the contract of its FP64 dgesv
 measured as the normwise backward error

@@ -1,9 +1,10 @@
 // Lower Cholesky, lapack::dpotrf (src/lapack/mod.rs:368-399).
 //
 // The crate runs a left-looking dot-product loop over the whole matrix.  Here it is a
-// right-looking blocked factorization: a 64x64 diagonal block is factored in LDS by one
-// workgroup (`potf2_lds`, the crate's own dot-product order per element, unfused, so a matrix
-// of order <= 64 factors bit-for-bit as the crate factors it), the panel below it is a
+// right-looking blocked factorization: a 64x64 diagonal block is factored by one wave with its
+// rows in registers (`potf2_wave`, the crate's own dot-product order per element, unfused, so a
+// matrix of order <= 64 factors bit-for-bit as the crate factors it; `potf2_lds` is the same
+// arithmetic with the block in LDS, kept as the reference form), the panel below it is a
 // triangular solve against the block's transpose, and the trailing matrix gets a
 // lower-triangle-only FP64 MFMA rank-k update.  Two levels (64 inside 256) keep the trailing
 // update at depth 256.  Only the lower triangle is read or written anywhere.
@@ -16,6 +17,52 @@ constexpr int PB = 64;
 constexpr int PBO = 256;
 
 #pragma clang fp contract(off)
+__device__ __forceinline__ double lane_d(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | lo);
+}
+
+// The 64 x 64 diagonal block in one wave, lane i holding row i of the lower triangle in
+// registers.  Column j: every lane runs s = a[i][j] - sum_k L[i][k] L[j][k] over k < j in
+// order, unfused, L[j][k] taken from lane j by v_readlane -- in lane j that is the diagonal's
+// own sum -- then lane j takes the square root and the lanes below divide by it.  Each element
+// sees exactly the operations of the crate's dot-product order (as potf2_lds below), with no
+// LDS or barrier on the chain.  On a non-positive pivot the block is left as potf2_lds leaves
+// it: columns before j factored, column j and those after it as given.
+__global__ __launch_bounds__(64) void potf2_wave(double* A, size_t lda, int nb, int col0, int* info) {
+    if (*info != 0) return;
+    const int lane = threadIdx.x;
+    const bool in = lane < nb;
+    double x[PB];
+#pragma unroll
+    for (int k = 0; k < PB; ++k) x[k] = (in && k <= lane && k < nb) ? A[(size_t)lane + (size_t)k * lda] : 0.0;
+    int done = nb;     // columns factored
+    bool alive = true;  // no non-positive pivot yet (uniform)
+#pragma unroll
+    for (int j = 0; j < PB; ++j) {
+        const bool col = alive && j < nb;
+        double sacc = x[j];
+#pragma unroll
+        for (int k = 0; k < j; ++k) sacc -= x[k] * lane_d(x[k], j);
+        const double sum = lane_d(sacc, j);  // the diagonal's sum, from lane j
+        if (col && sum <= 0.0) {
+            if (lane == 0) *info = col0 + j + 1;
+            done = j;
+            alive = false;
+        } else if (col) {
+            const double ljj = sqrt(sum);
+            x[j] = lane == j ? ljj : (lane > j ? sacc / ljj : x[j]);
+        }
+    }
+    if (in) {
+#pragma unroll
+        for (int k = 0; k < PB; ++k)
+            if (k <= lane && k < done) A[(size_t)lane + (size_t)k * lda] = x[k];
+    }
+}
+
 __global__ __launch_bounds__(256) void potf2_lds(double* A, size_t lda, int nb, int col0, int* info) {
     if (*info != 0) return;
     __shared__ double L[PB][PB + 1];
@@ -64,7 +111,7 @@ int potrf_level(hipStream_t s, size_t n, double* A, size_t lda, int* info, size_
         const size_t jb = (n - k) < nb ? (n - k) : nb;
         double* Akk = A + k + k * lda;
         if (jb <= (size_t)PB) {
-            hipLaunchKernelGGL(potf2_lds, dim3(1), dim3(256), 0, s, Akk, lda, (int)jb, (int)(col0 + k), info);
+            hipLaunchKernelGGL(potf2_wave, dim3(1), dim3(64), 0, s, Akk, lda, (int)jb, (int)(col0 + k), info);
             AL_HIP(hipGetLastError());
         } else {
             AL_CHECK(potrf_level(s, jb, Akk, lda, info, col0 + k, PB));

@@ -248,7 +248,10 @@ __global__ __launch_bounds__(NT, 1) void tslu_k(const float* __restrict__ F, siz
             for (int v = 0; v < W / 4; ++v) dst[v] = float4{x0[4 * v], x0[4 * v + 1], x0[4 * v + 2], x0[4 * v + 3]};
             cidx[node * W + step] = row;
         }
-        __threadfence();
+        // each wave's nominee stores reach the L2 before the barrier; thread 0's acquire-release
+        // arrival then writes the L2 back once for the whole workgroup (and, for the merging
+        // workgroup, invalidates once), instead of every wave fencing on its own
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
         __syncthreads();
         const int parent_level = level + 1;
         const int pidx = (node - tree.offset[level]) / ARITY;
@@ -261,7 +264,6 @@ __global__ __launch_bounds__(NT, 1) void tslu_k(const float* __restrict__ F, siz
             const int last = prev == (unsigned)(nchild - 1);
             if (last) counters[parent] = 0u;  // every child has arrived: ready for the next panel
             L.last = last;
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
         __syncthreads();
         if (!L.last) return;

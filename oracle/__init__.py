@@ -126,6 +126,12 @@ def dgemv(alpha, a, x, beta, y, m, n) -> np.ndarray:
     return y
 
 
+def dgemv_trans(alpha, a, x, beta, y, m, n) -> np.ndarray:
+    a, x, y = _f64(a), _f64(x), _f64(y).copy()
+    lib().or_dgemv_trans(alpha, _dp(a), _dp(x), beta, _dp(y), m, n)
+    return y
+
+
 def gemm(ta: bool, tb: bool, alpha, a, b, beta, c, m, n, k) -> np.ndarray:
     """``gemm_core`` of src/blas.rs:283; returns the new C."""
     a, b, c = _f64(a), _f64(b), _f64(c).copy()

@@ -331,3 +331,199 @@ def test_backward_error_is_compensated():
     x = np.array([1.0, 1.0])
     b = np.array([0.0, 2.0])
     assert O.backward_error(a, x, b, 2) == 0.0
+
+
+# ------------------------------------------------------------------------------------------
+# level 1/2 and norms: idamax, dnrm2, dgemv, dgemv_trans, dlange (src/blas.rs:49-158, :628-657)
+# ------------------------------------------------------------------------------------------
+
+def _a23():
+    """src/blas.rs:669-671: [[1,2,3],[4,5,6]] column-major."""
+    return np.array([1.0, 4.0, 2.0, 5.0, 3.0, 6.0])
+
+
+def _real_matrix(m, n):
+    """src/blas.rs:2377-2379."""
+    return np.array([float((i * 7) % 19) - 9.0 + i * 0.125 for i in range(m * n)])
+
+
+# Plausible mistakes in an idamax restatement; each pin vector below must tell at least one of
+# them from the crate's rule (strict '>' from -inf: first maximum, NaN never wins).
+def _idamax_last_max(x):
+    best, ba = 0, -math.inf
+    for i, v in enumerate(x):
+        if abs(v) >= ba:
+            best, ba = i, abs(v)
+    return best
+
+
+def _idamax_nan_wins(x):
+    best, ba = 0, -math.inf
+    for i, v in enumerate(x):
+        if not abs(v) <= ba:
+            best, ba = i, abs(v)
+    return best
+
+
+def _idamax_start_at_first(x):
+    # seeds the scan with x[0] instead of -inf: a NaN at index 0 then blocks every later entry
+    if len(x) == 0:
+        return 0
+    best, ba = 0, abs(x[0])
+    for i, v in enumerate(x):
+        if abs(v) > ba:
+            best, ba = i, abs(v)
+    return best
+
+
+IDAMAX_PINS = [
+    ([], 0),                                  # empty slice yields 0 (src/blas.rs:75-76)
+    ([1.0, -3.0, 3.0, 2.0], 1),               # ties: the first maximum wins
+    ([-2.0, 5.0, -5.0, 5.0], 1),
+    ([1.0, math.nan, 0.5], 0),                # a later NaN is skipped
+    ([math.nan, 2.0, -7.0], 2),               # a NaN at index 0 does not block later entries
+    ([math.nan, math.nan], 0),                # nothing beats -inf: index 0
+    ([0.0, -0.0, 0.0], 0),                    # signed zeros tie
+    ([3.0, -math.inf, math.inf], 1),          # infinities compare by magnitude, first wins
+]
+
+
+@pytest.mark.parametrize("x,want", IDAMAX_PINS)
+def test_idamax_first_maximum_and_nan_rule(x, want):
+    assert O.idamax(np.array(x, dtype=np.float64)) == want
+
+
+def test_idamax_pins_reject_plausible_mistakes():
+    for mutant in (_idamax_last_max, _idamax_nan_wins, _idamax_start_at_first):
+        assert any(mutant(x) != want for x, want in IDAMAX_PINS), mutant.__name__
+    rng = np.random.default_rng(3)
+    for n in (1, 2, 17, 1000):
+        x = rng.integers(-4, 5, size=n).astype(np.float64)  # many ties
+        assert O.idamax(x) == int(np.argmax(np.abs(x)))     # numpy's argmax is first-max too
+
+
+def test_dnrm2_closed_forms_and_scaling():
+    # src/blas.rs:691-692 and :697-708
+    assert O.dnrm2(np.array([3.0, 4.0])) == 5.0
+    assert O.dnrm2(np.array([])) == 0.0
+    big = O.dnrm2(np.array([1e200, 1e200]))
+    assert math.isfinite(big) and abs(big / (1e200 * math.sqrt(2.0)) - 1.0) < 1e-12
+    tiny = O.dnrm2(np.array([1e-200, 1e-200]))
+    assert tiny > 0.0 and abs(tiny / (1e-200 * math.sqrt(2.0)) - 1.0) < 1e-12
+    # a naive sum of squares fails both of the last two; the scaled one does not
+    with np.errstate(over="ignore", under="ignore"):
+        naive = [np.sqrt(np.sum(np.array([v, v]) * np.array([v, v]))) for v in (1e200, 1e-200)]
+    assert not np.isfinite(naive[0]) and naive[1] == 0.0
+    assert math.isnan(O.dnrm2(np.array([1.0, math.nan])))     # a NaN propagates
+    assert O.dnrm2(np.array([1.0, math.inf])) == math.inf
+    rng = np.random.default_rng(5)
+    for n in (1, 7, 1000):
+        x = rng.standard_normal(n)
+        assert abs(O.dnrm2(x) - np.linalg.norm(x)) <= 1e-14 * np.linalg.norm(x)
+
+
+def test_dgemv_hand_values_and_definition():
+    # src/blas.rs:711-723 on the 2x3 matrix a23
+    a = _a23()
+    assert list(O.dgemv(1.0, a, [1.0, 1.0, 1.0], 0.0, np.zeros(2), 2, 3)) == [6.0, 15.0]
+    assert list(O.dgemv(2.0, a, [1.0, 1.0, 1.0], 3.0, np.ones(2), 2, 3)) == [2 * 6.0 + 3, 2 * 15.0 + 3]
+    # a distinct x catches a transposed operand or a wrong column stride
+    assert list(O.dgemv(1.0, a, [1.0, 10.0, 100.0], 0.0, np.zeros(2), 2, 3)) == [321.0, 654.0]
+    # beta = 0 overwrites y (a NaN there does not survive); x_j = 0 skips column j
+    y = O.dgemv(1.0, np.array([1.0, 2.0, math.nan, math.nan]), [1.0, 0.0], 0.0, np.array([math.nan] * 2), 2, 2)
+    assert list(y) == [1.0, 2.0]
+    rng = np.random.default_rng(7)
+    for m, n in ((1, 1), (5, 3), (3, 5), (64, 33)):
+        A, x, y0 = rng.standard_normal((m, n)), rng.standard_normal(n), rng.standard_normal(m)
+        got = O.dgemv(-1.25, cm(A), x, 0.5, y0, m, n)
+        want = -1.25 * (A @ x) + 0.5 * y0
+        assert np.allclose(got, want, rtol=1e-13, atol=1e-13 * np.abs(A).sum())
+
+
+def test_dgemv_trans_matches_the_definition():
+    # src/blas.rs:2410-2432: a 2x3, x 2 entries, y 3 entries
+    m, n = 2, 3
+    a, x = _real_matrix(m, n), _real_matrix(m, 1)
+    for alpha, beta in ((1.0, 0.0), (-2.5, 1.75)):
+        y0 = _real_matrix(1, n)
+        want = np.array([alpha * sum(a[i + j * m] * x[i] for i in range(m)) + beta * y0[j] for j in range(n)])
+        got = O.dgemv_trans(alpha, a, x, beta, y0, m, n)
+        assert np.all(np.abs(got - want) / np.maximum(np.abs(want), 1.0) < 1e-13)
+    # beta = 0 ignores a NaN in y; the transposed product differs from the plain one here
+    got = O.dgemv_trans(1.0, _a23(), [1.0, 10.0], 0.0, np.array([math.nan] * 3), 2, 3)
+    assert list(got) == [41.0, 52.0, 63.0]
+    rng = np.random.default_rng(8)
+    for m, n in ((1, 4), (6, 2), (70, 31)):
+        A, x = rng.standard_normal((m, n)), rng.standard_normal(m)
+        assert np.allclose(O.dgemv_trans(1.0, cm(A), x, 0.0, np.zeros(n), m, n), A.T @ x, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("norm,want", [("M", 6.0), ("m", 6.0), ("1", 9.0), ("O", 9.0), ("o", 9.0), ("I", 15.0),
+                                       ("i", 15.0), ("F", math.sqrt(91.0)), ("f", math.sqrt(91.0)),
+                                       ("E", math.sqrt(91.0)), ("e", math.sqrt(91.0))])
+def test_dlange_every_norm_on_the_crate_matrix(norm, want):
+    # src/blas.rs:875-887; the one- and infinity-norms differ on a23, so swapped loops fail
+    got = O.dlange(norm, _a23(), 2, 3)
+    assert abs(got - want) <= 1e-12 * want
+
+
+def test_dlange_against_numpy_and_nan_rule():
+    rng = np.random.default_rng(9)
+    A = rng.standard_normal((7, 4))
+    a = cm(A)
+    assert O.dlange("M", a, 7, 4) == np.abs(A).max()
+    assert abs(O.dlange("1", a, 7, 4) - np.linalg.norm(A, 1)) <= 1e-14 * np.linalg.norm(A, 1)
+    assert abs(O.dlange("I", a, 7, 4) - np.linalg.norm(A, np.inf)) <= 1e-14 * np.linalg.norm(A, np.inf)
+    assert abs(O.dlange("F", a, 7, 4) - np.linalg.norm(A, "fro")) <= 1e-14 * np.linalg.norm(A, "fro")
+    # f64::max ignores a NaN operand, so 'M' skips NaN entries; 'F' goes through dnrm2 (NaN)
+    b = np.array([math.nan, -3.0, 2.0, 1.0])
+    assert O.dlange("M", b, 2, 2) == 3.0
+    assert math.isnan(O.dlange("F", b, 2, 2))
+    assert O.dlange("M", np.zeros(0), 0, 0) == 0.0
+    assert math.isnan(O.dlange("X", b, 2, 2))   # the crate panics; the oracle returns NaN
+
+
+def test_laruv_laran_are_the_uniform_stream():
+    # src/lapack/families.rs:230-260: DLARNV's uniform case copies DLARUV, and DLARAN is DLARUV
+    # with n = 1; both advance the seed by the draws taken
+    x, s1 = O.laruv(SEEDY, 40)
+    y, s2 = O.larnv(O.UNIFORM, SEEDY, 40)
+    assert np.array_equal(x, y) and list(s1) == list(s2)
+    seed = np.array(SEEDY, dtype=np.uint32)
+    for i in range(5):
+        v = O.lib().or_laran(seed.ctypes.data_as(O._c_u32_p))
+        assert v == x[i]
+    assert np.all((x > 0.0) & (x < 1.0))
+
+
+@pytest.mark.parametrize("mode", [1, 2, 3, 4, -3])
+def test_latm1_closed_form_spectra(mode):
+    # src/lapack/families.rs:262-356: modes 1-4 are closed forms of cond; a negative mode reverses
+    n, cond = 9, 1e4
+    info, d, _ = O.latm1(mode, cond, 0, O.UNIFORM, SEEDY, np.zeros(n))
+    assert info == 0
+    i = np.arange(n)
+    want = {1: np.where(i == 0, 1.0, 1.0 / cond), 2: np.where(i == n - 1, 1.0 / cond, 1.0),
+            3: cond ** (-i / (n - 1.0)), 4: 1.0 - i / (n - 1.0) * (1.0 - 1.0 / cond)}[abs(mode)]
+    if mode < 0:
+        want = want[::-1]
+    assert np.allclose(d, want, rtol=1e-14, atol=1e-15)
+    assert O.latm1(7, cond, 0, O.UNIFORM, SEEDY, np.zeros(n))[0] == -1
+    assert O.latm1(3, 0.5, 0, O.UNIFORM, SEEDY, np.zeros(n))[0] == -3
+
+
+def test_dgetrs_applies_the_pivots_and_solves():
+    # src/lapack/mod.rs:537-558: x = [1, -2, 3, 5] through dgetrf + dgetrs, one call per column
+    a = sample4()
+    x = np.array([1.0, -2.0, 3.0, 5.0])
+    b = mat(a, 4, 4) @ x
+    st, lu, ipiv, _ = O.dgetrf(a, 4)
+    assert st == 0
+    assert np.allclose(O.dgetrs(lu, ipiv, b), x, atol=1e-14)
+    # pivots that really swap: a permuted identity's LU is trivial, dgetrs must undo P
+    p = np.array([2, 0, 3, 1])
+    A = np.eye(4)[p] * np.array([2.0, 3.0, 5.0, 7.0])[:, None]
+    st, lu, ipiv, _ = O.dgetrf(cm(A), 4)
+    assert st == 0 and list(ipiv) != [0, 1, 2, 3]
+    b = A @ x
+    assert np.allclose(O.dgetrs(lu, ipiv, b), x, atol=1e-14)

@@ -94,6 +94,8 @@ SIGNATURES = {
     "al_dgesv_mixed": (_i, [_dp, _sz, _dp, _sz, _sz, _sz, _i, ctypes.POINTER(IrOptions), ctypes.POINTER(IrStats)]),
     "al_dev_dgesv_mixed": (_i, [_sz, _sz, _v, _sz, _v, _sz, _v, _sz, _i, ctypes.POINTER(IrOptions),
                                 ctypes.POINTER(IrStats), _v]),
+    "al_debug_set_wait_limit": (None, [ctypes.c_longlong]),
+    "al_debug_coop_units_in_use": (_i, []),
     "al_prof_enable": (None, [_i]),
     "al_prof_reset": (None, []),
     "al_prof_query": (_i, [ctypes.c_char_p, ctypes.POINTER(_i), ctypes.POINTER(_d), ctypes.POINTER(_d),

@@ -176,10 +176,31 @@ int perm_from_ipiv(hipStream_t s, const int* ipiv, size_t n, double* perm);    /
 int gather_rows(hipStream_t s, const double* B, size_t ldb, size_t n, size_t nrhs, const double* perm, double* W);
 int copy_block(hipStream_t s, const double* W, size_t n, size_t nrhs, double* B, size_t ldb);
 
+// Compute units of the current device (256 on MI355X).
+int cu_count();
+
+// Limit, in ticks of the 100 MHz wall clock, of every bounded device-side wait (the cooperative
+// panel's column exchange, the one-launch solves' block flags).  Default 2 s; a test may lower
+// it to force the give-up path from the host (al_debug_set_wait_limit).
+long long wait_limit_ticks();
+void set_wait_limit_ticks(long long ticks);  // <= 0 restores the default
+
+// Cooperative-panel room (coop_ledger.hpp): coop_reserve waits until the panels of an m-row
+// factorization fit beside those of unfinished ones and returns the units reserved (0 when its
+// panels use one workgroup, or while `s` is being captured into a graph); coop_hold ties them to
+// an event recorded on `s` now; coop_release returns them at once.  coop_in_use: units held.
+int coop_reserve(hipStream_t s, size_t m, int* units);
+int coop_hold(hipStream_t s, int units);
+void coop_release(int units);
+int coop_in_use();
+
 // Blocked LU with partial pivoting of the m x n matrix A (m >= n for the panel form).  If perm
 // is not null it receives the row origins (perm[i] = original row now at row i, m entries).
 int getrf_f64(hipStream_t s, Workspace* ws, size_t m, size_t n, double* A, size_t lda, int* ipiv, int* info,
               double* perm = nullptr);
+// The same without the cooperative-panel reservation (the caller holds one).
+int getrf_enqueue(hipStream_t s, Workspace* ws, size_t m, size_t n, double* A, size_t lda, int* ipiv, int* info,
+                  double* perm);
 
 // One cooperative 64-wide (ncols <= 64) partial-pivoting panel of an FP32 matrix: the m x
 // ncols block at P (rows row0.. of the whole matrix), pivots written to ipiv[row0..] as

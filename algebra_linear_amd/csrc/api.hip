@@ -2,6 +2,7 @@
 // per-thread streams and device buffers, host<->HBM staging, and status mapping.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <climits>
 #include <cstdio>
 #include <cstring>
@@ -219,6 +220,20 @@ int getrf_host(double* a, size_t M, size_t N, size_t* ipiv, hipStream_t s, Works
     return status_from_info(info);
 }
 
+// Copies a solve's result back and reports a device-side wait that gave up during it (its
+// rows of the result are NaN then) as AL_ERR_DEVICE.
+int download_solution(double* h, const double* d, size_t n, hipStream_t s, Workspace* ws) {
+    unsigned aborted = 0;
+    AL_CHECK(download(h, d, n, s));
+    AL_HIP(hipMemcpyAsync(&aborted, ws->abort_flag, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    AL_CHECK(finish(s));
+    if (aborted) {
+        set_error("device-side timeout in a triangular solve's block wait");
+        return AL_ERR_DEVICE;
+    }
+    return AL_OK;
+}
+
 int check_pivots(const size_t* ipiv, size_t n) {
     for (size_t k = 0; k < n; ++k)
         if (ipiv[k] >= n) {
@@ -263,6 +278,19 @@ int ws_reserve(Workspace* ws, size_t n, size_t nw) {
 }
 
 int workspace_on(hipStream_t s, Workspace** out) { return workspace_for(s, out, 1); }
+
+int cu_count() {
+    static std::atomic<int> cus{0};
+    int v = cus.load(std::memory_order_relaxed);
+    if (v == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
+            v = 256;
+        cus.store(v, std::memory_order_relaxed);
+    }
+    return v;
+}
 
 int ws_reserve_flags(Workspace* ws, size_t entries) {
     if (ws->trsv_flags_len < entries) {
@@ -411,8 +439,7 @@ int al_dgetrs_nrhs(const double* lu, size_t lu_len, const size_t* ipiv, size_t i
     AL_CHECK(upload(1, b, b_len, &dB, s));
     AL_HIP(hipMemcpyAsync(ws->perm, perm.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
     AL_CHECK(getrs_f64(s, ws, n, nrhs, dLU, n, nullptr, ws->perm, dB, n));
-    AL_CHECK(download(b, dB, b_len, s));
-    return finish(s);
+    return download_solution(b, dB, b_len, s, ws);
 }
 
 int al_dgetrs(const double* lu, size_t lu_len, const size_t* ipiv, size_t ipiv_len, double* b, size_t b_len) {
@@ -436,8 +463,7 @@ int al_dgesv_nrhs(double* a, size_t a_len, double* b, size_t b_len, size_t n, si
     AL_CHECK(upload(1, b, b_len, &dB, s));
     AL_CHECK(ws_reserve(ws, n, n * nrhs));
     AL_CHECK(getrs_f64(s, ws, n, nrhs, dA, n, ws->ipiv32, ws->perm, dB, n));
-    AL_CHECK(download(b, dB, b_len, s));
-    return finish(s);
+    return download_solution(b, dB, b_len, s, ws);
 }
 
 int al_dgesv(double* a, size_t a_len, double* b, size_t b_len) {
@@ -626,6 +652,13 @@ int al_dev_batch_solve(size_t count, const al_batch_entry* d_entries, double* A_
     unsigned* queue = (unsigned*)ws->bbuf;
     float* W = (float*)((char*)ws->bbuf + 256);
     return batch_solve(s, count, d_entries, A_pool, B_pool, W, d_info, d_steps, queue, mixed, 10);
+}
+
+void al_debug_set_wait_limit(long long ticks) { set_wait_limit_ticks(ticks); }
+
+int al_debug_coop_units_in_use(void) {
+    if (ensure_device() != AL_OK) return 0;
+    return coop_in_use();
 }
 
 int al_dev_dlaswp(size_t ncols, double* A, size_t lda, size_t k0, size_t k1, const int32_t* d_ipiv, void* stream) {

@@ -16,8 +16,12 @@
 //     update are separate kernels on the same stream; each checks the status word first so
 //     that a zero pivot stops the factorization like the crate's early return.
 #include "al_internal.hpp"
+#include "coop_ledger.hpp"
 
+#include <atomic>
 #include <cmath>
+#include <mutex>
+#include <vector>
 
 namespace al {
 namespace {
@@ -344,7 +348,7 @@ int launch_panel(hipStream_t s, Workspace* ws, T* Ppanel, size_t lda, size_t row
         AL_CHECK(fill_u32(s, ws->flags, (size_t)MAX_WG * 16, 0u));
         ws->gen = 0;
     }
-    const long long timeout = 200000000LL;  // 2 s at 100 MHz
+    const long long timeout = wait_limit_ticks();
     const unsigned gen0 = ws->gen;
     ws->gen += (unsigned)ncols;
     const int ticket = prof_begin(s);
@@ -395,13 +399,136 @@ int panel_f32(hipStream_t s, Workspace* ws, float* P, size_t lda, size_t row0, s
     return launch_panel<float>(s, ws, P, lda, row0, m, ncols, ipiv, info);
 }
 
+namespace {
+
+std::atomic<long long> g_wait_limit{0};
+
+// Cooperative panel workgroups in flight, per device (see coop_ledger.hpp), with a pool of
+// timing-free events that mark the end of each reserving factorization.
+struct PanelLedger {
+    CoopLedger<hipEvent_t> ledger;
+    std::mutex pool_mu;
+    std::vector<hipEvent_t> pool;
+};
+constexpr int MAX_DEVICES = 64;
+PanelLedger g_ledgers[MAX_DEVICES];
+
+PanelLedger* ledger_of_current() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return nullptr;
+    PanelLedger* L = &g_ledgers[dev];
+    if (L->ledger.capacity() == 0) L->ledger.set_capacity(cu_count());
+    return L;
+}
+
+bool event_done(hipEvent_t e) {
+    const hipError_t r = hipEventQuery(e);
+    if (r == hipErrorNotReady) return false;
+    if (r != hipSuccess) (void)hipGetLastError();  // a failed event frees its reservation too
+    return true;
+}
+
+// Workgroups of the largest (first) cooperative panel of an m-row factorization; 0 when the
+// panels run in one workgroup, which waits on nobody.
+int panel_units(size_t m) {
+    if (m <= (size_t)CHUNK_MAX) return 0;
+    size_t nwg = (m + CHUNK_MAX - 1) / CHUNK_MAX;
+    return (int)(nwg > (size_t)MAX_WG ? (size_t)MAX_WG : nwg);
+}
+
+}  // namespace
+
+long long wait_limit_ticks() {
+    const long long v = g_wait_limit.load(std::memory_order_relaxed);
+    return v > 0 ? v : 200000000LL;  // default: 2 s of the 100 MHz wall clock
+}
+
+void set_wait_limit_ticks(long long ticks) { g_wait_limit.store(ticks, std::memory_order_relaxed); }
+
+int coop_reserve(hipStream_t s, size_t m, int* units_out) {
+    *units_out = 0;
+    const int units = panel_units(m);
+    if (units == 0) return AL_OK;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) return AL_OK;
+    PanelLedger* L = ledger_of_current();
+    if (!L) return AL_OK;
+    auto recycle = [L](hipEvent_t e) {
+        std::lock_guard<std::mutex> lk(L->pool_mu);
+        L->pool.push_back(e);
+    };
+    *units_out = L->ledger.acquire(units, event_done, recycle);
+    return AL_OK;
+}
+
+int coop_hold(hipStream_t s, int units) {
+    if (units <= 0) return AL_OK;
+    PanelLedger* L = ledger_of_current();
+    if (!L) return AL_OK;
+    hipEvent_t e = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(L->pool_mu);
+        if (!L->pool.empty()) {
+            e = L->pool.back();
+            L->pool.pop_back();
+        }
+    }
+    if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) e = nullptr;
+    if (!e || hipEventRecord(e, s) != hipSuccess) {
+        // no event to tie the reservation to: wait for the work, then return it
+        if (e) {
+            std::lock_guard<std::mutex> lk(L->pool_mu);
+            L->pool.push_back(e);
+        }
+        (void)hipGetLastError();
+        const hipError_t r = hipStreamSynchronize(s);
+        L->ledger.release(units);
+        AL_HIP(r);
+        return AL_OK;
+    }
+    L->ledger.hold_until(e, units);
+    return AL_OK;
+}
+
+void coop_release(int units) {
+    if (units <= 0) return;
+    PanelLedger* L = ledger_of_current();
+    if (L) L->ledger.release(units);
+}
+
+int coop_in_use() {
+    PanelLedger* L = ledger_of_current();
+    if (!L) return 0;
+    auto recycle = [L](hipEvent_t e) {
+        std::lock_guard<std::mutex> lk(L->pool_mu);
+        L->pool.push_back(e);
+    };
+    return L->ledger.in_use(event_done, recycle);
+}
+
 int getrf_f64(hipStream_t s, Workspace* ws, size_t M, size_t N, double* A, size_t lda, int* ipiv, int* info,
               double* perm) {
-    const size_t kmax = M < N ? M : N;
     if (M > (size_t)CHUNK_MAX * MAX_WG || N > INT32_MAX) {
         set_error("getrf: more rows than the cooperative panel holds (32768)");
         return AL_ERR_ARG;
     }
+    // the panels' workgroups wait on each other: reserve room for them beside the cooperative
+    // panels of other factorizations in flight, and hold it until this one's last kernel ends
+    int units = 0;
+    AL_CHECK(coop_reserve(s, M, &units));
+    const int st = getrf_enqueue(s, ws, M, N, A, lda, ipiv, info, perm);
+    if (st != AL_OK) {
+        // kernels already enqueued may still run: let them finish before the room is returned
+        (void)hipStreamSynchronize(s);
+        coop_release(units);
+        return st;
+    }
+    return coop_hold(s, units);
+}
+
+int getrf_enqueue(hipStream_t s, Workspace* ws, size_t M, size_t N, double* A, size_t lda, int* ipiv, int* info,
+                  double* perm) {
+    const size_t kmax = M < N ? M : N;
     AL_CHECK(fill_i32(s, info, 1, 0));
     AL_CHECK(fill_u32(s, ws->abort_flag, 1, 0u));
     if (perm) AL_CHECK(perm_init(s, perm, M));
@@ -425,6 +552,9 @@ int getrs_f64(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const double*
               const double* perm, double* B, size_t ldb, const int* info) {
     if (n == 0 || nrhs == 0) return AL_OK;
     AL_CHECK(ws_reserve(ws, n, n * nrhs));
+    // the solves' waits raise this flag when they give up; it must not carry over from an
+    // earlier call on this workspace
+    AL_CHECK(fill_u32(s, ws->abort_flag, 1, 0u));
     if (!perm) {
         AL_CHECK(perm_from_ipiv(s, ipiv, n, ws->perm));
         perm = ws->perm;

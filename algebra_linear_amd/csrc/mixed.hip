@@ -321,7 +321,7 @@ struct Arena {
     double* dinv = nullptr;   // FP64 inverses of the factor's 128-row diagonal blocks: L's, then U's
 };
 
-constexpr int NORM_A = 0, NORM_B = 1, NORM_R = 17, NORM_X = 33, NORM_WORDS = 64;
+constexpr int NORM_A = 0, NORM_B = 1, NORM_R = 17, NORM_X = 33, NORM_ABORT = 63, NORM_WORDS = 64;
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
@@ -572,10 +572,16 @@ int residual(hipStream_t s, const Arena& a, size_t nr, const double* A, size_t l
     return AL_OK;
 }
 
+// Reads the norm words back, and the workspace's abort flag into host[NORM_ABORT] (the device
+// word there is never written): a raised flag means a solve's block wait gave up.
 int read_norms(hipStream_t s, Workspace* ws, const Arena& a, u64* host) {
     AL_HIP(hipMemcpyAsync(host, a.norms, NORM_WORDS * sizeof(u64), hipMemcpyDeviceToHost, s));
+    AL_HIP(hipMemcpyAsync(host + NORM_ABORT, ws->abort_flag, sizeof(unsigned), hipMemcpyDeviceToHost, s));
     AL_HIP(hipStreamSynchronize(s));
-    (void)ws;
+    if ((unsigned)host[NORM_ABORT] != 0u) {
+        set_error("dgesv_mixed: a device-side wait timed out in a triangular solve");
+        return AL_ERR_DEVICE;
+    }
     return AL_OK;
 }
 
@@ -657,6 +663,8 @@ int gesv_mixed(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const double
     }
     if (!ws->hnorms) AL_HIP(hipHostMalloc((void**)&ws->hnorms, NORM_WORDS * sizeof(u64), hipHostMallocDefault));
     u64* hn = (u64*)ws->hnorms;
+    // a flag raised by an earlier call's timed-out wait must not fail this one's solves
+    AL_CHECK(fill_u32(s, ws->abort_flag, 1, 0u));
 
     // |A|_inf (also the padding scale)
     AL_CHECK(fill_u32(s, (unsigned*)a.norms, 2 * NORM_WORDS, 0u));
@@ -779,6 +787,10 @@ int gesv_mixed(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const double
                 }
                 stats.backward_error = worst;
                 stats.converged = worst <= opt.tol ? 1 : 0;
+                if (!stats.converged) {
+                    status = AL_CONVERGENCE;
+                    set_error("dgesv_mixed: the FP64 solve did not reach the requested backward error");
+                }
             }
         } else {
             status = AL_CONVERGENCE;

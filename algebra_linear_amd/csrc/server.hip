@@ -142,7 +142,10 @@ struct Worker {
             al_ir_stats st{};
             const int rc = gesv_mixed(s, ws, n, nr, A, n, B, n, X, n, AL_FACTOR_FP32, nullptr, &st,
                                       reuse ? &reuse->mf : nullptr, keep ? &keep->mf : nullptr);
-            if (rc != AL_OK && rc != AL_SINGULAR && rc != AL_CONVERGENCE) return rc;
+            if (rc != AL_OK && rc != AL_SINGULAR && rc != AL_CONVERGENCE) {
+                res.status = rc;  // e.g. a device-side timeout: no result, nothing kept
+                return AL_OK;
+            }
             res.status = rc;
             res.steps = st.fell_back ? -1 : st.iterations;
         } else {
@@ -157,17 +160,30 @@ struct Worker {
                 AL_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s));
                 AL_HIP(hipStreamSynchronize(s));
             }
+            unsigned aborted = 0;
             if (hinfo == 0 && nr) {
                 if (chol) {
                     AL_CHECK(trsm_f64(s, true, true, false, false, n, nr, 1.0, A, n, X, n));  // L y = b
                     AL_CHECK(trsm_f64(s, true, true, true, false, n, nr, 1.0, A, n, X, n));   // L^T x = y
                 } else {
                     AL_CHECK(al_dev_dgetrs(n, nr, A, n, ipiv, X, n, s));
+                    // a solve whose block waits gave up leaves NaN rows and raises the flag
+                    Workspace* ws;
+                    AL_CHECK(workspace_on(s, &ws));
+                    AL_HIP(hipMemcpyAsync(&aborted, ws->abort_flag, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+                    AL_HIP(hipStreamSynchronize(s));
                 }
             }
-            res.status = hinfo > 0 ? AL_SINGULAR : AL_OK;
+            // info < 0: a bounded wait of the factorization gave up, so neither the factors nor
+            // a solution exist; the request fails and nothing is kept
+            if (hinfo < 0 || aborted) {
+                set_error("server: device-side timeout in a cooperative wait");
+                res.status = AL_ERR_DEVICE;
+            } else {
+                res.status = hinfo > 0 ? AL_SINGULAR : AL_OK;
+            }
         }
-        if (nb && res.status != AL_SINGULAR) AL_HIP(hipMemcpyAsync(r.x, X, nb, hipMemcpyDeviceToHost, s));
+        if (nb && (res.status == AL_OK || res.status == AL_CONVERGENCE)) AL_HIP(hipMemcpyAsync(r.x, X, nb, hipMemcpyDeviceToHost, s));
         AL_HIP(hipStreamSynchronize(s));
         return AL_OK;
     }
@@ -309,6 +325,7 @@ struct Worker {
                         }
                     }
                     if (rc == AL_OK) rc = solve_one(r, out[0], reuse, keep);
+                    // only a factorization that completed (status AL_OK) may be reused
                     kept_ok = keep && rc == AL_OK && out[0].status == AL_OK;
                     if (keep && !kept_ok) (void)hipFree(kept.mem);
                 }

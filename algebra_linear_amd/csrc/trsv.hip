@@ -131,7 +131,13 @@ __global__ __launch_bounds__(256) void trsv_blocked(int n, int nrhs, const TA* _
             }
         }
     }
-    if (!ok) return;
+    if (!ok) {
+        // the chain was aborted: poison this block's rows so the partial solve cannot pass for a
+        // result (the host entry points also read the raised abort flag and report an error)
+        for (int idx = t; idx < nb * nrhs; idx += 256)
+            X[(size_t)(r0 + idx % nb) + (size_t)(idx / nb) * ldx] = __builtin_nan("");
+        return;
+    }
 
 #pragma unroll
     for (int k = 0; k < NR; ++k) red[part][row][k] = acc[k];
@@ -327,7 +333,11 @@ __global__ __launch_bounds__(256) void trsv128_f32(int n, int nrhs, const float*
             }
         }
     }
-    if (!ok) return;
+    if (!ok) {
+        for (int idx = t; idx < nb * nrhs; idx += 256)
+            X[(size_t)(r0 + idx % nb) + (size_t)(idx / nb) * ldx] = __builtin_nan("");
+        return;
+    }
 
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
@@ -574,18 +584,6 @@ __global__ __launch_bounds__(256) void diag_inv_k(int n, const float* __restrict
     for (int i = 0; i < 64; ++i) out[i] = z[i];
 }
 
-int cu_count() {
-    static int cus = 0;  // benign race: every thread computes the same value
-    if (cus == 0) {
-        int dev = 0, v = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
-            v = 256;
-        cus = v;
-    }
-    return cus;
-}
-
 template <bool LOWER, bool UNIT, int NR, bool INV>
 int launch128_t(hipStream_t s, unsigned nblk, int n, int nrhs, const float* A, size_t lda, double* X, size_t ldx,
                 unsigned* flags, unsigned gen, unsigned* abort_flag, long long timeout, const double* Dinv,
@@ -632,7 +630,7 @@ int launch_trsv(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const TA* A
         ws->trsv_gen = 0;
     }
     const unsigned gen = ++ws->trsv_gen;
-    const long long timeout = 200000000LL;  // 2 s at 100 MHz
+    const long long timeout = wait_limit_ticks();
     const int ticket = prof_begin(s);
     const unsigned g = (unsigned)nblk;
     if (nch > 1) {

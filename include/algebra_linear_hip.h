@@ -268,6 +268,19 @@ void al_prof_enable(int on);
 void al_prof_reset(void);
 int al_prof_query(const char* kernel_class, int* launches, double* total_ms, double* flops, double* bytes);
 
+/* ------------------------------------------------------------------------------------
+ * Test hooks (extension).  Bounded device-side waits -- the cooperative panel's column
+ * exchange and the one-launch triangular solves' block flags -- give up after a limit in ticks
+ * of the 100 MHz wall clock (default 2 s); al_debug_set_wait_limit changes it for the process
+ * (<= 0 restores the default), so a test can make them give up from the host and check that
+ * every entry point reports AL_ERR_DEVICE.  al_debug_coop_units_in_use returns the
+ * cooperative-panel workgroups currently reserved by unfinished FP64 factorizations: their sum
+ * is kept at or below the device's CU count, so concurrent factorizations on several streams
+ * cannot fill the device with panels that wait on each other.
+ * ------------------------------------------------------------------------------------ */
+void al_debug_set_wait_limit(long long ticks);
+int al_debug_coop_units_in_use(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -75,6 +75,18 @@ SIGNATURES = {
     "al_dpotrf": (_i, [_dp, _sz, _sz]),
     "al_dgesv_nrhs": (_i, [_dp, _sz, _dp, _sz, _sz, _sz, _sp]),
     "al_dgetrs_nrhs": (_i, [_dp, _sz, _sp, _sz, _dp, _sz, _sz, _sz]),
+    "al_dgemv": (_i, [_d, _dp, _sz, _dp, _sz, _d, _dp, _sz, _sz, _sz]),
+    "al_dgemv_trans": (_i, [_d, _dp, _sz, _dp, _sz, _d, _dp, _sz, _sz, _sz]),
+    "al_dlange": (_i, [ctypes.c_char, _dp, _sz, _sz, _sz, _dp]),
+    "al_dtrsv": (_i, [_dp, _sz, _dp, _sz, _sz, _i, _i]),
+    "al_dtrsv_transpose": (_i, [_dp, _sz, _dp, _sz, _sz, _i, _i]),
+    "al_dgetri": (_i, [_dp, _sz, _sz, _sp, _sz]),
+    "al_dsyr2k": (_i, [_i, _sz, _sz, _d, _dp, _sz, _dp, _sz, _d, _dp, _sz]),
+    "al_dev_dgemv": (_i, [_i, _sz, _sz, _d, _v, _sz, _v, _d, _v, _v]),
+    "al_dev_dlange": (_i, [ctypes.c_char, _sz, _sz, _v, _sz, _v, _v]),
+    "al_dev_dtrsv": (_i, [_i, _i, _i, _sz, _v, _sz, _v, _v]),
+    "al_dev_dgetri": (_i, [_sz, _v, _sz, _v, _v, _v]),
+    "al_dev_dsyr2k": (_i, [_i, _sz, _sz, _d, _v, _sz, _v, _sz, _d, _v, _sz, _v]),
     "al_dev_dgemm": (_i, [_i, _i, _sz, _sz, _sz, _d, _v, _sz, _v, _sz, _d, _v, _sz, _v]),
     "al_dev_dsyrk": (_i, [_i, _sz, _sz, _d, _v, _sz, _d, _v, _sz, _v]),
     "al_dev_dtrsm": (_i, [_i, _i, _i, _sz, _sz, _d, _v, _sz, _v, _sz, _v]),

@@ -12,7 +12,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from .errors import check_blas
+from .errors import BlasPanic, check_blas
 
 _dp = ctypes.POINTER(ctypes.c_double)
 
@@ -70,3 +70,39 @@ def dsyrk(alpha: float, a, beta: float, c: np.ndarray, n: int, k: int, upper: bo
     """One triangle of C := alpha*A*A^T + beta*C; ``a`` n x k (src/blas.rs:609)."""
     a, cc = _in(a), _out(c)
     check_blas(_lib.load().al_dsyrk(float(alpha), _p(a), a.size, float(beta), _p(cc), cc.size, n, k, int(upper)))
+
+
+def dgemv(alpha: float, a, x, beta: float, y: np.ndarray, m: int, n: int) -> None:
+    """y := alpha*A*x + beta*y; ``a`` m x n, ``x`` n, ``y`` m (src/blas.rs:102); bit for bit the crate's."""
+    a, x, yy = _in(a), _in(x), _out(y)
+    check_blas(_lib.load().al_dgemv(float(alpha), _p(a), a.size, _p(x), x.size, float(beta), _p(yy), yy.size, m, n))
+
+
+def dgemv_trans(alpha: float, a, x, beta: float, y: np.ndarray, m: int, n: int) -> None:
+    """y := alpha*A^T*x + beta*y; ``a`` m x n, ``x`` m, ``y`` n (src/blas.rs:144); bit for bit the crate's."""
+    a, x, yy = _in(a), _in(x), _out(y)
+    check_blas(_lib.load().al_dgemv_trans(float(alpha), _p(a), a.size, _p(x), x.size, float(beta), _p(yy), yy.size,
+                                          m, n))
+
+
+def dlange(norm: str, a, m: int, n: int) -> float:
+    """|A| for ``norm`` in 1/O, I, F/E, M (src/blas.rs:638); an unknown norm panics, as in the crate."""
+    a = _in(a)
+    out = ctypes.c_double(0.0)
+    st = _lib.load().al_dlange(str(norm).encode()[:1] or b"?", _p(a), a.size, m, n, ctypes.byref(out))
+    if st == _lib.AL_ERR_ARG and _lib.last_error().startswith("dlange: unrecognized norm"):
+        raise BlasPanic(_lib.last_error())
+    check_blas(st)
+    return out.value
+
+
+def dtrsv(a, x: np.ndarray, n: int, lower: bool, unit: bool) -> None:
+    """Solves A x = b in place, ``a`` n x n triangular (src/blas.rs:1355)."""
+    a, xx = _in(a), _out(x)
+    check_blas(_lib.load().al_dtrsv(_p(a), a.size, _p(xx), xx.size, n, int(lower), int(unit)))
+
+
+def dtrsv_transpose(a, x: np.ndarray, n: int, lower: bool, unit: bool) -> None:
+    """Solves A^T x = b in place (src/blas.rs:1366)."""
+    a, xx = _in(a), _out(x)
+    check_blas(_lib.load().al_dtrsv_transpose(_p(a), a.size, _p(xx), xx.size, n, int(lower), int(unit)))

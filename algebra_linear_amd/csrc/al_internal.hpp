@@ -255,6 +255,22 @@ size_t batch_lds_bytes();
 int batch_solve(hipStream_t s, size_t count, const al_batch_entry* d_entries, double* A_pool, double* B_pool,
                 float* W_pool, int* d_info, int* d_steps, unsigned* queue, bool mixed, int max_iter);
 
+// Level-2 and support routines (blas2.hip).
+// y := alpha op(A) x + beta y with the crate's summation order (bit for bit), A m x n.
+int gemv_f64(hipStream_t s, bool trans, size_t m, size_t n, double alpha, const double* A, size_t lda,
+             const double* x, double beta, double* y);
+// *out (device) := the norm of the m x n matrix A ('M', '1'/'O', 'I', 'F'/'E'); scratch holds
+// lange_scratch_doubles(m, n) doubles.
+size_t lange_scratch_doubles(size_t m, size_t n);
+int lange_f64(hipStream_t s, char norm, size_t m, size_t n, const double* A, size_t lda, double* out,
+              double* scratch);
+// In-place inverse from getrf's factors and pivots (device int32); *info = 1-based index of a
+// zero on U's diagonal (the factor is then left untouched), else 0.  Uses ws->perm and ws->wbuf.
+int getri_f64(hipStream_t s, Workspace* ws, size_t n, double* LU, size_t lda, const int* ipiv, int* info);
+// One triangle of C := alpha V W^T + alpha W V^T + beta C (V, W m x k).
+int syr2k_f64(hipStream_t s, bool upper, size_t m, size_t k, double alpha, const double* V, size_t lv,
+              const double* W, size_t lw, double beta, double* C, size_t lc);
+
 // Lower Cholesky in place; only the lower triangle is read or written.
 int potrf_f64(hipStream_t s, Workspace* ws, size_t n, double* A, size_t lda, int* info);
 

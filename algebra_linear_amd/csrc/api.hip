@@ -654,6 +654,182 @@ int al_dev_batch_solve(size_t count, const al_batch_entry* d_entries, double* A_
     return batch_solve(s, count, d_entries, A_pool, B_pool, W, d_info, d_steps, queue, mixed, 10);
 }
 
+// ------------------------------------------------------------------------------------------
+// support routines: dgemv, dgemv_trans, dlange, dtrsv, dtrsv_transpose, dgetri, dsyr2k
+// ------------------------------------------------------------------------------------------
+
+static int gemv_host(const char* nm, bool trans, double alpha, const double* a, size_t a_len, const double* x,
+                     size_t x_len, double beta, double* y, size_t y_len, size_t m, size_t n) {
+    // messages of src/blas.rs:103-105 and :145-147
+    const std::string name(nm);
+    if (!len_is(a_len, m, n)) return bad_shape((name + ": a must be m*n").c_str());
+    if (x_len != (trans ? m : n)) return bad_shape((name + (trans ? ": x must have m entries" : ": x must have n entries")).c_str());
+    if (y_len != (trans ? n : m)) return bad_shape((name + (trans ? ": y must have n entries" : ": y must have m entries")).c_str());
+    if (y_len == 0) return AL_OK;
+    hipStream_t s;
+    AL_CHECK(thread_stream(&s));
+    double *dA = nullptr, *dX = nullptr, *dY;
+    if (a_len) AL_CHECK(upload(0, a, a_len, &dA, s));
+    if (x_len) AL_CHECK(upload(1, x, x_len, &dX, s));
+    AL_CHECK(upload(2, y, y_len, &dY, s));
+    AL_CHECK(gemv_f64(s, trans, m, n, alpha, dA, m ? m : 1, dX, beta, dY));
+    AL_CHECK(download(y, dY, y_len, s));
+    return finish(s);
+}
+
+int al_dgemv(double alpha, const double* a, size_t a_len, const double* x, size_t x_len, double beta, double* y,
+             size_t y_len, size_t m, size_t n) {
+    return gemv_host("dgemv", false, alpha, a, a_len, x, x_len, beta, y, y_len, m, n);
+}
+
+int al_dgemv_trans(double alpha, const double* a, size_t a_len, const double* x, size_t x_len, double beta, double* y,
+                   size_t y_len, size_t m, size_t n) {
+    return gemv_host("dgemv_trans", true, alpha, a, a_len, x, x_len, beta, y, y_len, m, n);
+}
+
+int al_dlange(char norm, const double* a, size_t a_len, size_t m, size_t n, double* out) {
+    if (!len_is(a_len, m, n)) return bad_shape("dlange: a must be m*n");  // src/blas.rs:639
+    if (!out) {
+        set_error("dlange: null output");
+        return AL_ERR_ARG;
+    }
+    hipStream_t s;
+    AL_CHECK(thread_stream(&s));
+    Workspace* ws;
+    AL_CHECK(workspace_for(s, &ws, 1));
+    AL_CHECK(ws_reserve(ws, 0, lange_scratch_doubles(m, n) + 1));
+    double* dA = nullptr;
+    if (a_len) AL_CHECK(upload(0, a, a_len, &dA, s));
+    double* dOut = ws->wbuf + lange_scratch_doubles(m, n);
+    AL_CHECK(lange_f64(s, norm, m, n, dA, m ? m : 1, dOut, ws->wbuf));
+    AL_CHECK(download(out, dOut, 1, s));
+    return finish(s);
+}
+
+static int trsv_host(const char* nm, bool trans, const double* a, size_t a_len, double* x, size_t x_len, size_t n,
+                     int lower, int unit) {
+    // messages of src/blas.rs:1356-1357 and :1367-1368
+    const std::string name(nm);
+    if (!len_is(a_len, n, n)) return bad_shape((name + ": a must be n*n").c_str());
+    if (x_len != n) return bad_shape((name + ": x must be n").c_str());
+    if (n == 0) return AL_OK;
+    hipStream_t s;
+    AL_CHECK(thread_stream(&s));
+    Workspace* ws;
+    AL_CHECK(workspace_for(s, &ws, 1));
+    double *dA, *dX;
+    AL_CHECK(upload(0, a, a_len, &dA, s));
+    AL_CHECK(upload(1, x, x_len, &dX, s));
+    AL_CHECK(fill_u32(s, ws->abort_flag, 1, 0u));
+    if (trans)
+        AL_CHECK(trsm_f64(s, true, lower != 0, true, unit != 0, n, 1, 1.0, dA, n, dX, n));
+    else
+        AL_CHECK(trsv_solve<double>(s, ws, lower != 0, unit != 0, n, 1, dA, n, dX, n));
+    return download_solution(x, dX, x_len, s, ws);
+}
+
+int al_dtrsv(const double* a, size_t a_len, double* x, size_t x_len, size_t n, int lower, int unit) {
+    return trsv_host("dtrsv", false, a, a_len, x, x_len, n, lower, unit);
+}
+
+int al_dtrsv_transpose(const double* a, size_t a_len, double* x, size_t x_len, size_t n, int lower, int unit) {
+    return trsv_host("dtrsv_transpose", true, a, a_len, x, x_len, n, lower, unit);
+}
+
+int al_dgetri(double* lu, size_t lu_len, size_t n, const size_t* ipiv, size_t ipiv_len) {
+    // src/lapack/mod.rs:258-260
+    if (!len_is(lu_len, n, n) || ipiv_len != n) return bad_shape("dgetri: lu must be n*n and ipiv n");
+    if (n == 0) return AL_OK;
+    AL_CHECK(check_pivots(ipiv, n));
+    if (n > INT32_MAX) {
+        set_error("dgetri: order beyond 2^31");
+        return AL_ERR_ARG;
+    }
+    hipStream_t s;
+    AL_CHECK(thread_stream(&s));
+    Workspace* ws;
+    AL_CHECK(workspace_for(s, &ws, n));
+    std::vector<int> piv(n);
+    for (size_t i = 0; i < n; ++i) piv[i] = (int)ipiv[i];
+    double* dLU;
+    AL_CHECK(upload(0, lu, lu_len, &dLU, s));
+    AL_HIP(hipMemcpyAsync(ws->ipiv32, piv.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
+    AL_CHECK(getri_f64(s, ws, n, dLU, n, ws->ipiv32, ws->info));
+    int info = 0;
+    AL_HIP(hipMemcpyAsync(&info, ws->info, sizeof(int), hipMemcpyDeviceToHost, s));
+    AL_CHECK(finish(s));
+    if (info != 0) {
+        set_error("dgetri: U has a zero on its diagonal");
+        return AL_SINGULAR;  // the factor is left as it was, as the crate leaves it
+    }
+    AL_CHECK(download(lu, dLU, lu_len, s));
+    return finish(s);
+}
+
+int al_dsyr2k(int upper, size_t m, size_t k, double alpha, const double* v, size_t v_len, const double* w,
+              size_t w_len, double beta, double* c, size_t c_len) {
+    if (!len_is(v_len, m, k)) return bad_shape("dsyr2k: v must be m*k");
+    if (!len_is(w_len, m, k)) return bad_shape("dsyr2k: w must be m*k");
+    if (!len_is(c_len, m, m)) return bad_shape("dsyr2k: c must be m*m");
+    if (m == 0) return AL_OK;
+    hipStream_t s;
+    AL_CHECK(thread_stream(&s));
+    double *dV = nullptr, *dW = nullptr, *dC;
+    if (v_len) AL_CHECK(upload(0, v, v_len, &dV, s));
+    if (w_len) AL_CHECK(upload(1, w, w_len, &dW, s));
+    AL_CHECK(upload(2, c, c_len, &dC, s));  // the other triangle must come back unchanged
+    AL_CHECK(syr2k_f64(s, upper != 0, m, k, alpha, dV, m, dW, m, beta, dC, m));
+    AL_CHECK(download(c, dC, c_len, s));
+    return finish(s);
+}
+
+int al_dev_dgemv(int trans, size_t m, size_t n, double alpha, const double* A, size_t lda, const double* x, double beta,
+                 double* y, void* stream) {
+    if (lda < (m ? m : 1)) return bad_shape("dgemv: leading dimension too small");
+    hipStream_t s;
+    AL_CHECK(dev_stream(stream, &s));
+    return gemv_f64(s, trans != 0, m, n, alpha, A, lda, x, beta, y);
+}
+
+int al_dev_dlange(char norm, size_t m, size_t n, const double* A, size_t lda, double* d_out, void* stream) {
+    if (lda < (m ? m : 1)) return bad_shape("dlange: leading dimension too small");
+    hipStream_t s;
+    AL_CHECK(dev_stream(stream, &s));
+    Workspace* ws;
+    AL_CHECK(workspace_for(s, &ws, 1));
+    AL_CHECK(ws_reserve(ws, 0, lange_scratch_doubles(m, n)));
+    return lange_f64(s, norm, m, n, A, lda, d_out, ws->wbuf);
+}
+
+int al_dev_dtrsv(int lower, int trans, int unit, size_t n, const double* A, size_t lda, double* x, void* stream) {
+    if (lda < (n ? n : 1)) return bad_shape("dtrsv: leading dimension too small");
+    hipStream_t s;
+    AL_CHECK(dev_stream(stream, &s));
+    if (n == 0) return AL_OK;
+    Workspace* ws;
+    AL_CHECK(workspace_for(s, &ws, 1));
+    if (trans) return trsm_f64(s, true, lower != 0, true, unit != 0, n, 1, 1.0, A, lda, x, n);
+    AL_CHECK(fill_u32(s, ws->abort_flag, 1, 0u));
+    return trsv_solve<double>(s, ws, lower != 0, unit != 0, n, 1, A, lda, x, n);
+}
+
+int al_dev_dgetri(size_t n, double* LU, size_t lda, const int32_t* d_ipiv, int32_t* d_info, void* stream) {
+    if (lda < (n ? n : 1)) return bad_shape("dgetri: leading dimension too small");
+    hipStream_t s;
+    AL_CHECK(dev_stream(stream, &s));
+    Workspace* ws;
+    AL_CHECK(workspace_for(s, &ws, 1));
+    return getri_f64(s, ws, n, LU, lda, d_ipiv, d_info);
+}
+
+int al_dev_dsyr2k(int upper, size_t m, size_t k, double alpha, const double* V, size_t lv, const double* W, size_t lw,
+                  double beta, double* C, size_t ldc, void* stream) {
+    if (ldc < (m ? m : 1) || lv < (m ? m : 1) || lw < (m ? m : 1)) return bad_shape("dsyr2k: leading dimension too small");
+    hipStream_t s;
+    AL_CHECK(dev_stream(stream, &s));
+    return syr2k_f64(s, upper != 0, m, k, alpha, V, lv, W, lw, beta, C, ldc);
+}
+
 void al_debug_set_wait_limit(long long ticks) { set_wait_limit_ticks(ticks); }
 
 int al_debug_coop_units_in_use(void) {

@@ -12,7 +12,7 @@ import numpy as np
 
 from . import _lib
 from .blas import _in, _out, _p
-from .errors import BAD_SHAPE, LapackError, check_lapack
+from .errors import BAD_SHAPE, BlasPanic, LapackError, check_lapack
 
 _sp = ctypes.POINTER(ctypes.c_size_t)
 
@@ -66,7 +66,36 @@ def dpotrf(a: np.ndarray, n: int) -> None:
     check_lapack(_lib.load().al_dpotrf(_p(aa), aa.size, n))
 
 
-__all__ = ["dgetrf", "dgetrf_rect", "dgetrs", "dgesv", "dgesv_nrhs", "dpotrf", "LapackError", "BAD_SHAPE"]
+def dgetrs_nrhs(lu, ipiv, b: np.ndarray, n: int, nrhs: int) -> None:
+    """Extension: solve with a ``dgetrf`` factorization for B n x nrhs column-major, in place."""
+    lu = _in(lu)
+    bb = _out(b)
+    pv = np.ascontiguousarray(np.asarray(ipiv, dtype=np.uintp))
+    check_lapack(_lib.load().al_dgetrs_nrhs(_p(lu), lu.size, pv.ctypes.data_as(_sp), pv.size, _p(bb), bb.size, n,
+                                            nrhs))
+
+
+def dgetri(lu: np.ndarray, n: int, ipiv) -> None:
+    """In-place inverse from a ``dgetrf`` factorization (src/lapack/mod.rs:249); raises ``Singular``
+    (``lu`` unchanged) when U has a zero on its diagonal."""
+    aa = _out(lu)
+    pv = np.ascontiguousarray(np.asarray(ipiv, dtype=np.uintp))
+    check_lapack(_lib.load().al_dgetri(_p(aa), aa.size, n, pv.ctypes.data_as(_sp), pv.size))
+
+
+def dsyr2k(upper: bool, m: int, k: int, alpha: float, v, w, beta: float, c: np.ndarray) -> None:
+    """dsytrd's symmetric rank-2k update (src/lapack/sym.rs:582): one triangle of
+    C := alpha*V*W^T + alpha*W*V^T + beta*C with V, W m x k and C m x m, packed column-major."""
+    vv, ww, cc = _in(v), _in(w), _out(c)
+    st = _lib.load().al_dsyr2k(int(upper), m, k, float(alpha), _p(vv), vv.size, _p(ww), ww.size, float(beta), _p(cc),
+                               cc.size)
+    if st == _lib.AL_BAD_SHAPE:
+        raise BlasPanic(_lib.last_error())
+    check_lapack(st)
+
+
+__all__ = ["dgetrf", "dgetrf_rect", "dgetrs", "dgetrs_nrhs", "dgetri", "dgesv", "dgesv_nrhs", "dpotrf", "dsyr2k",
+           "LapackError", "BAD_SHAPE"]
 
 
 FACTORS = {"fp64": _lib.AL_FACTOR_FP64, "fp32": _lib.AL_FACTOR_FP32, "bf16": _lib.AL_FACTOR_BF16}

@@ -100,6 +100,31 @@ int al_dgesv_nrhs(double* a, size_t a_len, double* b, size_t b_len, size_t n, si
 int al_dgetrs_nrhs(const double* lu, size_t lu_len, const size_t* ipiv, size_t ipiv_len, double* b, size_t b_len,
                    size_t n, size_t nrhs);
 
+/* Support routines of the crate around the solve path. */
+/* blas::dgemv, src/blas.rs:102-130: y := alpha*A*x + beta*y; a m*n, x n, y m.  The crate's
+ * summation order and unfused arithmetic, so the result is the crate's bit for bit. */
+int al_dgemv(double alpha, const double* a, size_t a_len, const double* x, size_t x_len, double beta, double* y,
+             size_t y_len, size_t m, size_t n);
+/* blas::dgemv_trans, src/blas.rs:145-158: y := alpha*A^T*x + beta*y; a m*n, x m, y n; bit for bit. */
+int al_dgemv_trans(double alpha, const double* a, size_t a_len, const double* x, size_t x_len, double beta, double* y,
+                   size_t y_len, size_t m, size_t n);
+/* blas::dlange, src/blas.rs:638-657: *out := 'M' max |a_ij|, '1'/'O' max column sum, 'I' max row sum
+ * (bit for bit), 'F'/'E' Frobenius (the crate's scaled dnrm2, within m*n*eps relative).  An
+ * unrecognized norm is AL_ERR_ARG with the crate's panic text. */
+int al_dlange(char norm, const double* a, size_t a_len, size_t m, size_t n, double* out);
+/* blas::dtrsv / dtrsv_transpose, src/blas.rs:1355-1370: solves A x = b (A^T x = b) in place for a
+ * triangular a (n*n, only the named triangle read; unit: the diagonal is not read). */
+int al_dtrsv(const double* a, size_t a_len, double* x, size_t x_len, size_t n, int lower, int unit);
+int al_dtrsv_transpose(const double* a, size_t a_len, double* x, size_t x_len, size_t n, int lower, int unit);
+/* lapack::dgetri, src/lapack/mod.rs:249-298: lu (dgetrf's factors, n*n) := A^{-1}; AL_SINGULAR
+ * when U has a zero on its diagonal, lu then unchanged. */
+int al_dgetri(double* lu, size_t lu_len, size_t n, const size_t* ipiv, size_t ipiv_len);
+/* dsytrd's dsyr2k (src/lapack/sym.rs:582-668, crate-private): one triangle of
+ * C := alpha*V*W^T + alpha*W*V^T + beta*C; v, w m*k, c m*m; the other triangle untouched.  With
+ * k = 0 only alpha = 0 touches C (beta*C on the triangle), as in the crate. */
+int al_dsyr2k(int upper, size_t m, size_t k, double alpha, const double* v, size_t v_len, const double* w,
+              size_t w_len, double beta, double* c, size_t c_len);
+
 /* ------------------------------------------------------------------------------------
  * Device-pointer API (extension): the same operations on HBM-resident operands with
  * explicit leading dimensions, enqueued on `stream` (a hipStream_t; NULL = HIP's null
@@ -121,6 +146,24 @@ int al_dev_dpotrf(size_t n, double* A, size_t lda, int32_t* d_info, void* stream
 /* Row interchanges d_ipiv[k0..k1) applied in order to the ncols columns of A (the crate's
  * full-row swap, src/lapack/mod.rs:141-146; LAPACK's dlaswp with incx = 1). */
 int al_dev_dlaswp(size_t ncols, double* A, size_t lda, size_t k0, size_t k1, const int32_t* d_ipiv, void* stream);
+
+/* Support routines on the device (the host-slice forms above; vectors have unit stride):
+ *   al_dev_dgemv:  y := alpha op(A) x + beta y (trans: op(A) = A^T), bit for bit the crate's;
+ *   al_dev_dlange: *d_out (device) := the norm of the m x n matrix A;
+ *   al_dev_dtrsv:  op(T) x = b in place (trans selects T^T);
+ *   al_dev_dgetri: LU := A^{-1} from al_dev_dgetrf's factors and pivots; *d_info = 1-based index
+ *                  of a zero on U's diagonal (LU then untouched), else 0;
+ *   al_dev_dsyr2k: one triangle of C := alpha V W^T + alpha W V^T + beta C (V, W m x k; V and W
+ *                  must not overlap C).
+ * A one-launch triangular solve (al_dev_dtrsv, al_dev_dgetrs) whose bounded block waits give up
+ * leaves NaN in the rows it could not solve. */
+int al_dev_dgemv(int trans, size_t m, size_t n, double alpha, const double* A, size_t lda, const double* x, double beta,
+                 double* y, void* stream);
+int al_dev_dlange(char norm, size_t m, size_t n, const double* A, size_t lda, double* d_out, void* stream);
+int al_dev_dtrsv(int lower, int trans, int unit, size_t n, const double* A, size_t lda, double* x, void* stream);
+int al_dev_dgetri(size_t n, double* LU, size_t lda, const int32_t* d_ipiv, int32_t* d_info, void* stream);
+int al_dev_dsyr2k(int upper, size_t m, size_t k, double alpha, const double* V, size_t lv, const double* W, size_t lw,
+                  double beta, double* C, size_t ldc, void* stream);
 
 /* Low-precision GEMMs (extension; the crate's GEMM is FP64 only, src/blas.rs:201-257), the
  * building blocks of the mixed-precision factorization, column-major FP32 on the device:

@@ -57,6 +57,7 @@ def _declare(L: ctypes.CDLL) -> None:
         "or_gemm": (i, [i, i, d, _c_double_p, _c_double_p, d, _c_double_p, sz, sz, sz]),
         "or_dsyrk": (i, [d, _c_double_p, d, _c_double_p, sz, sz, i]),
         "or_dsyrk_rank": (i, [d, _c_double_p, d, _c_double_p, sz, sz, i, i]),
+        "or_dsyr2k": (None, [i, sz, sz, d, _c_double_p, sz, _c_double_p, sz, d, _c_double_p, sz]),
         "or_dtrsv": (i, [_c_double_p, _c_double_p, sz, i, i]),
         "or_dtrsv_transpose": (i, [_c_double_p, _c_double_p, sz, i, i]),
         "or_dtrsm": (i, [d, _c_double_p, _c_double_p, sz, sz, i, i, i]),
@@ -170,6 +171,13 @@ def dsyrk(alpha, a, beta, c, n, k, upper) -> np.ndarray:
 def dsyrk_rank(alpha, a, beta, c, n, k, upper, transposed) -> np.ndarray:
     a, c = _f64(a), _f64(c).copy()
     lib().or_dsyrk_rank(alpha, _dp(a), beta, _dp(c), n, k, int(upper), int(transposed))
+    return c
+
+
+def dsyr2k(upper, m, k, alpha, v, w, beta, c) -> np.ndarray:
+    """dsytrd's dsyr2k (src/lapack/sym.rs:582-668) on packed V, W (m x k) and C (m x m)."""
+    v, w, c = _f64(v), _f64(w), _f64(c).copy()
+    lib().or_dsyr2k(int(upper), m, k, alpha, _dp(v), max(m, 1), _dp(w), max(m, 1), beta, _dp(c), max(m, 1))
     return c
 
 

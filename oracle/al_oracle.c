@@ -289,6 +289,51 @@ int or_dsyrk_rank(double alpha, const double* a, double beta, double* c, size_t 
     return 0;
 }
 
+/* src/lapack/sym.rs:582-668: dsytrd's dsyr2k, C := alpha V W^T + alpha W V^T + beta C on one
+ * triangle (V, W m x k with leading dimensions lv, lw; C m x m with lc).  The crate gathers
+ * compact copies of V, W and their transposes, forms the full product t with two plain dgemm
+ * calls (t = alpha V W^T, then t = alpha W V^T + t), and folds t into the named triangle
+ * (beta == 0 writes t).  m == 0 or k == 0: only alpha == 0 scales the triangle by beta. */
+void or_dsyr2k(int upper, size_t m, size_t k, double alpha, const double* v, size_t lv, const double* w,
+               size_t lw, double beta, double* c, size_t lc) {
+    if (m == 0 || k == 0) {
+        if (alpha == 0.0 && beta != 1.0) {
+            for (size_t j = 0; j < m; j++) {
+                size_t i0 = upper ? 0 : j, i1 = upper ? j + 1 : m;
+                for (size_t i = i0; i < i1; i++) c[i + j * lc] = (beta == 0.0) ? 0.0 : c[i + j * lc] * beta;
+            }
+        }
+        return;
+    }
+    double* vc = calloc(m * k, sizeof(double));
+    double* wc = calloc(m * k, sizeof(double));
+    double* vt = calloc(k * m, sizeof(double));
+    double* wt = calloc(k * m, sizeof(double));
+    double* t = calloc(m * m, sizeof(double));
+    for (size_t l = 0; l < k; l++)
+        for (size_t i = 0; i < m; i++) {
+            double x = v[i + l * lv], y = w[i + l * lw];
+            vc[i + l * m] = x;
+            wc[i + l * m] = y;
+            vt[l + i * k] = x;
+            wt[l + i * k] = y;
+        }
+    or_gemm(0, 0, alpha, vc, wt, 0.0, t, m, m, k);
+    or_gemm(0, 0, alpha, wc, vt, 1.0, t, m, m, k);
+    for (size_t j = 0; j < m; j++) {
+        size_t i0 = upper ? 0 : j, i1 = upper ? j + 1 : m;
+        for (size_t i = i0; i < i1; i++) {
+            double tv = t[i + j * m];
+            c[i + j * lc] = (beta == 0.0) ? tv : beta * c[i + j * lc] + tv;
+        }
+    }
+    free(vc);
+    free(wc);
+    free(vt);
+    free(wt);
+    free(t);
+}
+
 /* ------------------------------------------------------------------------- */
 /* triangular solves                                                         */
 /* ------------------------------------------------------------------------- */

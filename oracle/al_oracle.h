@@ -39,6 +39,11 @@ int or_dsyrk(double alpha, const double* a, double beta, double* c, size_t n, si
 int or_dsyrk_rank(double alpha, const double* a, double beta, double* c, size_t n, size_t k, int upper,
                   int transposed);
 
+/* src/lapack/sym.rs:582-668 (dsytrd's private dsyr2k): one triangle of
+ * C := alpha V W^T + alpha W V^T + beta C, V and W m x k. */
+void or_dsyr2k(int upper, size_t m, size_t k, double alpha, const double* v, size_t lv, const double* w,
+               size_t lw, double beta, double* c, size_t lc);
+
 /* ---- triangular solves (src/blas.rs:1344-1438) ---- */
 int or_dtrsv(const double* a, double* x, size_t n, int lower, int unit);
 int or_dtrsv_transpose(const double* a, double* x, size_t n, int lower, int unit);

@@ -527,3 +527,42 @@ def test_dgetrs_applies_the_pivots_and_solves():
     assert st == 0 and list(ipiv) != [0, 1, 2, 3]
     b = A @ x
     assert np.allclose(O.dgetrs(lu, ipiv, b), x, atol=1e-14)
+
+
+# ------------------------------------------------------------------------------------------
+# dsytrd's dsyr2k (src/lapack/sym.rs:582-668)
+# ------------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("upper", [True, False])
+@pytest.mark.parametrize("m,k", [(1, 1), (5, 3), (17, 9), (40, 64)])
+def test_dsyr2k_matches_the_definition_on_one_triangle(upper, m, k):
+    rng = np.random.default_rng(m * 100 + k)
+    V, W, C = rng.standard_normal((m, k)), rng.standard_normal((m, k)), rng.standard_normal((m, m))
+    tri = np.triu(np.ones((m, m), bool)) if upper else np.tril(np.ones((m, m), bool))
+    for alpha, beta in ((-1.0, 1.0), (0.75, -2.0), (1.5, 0.0)):
+        got = mat(O.dsyr2k(upper, m, k, alpha, cm(V), cm(W), beta, cm(C)), m, m)
+        want = alpha * (V @ W.T) + alpha * (W @ V.T) + beta * C
+        bound = 4 * k * np.finfo(float).eps * (abs(alpha) * 2 * np.abs(V) @ np.abs(W).T + abs(beta) * np.abs(C))
+        assert np.all(np.abs(got - want)[tri] <= bound[tri] + 1e-300)
+        assert np.array_equal(got[~tri], C[~tri])   # the other triangle is untouched
+    # a plausible mistake, 2 alpha V W^T (one product counted twice), breaks the definition
+    wrong = 2 * (V @ W.T)
+    assert m == 1 or not np.allclose(wrong[tri], (V @ W.T + W @ V.T)[tri])
+
+
+def test_dsyr2k_beta_zero_and_the_empty_cases():
+    m, k = 4, 2
+    rng = np.random.default_rng(1)
+    V, W = rng.standard_normal((m, k)), rng.standard_normal((m, k))
+    C = np.full((m, m), np.nan)
+    got = mat(O.dsyr2k(False, m, k, 1.0, cm(V), cm(W), 0.0, cm(C)), m, m)
+    tri = np.tril(np.ones((m, m), bool))
+    assert np.all(np.isfinite(got[tri])) and np.all(np.isnan(got[~tri]))   # beta = 0 does not read C
+    C = rng.standard_normal((m, m))
+    # k = 0: only alpha = 0 touches C (beta scales the triangle, beta = 0 clears it)
+    assert np.array_equal(O.dsyr2k(True, m, 0, 1.0, np.zeros(0), np.zeros(0), 3.0, cm(C)), cm(C))
+    got = mat(O.dsyr2k(True, m, 0, 0.0, np.zeros(0), np.zeros(0), 3.0, cm(C)), m, m)
+    up = np.triu(np.ones((m, m), bool))
+    assert np.array_equal(got[up], 3.0 * C[up]) and np.array_equal(got[~up], C[~up])
+    got = mat(O.dsyr2k(False, m, 0, 0.0, np.zeros(0), np.zeros(0), 0.0, cm(C)), m, m)
+    assert np.all(got[tri] == 0.0) and np.array_equal(got[~tri], C[~tri])

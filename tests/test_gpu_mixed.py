@@ -110,12 +110,15 @@ def test_multiple_right_hand_sides(nrhs):
     assert st["fell_back"] == 0 and st["converged"] == 1
     for j in range(nrhs):
         assert O.backward_error(a, x[j * n:(j + 1) * n], b[j * n:(j + 1) * n], n) <= TOL
-    # each column is the single-right-hand-side solution, bit for bit (the factor and the
-    # refinement arithmetic do not depend on the other columns; iteration counts can differ,
-    # so compare at the tolerance only when they do)
+    # each column agrees with its single-right-hand-side solve: the factor and the refinement
+    # arithmetic do not depend on the other columns, but a column solved beside others may take
+    # more refinement steps (the group stops when all its columns are done), so the two agree
+    # to the solutions' accuracy (forward error <= cond(A) * backward error), not bit for bit
     for j in (0, nrhs - 1):
         xj, stj = solve(a, b[j * n:(j + 1) * n].copy(), n, 1, "fp32")
         assert O.backward_error(a, xj, b[j * n:(j + 1) * n], n) <= TOL
+        col = x[j * n:(j + 1) * n]
+        assert np.abs(xj - col).max() <= 1e-10 * np.abs(col).max()
 
 
 def test_fp64_factor_is_the_plain_pivoted_solve():

@@ -228,6 +228,12 @@ int getrs_f64(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const double*
 // nearest even: At[r][p] = L[r + p*ld] (L m x k) and Bt[c][p] = U[p + c*ld] (U k x n).
 int bf16_rows(hipStream_t s, size_t m, size_t k, size_t kp, const float* L, size_t ld, __bf16* At);
 int bf16_cols(hipStream_t s, size_t k, size_t kp, size_t n, const float* U, size_t ld, __bf16* Bt);
+// Split staging of an FP32 product for three bf16 products: each value x becomes hi = bf16(x)
+// and lo = bf16(x - hi); At rows are [Lhi | Lhi | Llo] and Bt rows [Uhi | Ulo | Uhi] (3k
+// entries each), so gemm_bf16 with depth 3k gives Lhi Uhi + Lhi Ulo + Llo Uhi, the product to
+// about 2^-16 relative instead of bf16's 2^-8.  k % 64 == 0 keeps the depth a multiple of 64.
+int bf16x3_rows(hipStream_t s, size_t m, size_t k, const float* L, size_t ld, __bf16* At);
+int bf16x3_cols(hipStream_t s, size_t k, size_t n, const float* U, size_t ld, __bf16* Bt);
 // C (m x n, FP32) := alpha sum_p At[i][p] Bt[j][p] + beta C, bf16 products accumulated in
 // FP32 on the matrix cores; k % 64 == 0.
 int gemm_bf16(hipStream_t s, size_t m, size_t n, size_t k, float alpha, const __bf16* At, const __bf16* Bt, float beta,

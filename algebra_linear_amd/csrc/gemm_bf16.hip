@@ -159,7 +159,73 @@ __global__ __launch_bounds__(256) void to_bf16_cols_k(int k, int kp, int n, cons
     Bt[idx] = p < k ? (__bf16)U[(size_t)p + (size_t)c * ld] : (__bf16)0.f;
 }
 
+// Two-term bf16 split of an FP32 value: hi = bf16(x), lo = bf16(x - hi) (x - hi is exact in
+// FP32), so hi + lo carries x to about 2^-17 relative.
+__device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
+    hi = (__bf16)x;
+    lo = (__bf16)(x - (float)hi);
+}
+
+// At[r] = [hi(L[r, 0..k)) | hi(L[r, 0..k)) | lo(L[r, 0..k))], rows of 3k entries: the left
+// operand of the split product sum_p At[r][p] Bt[c][p] = Lhi Uhi + Lhi Ulo + Llo Uhi.
+__global__ __launch_bounds__(256) void to_bf16x3_rows_k(int m, int k, const float* __restrict__ L, size_t ld,
+                                                         __bf16* __restrict__ At) {
+    __shared__ float tile[64][65];
+    const int r0 = blockIdx.x * 64, p0 = blockIdx.y * 64;
+    const int t = threadIdx.x;
+    const size_t row_len = 3 * (size_t)k;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int idx = t + 256 * q;
+        const int r = idx & 63, p = idx >> 6;
+        tile[p][r] = (r0 + r < m && p0 + p < k) ? L[(size_t)(r0 + r) + (size_t)(p0 + p) * ld] : 0.f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int idx = t + 256 * q;
+        const int p = idx & 63, r = idx >> 6;
+        if (r0 + r < m && p0 + p < k) {
+            __bf16 hi, lo;
+            split_bf16(tile[p][r], hi, lo);
+            __bf16* row = At + (size_t)(r0 + r) * row_len + p0 + p;
+            row[0] = hi;
+            row[k] = hi;
+            row[2 * (size_t)k] = lo;
+        }
+    }
+}
+
+// Bt[c] = [hi(U[0..k, c]) | lo(U[0..k, c]) | hi(U[0..k, c])], rows of 3k entries.
+__global__ __launch_bounds__(256) void to_bf16x3_cols_k(int k, int n, const float* __restrict__ U, size_t ld,
+                                                         __bf16* __restrict__ Bt) {
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (size_t)k * n) return;
+    const int p = (int)(idx % k), c = (int)(idx / k);
+    __bf16 hi, lo;
+    split_bf16(U[(size_t)p + (size_t)c * ld], hi, lo);
+    __bf16* row = Bt + (size_t)c * 3 * (size_t)k + p;
+    row[0] = hi;
+    row[k] = lo;
+    row[2 * (size_t)k] = hi;
+}
+
 }  // namespace
+
+int bf16x3_rows(hipStream_t s, size_t m, size_t k, const float* L, size_t ld, __bf16* At) {
+    if (m == 0 || k == 0) return AL_OK;
+    hipLaunchKernelGGL(to_bf16x3_rows_k, dim3(ceil_div(m, 64), ceil_div(k, 64)), dim3(256), 0, s, (int)m, (int)k, L,
+                       ld, At);
+    AL_HIP(hipGetLastError());
+    return AL_OK;
+}
+
+int bf16x3_cols(hipStream_t s, size_t k, size_t n, const float* U, size_t ld, __bf16* Bt) {
+    if (n == 0 || k == 0) return AL_OK;
+    hipLaunchKernelGGL(to_bf16x3_cols_k, dim3(ceil_div(k * n, 256)), dim3(256), 0, s, (int)k, (int)n, U, ld, Bt);
+    AL_HIP(hipGetLastError());
+    return AL_OK;
+}
 
 int bf16_rows(hipStream_t s, size_t m, size_t k, size_t kp, const float* L, size_t ld, __bf16* At) {
     if (m == 0 || kp == 0) return AL_OK;

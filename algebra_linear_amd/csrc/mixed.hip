@@ -367,9 +367,9 @@ int make_arena(hipStream_t s, Workspace* ws, size_t n, size_t np, size_t nr, siz
     const size_t oI = off;
     off += 256;
     const size_t oA = off;
-    if (bf16) off += align256(np * nbo * 2);
+    if (bf16) off += align256(np * 3 * nbo * 2);  // split (hi, hi, lo) rows of the outer block's L21
     const size_t oB = off;
-    if (bf16) off += align256(np * nbo * 2);
+    if (bf16) off += align256(np * 3 * nbo * 2);  // split (hi, lo, hi) columns of its U12
     const size_t oT = off;
     if (lowp) off += align256(tslu_scratch_bytes(np));
     const size_t oS = off;
@@ -419,8 +419,12 @@ int factor_panel(hipStream_t q, Workspace* ws, Arena& a, int K, int KE, int* lis
     return AL_OK;
 }
 
-// A22[:, c0:c1) -= L21 U12[:, c0:c1) for the outer panel [K, KE): FP32 matrix cores, or bf16
-// operands with FP32 accumulation (L21 staged by the call with c0 == KE, U12 per call).
+// A22[:, c0:c1) -= L21 U12[:, c0:c1) for the outer panel [K, KE): FP32 matrix cores, or the
+// bf16 matrix cores with FP32 accumulation (L21 staged by the call with c0 == KE, U12 per
+// call).  The bf16 form splits each FP32 operand into hi + lo bf16 parts and forms
+// Lhi Uhi + Lhi Ulo + Llo Uhi as one product of depth 3 kb: with single bf16 operands
+// (u = 2^-8) the factors are too coarse for FP64 refinement to converge at these orders, while
+// the split keeps about 16 bits of every product.
 int trailing_update(hipStream_t s, Arena& a, bool bf16, int K, int KE, int c0, int c1) {
     const int np = (int)a.np;
     const size_t ld = a.np;
@@ -429,10 +433,10 @@ int trailing_update(hipStream_t s, Arena& a, bool bf16, int K, int KE, int c0, i
     auto at = [&](int i, int j) { return F + (size_t)i + (size_t)j * ld; };
     if (c1 <= c0) return AL_OK;
     if (bf16) {
-        if (c0 == KE) AL_CHECK(bf16_rows(s, m2, kb, kb, at(KE, K), ld, a.At));
-        __bf16* Bt = a.Bt + (size_t)(c0 - KE) * kb;
-        AL_CHECK(bf16_cols(s, kb, kb, c1 - c0, at(K, c0), ld, Bt));
-        return gemm_bf16(s, m2, c1 - c0, kb, -1.f, a.At, Bt, 1.f, at(KE, c0), ld);
+        if (c0 == KE) AL_CHECK(bf16x3_rows(s, m2, kb, at(KE, K), ld, a.At));
+        __bf16* Bt = a.Bt + (size_t)(c0 - KE) * 3 * kb;
+        AL_CHECK(bf16x3_cols(s, kb, c1 - c0, at(K, c0), ld, Bt));
+        return gemm_bf16(s, m2, c1 - c0, 3 * (size_t)kb, -1.f, a.At, Bt, 1.f, at(KE, c0), ld);
     }
     return gemm_f32(s, false, false, m2, c1 - c0, kb, -1.f, at(KE, K), ld, at(K, c0), ld, 1.f, at(KE, c0), ld);
 }

@@ -217,6 +217,22 @@ def test_mixed_fp32_at_config_sizes(n, kappa, nrhs):
         assert O.backward_error(a, x[k * n:(k + 1) * n], b[k * n:(k + 1) * n], n) <= 1e-14
 
 
+@pytest.mark.parametrize("n,kappa,nrhs", [(4096, 1e2, 1), (4096, 1e2, 16), (16384, 1e2, 1)])
+def test_mixed_bf16_at_config_sizes(n, kappa, nrhs):
+    # the bf16 form (split-bf16 trailing updates on the bf16 matrix cores) refines to FP64
+    # accuracy without the fallback at kappa 1e2
+    At = config3(n, kappa)
+    st, stats, b, x = solve_mixed(At, n, nrhs, _lib.AL_FACTOR_BF16, 77 + nrhs)
+    assert st == 0, (st, _lib.last_error())
+    a = At.cpu().numpy().ravel()
+    del At
+    print(f"bf16 n={n} kappa={kappa:g} nrhs={nrhs}: steps {stats.iterations}, fell_back {stats.fell_back}, "
+          f"berr {stats.backward_error:.2e}", flush=True)
+    assert stats.fell_back == 0 and stats.converged == 1
+    for k in range(nrhs):
+        assert O.backward_error(a, x[k * n:(k + 1) * n], b[k * n:(k + 1) * n], n) <= 1e-14
+
+
 # ------------------------------------------------------------------------------------------
 # batched: the full 20,000-system launch
 # ------------------------------------------------------------------------------------------

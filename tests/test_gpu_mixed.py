@@ -90,14 +90,15 @@ def test_refinement_failure_without_fallback_raises_convergence():
     assert e.value.kind == "Convergence"
 
 
-@pytest.mark.parametrize("n,kappa", [(256, 1e2), (600, 10.0), (1100, 10.0)])
+@pytest.mark.parametrize("n,kappa", [(256, 1e2), (600, 10.0), (1100, 1e2), (1500, 1e3)])
 def test_bf16_factor_reaches_fp64_accuracy(n, kappa):
     # n = 1100 pads to 1152: two 512-wide outer blocks plus a 128-wide remainder, so the
-    # bf16 trailing update runs on full and edge tiles
+    # bf16 trailing update runs on full and edge tiles.  The result must come from the bf16
+    # factorization and FP64 refinement, not from the FP64 fallback.
     a = conditioned(n, kappa)
     b = normal(n, [3, 5, 7, 9])
     x, st = solve(a, b, n, 1, "bf16")
-    assert st["converged"] == 1, st
+    assert st["converged"] == 1 and st["fell_back"] == 0, st
     assert O.backward_error(a, x, b, n) <= TOL
 
 

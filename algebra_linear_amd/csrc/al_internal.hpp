@@ -256,10 +256,15 @@ int gesv_mixed(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const double
                size_t ldb, double* X, size_t ldx, int factor, const al_ir_options* opt, al_ir_stats* stats,
                const MixedFactor* reuse = nullptr, MixedFactor* keep = nullptr);
 
-// Persistent batched solver (orders <= 512): see batch.hip.  queue: one device word.
-size_t batch_lds_bytes();
-int batch_solve(hipStream_t s, size_t count, const al_batch_entry* d_entries, double* A_pool, double* B_pool,
-                float* W_pool, int* d_info, int* d_steps, unsigned* queue, bool mixed, int max_iter);
+// Batched solver (orders <= 512, size-bucketed persistent kernels): see batch.hip.  scratch:
+// batch_scratch_bytes(count) device bytes; W_pool (mixed mode): an FP32 pool shaped like A_pool.
+// Uses the workspace's side stream and events.
+size_t batch_scratch_bytes(size_t count);
+int batch_solve(hipStream_t s, Workspace* ws, size_t count, const al_batch_entry* d_entries, double* A_pool,
+                double* B_pool, float* W_pool, int* d_info, int* d_steps, void* scratch, bool mixed, int max_iter);
+// The per-bucket launch configuration (bucket 0: n <= 64, 1: <= 128, 2: <= 256, 3: <= 512).
+int batch_tune_set(int bucket, int kb, int wgs_per_cu);
+int batch_tune_get(int bucket, int* kb, int* wgs_per_cu);
 
 // Level-2 and support routines (blas2.hip).
 // y := alpha op(A) x + beta y with the crate's summation order (bit for bit), A m x n.

@@ -641,7 +641,8 @@ int al_dev_batch_solve(size_t count, const al_batch_entry* d_entries, double* A_
     Workspace* ws;
     AL_CHECK(workspace_for(s, &ws, 1));
     const bool mixed = factor == AL_FACTOR_FP32;
-    const size_t need = 256 + (mixed ? a_pool_len * sizeof(float) : 0);
+    const size_t sbytes = (batch_scratch_bytes(count) + 255) / 256 * 256;
+    const size_t need = sbytes + (mixed ? a_pool_len * sizeof(float) : 0);
     if (ws->bbuf_bytes < need) {
         if (ws->bbuf) AL_HIP(hipFree(ws->bbuf));
         ws->bbuf = nullptr;
@@ -649,10 +650,13 @@ int al_dev_batch_solve(size_t count, const al_batch_entry* d_entries, double* A_
         AL_HIP(hipMalloc(&ws->bbuf, need));
         ws->bbuf_bytes = need;
     }
-    unsigned* queue = (unsigned*)ws->bbuf;
-    float* W = (float*)((char*)ws->bbuf + 256);
-    return batch_solve(s, count, d_entries, A_pool, B_pool, W, d_info, d_steps, queue, mixed, 10);
+    float* W = (float*)((char*)ws->bbuf + sbytes);
+    return batch_solve(s, ws, count, d_entries, A_pool, B_pool, W, d_info, d_steps, ws->bbuf, mixed, 10);
 }
+
+int al_batch_tune_set(int bucket, int kb, int wgs_per_cu) { return batch_tune_set(bucket, kb, wgs_per_cu); }
+
+int al_batch_tune_get(int bucket, int* kb, int* wgs_per_cu) { return batch_tune_get(bucket, kb, wgs_per_cu); }
 
 // ------------------------------------------------------------------------------------------
 // support routines: dgemv, dgemv_trans, dlange, dtrsv, dtrsv_transpose, dgetri, dsyr2k

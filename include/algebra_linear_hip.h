@@ -230,7 +230,8 @@ int al_dev_dgesv_mixed(size_t n, size_t nrhs, const double* A, size_t lda, const
 
 /* ------------------------------------------------------------------------------------
  * Batched solves (extension; the crate solves one system per call): many independent
- * systems of orders 1..512 in one persistent launch, each a dgesv (LU with partial pivoting,
+ * systems of orders 1..512 in one call (size-bucketed persistent kernels; systems of order
+ * <= 64 in FP64 mode are solved bit for bit as the crate's dgesv / dpotrf + dtrsv would), each a dgesv (LU with partial pivoting,
  * lapack::dgesv, src/lapack/mod.rs:309-323) or a dposv (lower Cholesky, lapack::dpotrf,
  * :368-399, then the two triangular solves).  Matrices live column-major in one device pool
  * (entry i's n x n matrix at A_pool + a_off), right-hand sides in another (n x nrhs at
@@ -257,6 +258,16 @@ typedef struct al_batch_entry {
 
 int al_dev_batch_solve(size_t count, const al_batch_entry* d_entries, double* A_pool, size_t a_pool_len,
                        double* B_pool, int32_t* d_info, int32_t* d_steps, int factor, void* stream);
+
+/* The batched solver's launch table (extension): the systems are split on the device into
+ * size buckets -- 0: n <= 64 (one wave per system), 1: n <= 128, 2: n <= 256, 3: n <= 512 (one
+ * workgroup per system) -- each served by its own persistent kernel.  Per bucket, kb is the
+ * panel width (16 or 32; bucket 3: 16; ignored for bucket 0) and wgs_per_cu the persistent
+ * workgroups launched per CU (capped by what the bucket's LDS allows).  The table is read at
+ * first use from batch_tuning.json beside the library (written by algebra_linear_amd/
+ * tune_batch.py); al_batch_tune_set overrides one entry for the process. */
+int al_batch_tune_set(int bucket, int kb, int wgs_per_cu);
+int al_batch_tune_get(int bucket, int* kb, int* wgs_per_cu);
 
 /* ------------------------------------------------------------------------------------
  * Solve server (extension): continuous batching of independent requests on one device.

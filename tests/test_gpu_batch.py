@@ -211,3 +211,77 @@ def test_graph_captured_batch_replays_match_eager():
         torch.cuda.synchronize()
         np.testing.assert_array_equal(dB.cpu().numpy(), eager)
         assert info.cpu().tolist() == [0] * len(entries)
+
+
+def test_small_orders_are_solved_bit_for_bit_as_the_crate_solves_them():
+    # orders <= 64 go to the one-wave-per-system kernel, which follows the crate's own loops:
+    # dgetrf_with_pivots + dgetrs (src/lapack/mod.rs:112-163, :330-354), dpotrf + dtrsv +
+    # dtrsv_transpose (:368-399; src/blas.rs:1355-1438), unfused and in the crate's order
+    specs = [(n, nrhs, kind) for n, nrhs in [(1, 1), (2, 2), (5, 3), (16, 8), (31, 1), (33, 9), (48, 2), (64, 5)]
+             for kind in (_lib.AL_BATCH_GESV, _lib.AL_BATCH_POSV)]
+    A, B, entries, A_out, X, info, steps = run_batch(specs, _lib.AL_FACTOR_FP64, seed=21)
+    for i, e in enumerate(entries):
+        n, nrhs = e.n, e.nrhs
+        assert info[i] == 0, (i, n)
+        a = system(A, e.a_off, n, n)
+        if e.kind == _lib.AL_BATCH_GESV:
+            st, lu, ipiv, _ = O.dgetrf(a, n)
+            assert st == 0
+            np.testing.assert_array_equal(system(A_out, e.a_off, n, n), lu)
+            for k in range(nrhs):
+                want = O.dgetrs(lu, ipiv, system(B, e.b_off + k * n, n, 1))
+                np.testing.assert_array_equal(system(X, e.b_off + k * n, n, 1), want)
+        else:
+            st, l = O.dpotrf(a, n)
+            assert st == 0
+            low = np.tril(np.ones((n, n), bool)).reshape(-1, order="F")
+            np.testing.assert_array_equal(system(A_out, e.a_off, n, n)[low], l[low])
+            for k in range(nrhs):
+                y = O.dtrsv(l, system(B, e.b_off + k * n, n, 1), n, True, False)
+                want = O.dtrsv_transpose(l, y, n, True, False)
+                np.testing.assert_array_equal(system(X, e.b_off + k * n, n, 1), want)
+
+
+def test_every_bucket_and_its_boundaries():
+    # the bucket edges (64/65, 128/129, 256/257, 512) in one call, in the caller's order, mixed
+    specs = [(n, 1 + n % 5, kind) for n in (512, 3, 257, 64, 256, 65, 129, 128, 1, 300, 100, 40)
+             for kind in (_lib.AL_BATCH_GESV, _lib.AL_BATCH_POSV)]
+    for factor in (_lib.AL_FACTOR_FP64, _lib.AL_FACTOR_FP32):
+        A, B, entries, A_out, X, info, steps = run_batch(specs, factor, seed=31)
+        for i, e in enumerate(entries):
+            n = e.n
+            assert info[i] == 0, (factor, i, n)
+            a = system(A, e.a_off, n, n)
+            if e.kind == _lib.AL_BATCH_POSV:
+                a2 = a.reshape(n, n, order="F")
+                a = (np.tril(a2) + np.tril(a2, -1).T).reshape(-1, order="F")
+            for k in range(e.nrhs):
+                assert O.backward_error(a, system(X, e.b_off + k * n, n, 1), system(B, e.b_off + k * n, n, 1),
+                                        n) <= TOL, (factor, i, n, k)
+
+
+def test_tuning_table_entries_change_the_launch_not_the_result():
+    specs = [(n, 2, kind) for n in (70, 150, 300) for kind in (_lib.AL_BATCH_GESV, _lib.AL_BATCH_POSV)]
+    kb, w = ctypes.c_int(), ctypes.c_int()
+    saved = []
+    for b in range(4):
+        assert LIB.al_batch_tune_get(b, ctypes.byref(kb), ctypes.byref(w)) == 0
+        saved.append((kb.value, w.value))
+    try:
+        _, _, _, _, X1, info1, _ = run_batch(specs, _lib.AL_FACTOR_FP64, seed=41)
+        assert LIB.al_batch_tune_set(1, 32, 1) == 0 and LIB.al_batch_tune_set(2, 32, 1) == 0
+        assert LIB.al_batch_tune_set(3, 32, 1) != 0          # bucket 3 holds a 16-wide panel only
+        A, B, entries, _, X2, info2, _ = run_batch(specs, _lib.AL_FACTOR_FP64, seed=41)
+        assert list(info2) == [0] * len(specs)
+        for i, e in enumerate(entries):
+            n = e.n
+            a = system(A, e.a_off, n, n)
+            if e.kind == _lib.AL_BATCH_POSV:
+                a2 = a.reshape(n, n, order="F")
+                a = (np.tril(a2) + np.tril(a2, -1).T).reshape(-1, order="F")
+            for k in range(e.nrhs):
+                assert O.backward_error(a, system(X2, e.b_off + k * n, n, 1), system(B, e.b_off + k * n, n, 1),
+                                        n) <= TOL
+    finally:
+        for b, (k_, w_) in enumerate(saved):
+            LIB.al_batch_tune_set(b, k_ if b else 16, w_)

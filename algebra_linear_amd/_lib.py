@@ -97,6 +97,7 @@ SIGNATURES = {
     "al_dev_sgemm": (_i, [_i, _i, _sz, _sz, _sz, ctypes.c_float, _v, _sz, _v, _sz, ctypes.c_float, _v, _sz, _v]),
     "al_dev_gemm_bf16": (_i, [_sz, _sz, _sz, ctypes.c_float, _v, _sz, _v, _sz, ctypes.c_float, _v, _sz, _v]),
     "al_dev_batch_solve": (_i, [_sz, _v, _v, _sz, _v, _v, _v, _i, _v]),
+    "al_dev_batch_solve_noalloc": (_i, [_sz, _v, _v, _v, _v, _v, _v]),
     "al_batch_tune_set": (_i, [_i, _i, _i]),
     "al_batch_tune_get": (_i, [_i, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "al_server_create": (_v, [_sz, _i]),

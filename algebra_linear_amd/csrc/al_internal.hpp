@@ -256,12 +256,15 @@ int gesv_mixed(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const double
                size_t ldb, double* X, size_t ldx, int factor, const al_ir_options* opt, al_ir_stats* stats,
                const MixedFactor* reuse = nullptr, MixedFactor* keep = nullptr);
 
-// Batched solver (orders <= 512, size-bucketed persistent kernels): see batch.hip.  scratch:
+// Batched solver (orders <= 512, size-bucketed persistent kernels): see batch.hip.  work:
 // batch_scratch_bytes(count) device bytes; W_pool (mixed mode): an FP32 pool shaped like A_pool.
 // Uses the workspace's side stream and events.
 size_t batch_scratch_bytes(size_t count);
 int batch_solve(hipStream_t s, Workspace* ws, size_t count, const al_batch_entry* d_entries, double* A_pool,
-                double* B_pool, float* W_pool, int* d_info, int* d_steps, void* scratch, bool mixed, int max_iter);
+                double* B_pool, float* W_pool, int* d_info, int* d_steps, void* work, bool mixed, int max_iter);
+// The same FP64 solve with its scratch in a static device array (al_dev_batch_solve_noalloc).
+int batch_solve_noalloc(hipStream_t s, size_t count, const al_batch_entry* d_entries, double* A_pool, double* B_pool,
+                        int* d_info, int* d_steps);
 // The per-bucket launch configuration (bucket 0: n <= 64, 1: <= 128, 2: <= 256, 3: <= 512).
 int batch_tune_set(int bucket, int kb, int wgs_per_cu);
 int batch_tune_get(int bucket, int* kb, int* wgs_per_cu);

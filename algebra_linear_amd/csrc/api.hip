@@ -654,6 +654,20 @@ int al_dev_batch_solve(size_t count, const al_batch_entry* d_entries, double* A_
     return batch_solve(s, ws, count, d_entries, A_pool, B_pool, W, d_info, d_steps, ws->bbuf, mixed, 10);
 }
 
+int al_dev_batch_solve_noalloc(size_t count, const al_batch_entry* d_entries, double* A_pool, double* B_pool,
+                               int32_t* d_info, int32_t* d_steps, void* stream) {
+    if (count > (size_t)INT32_MAX) {
+        set_error("batch_solve_noalloc: too many systems");
+        return AL_ERR_ARG;
+    }
+    int devs = 0;
+    if (hipGetDeviceCount(&devs) != hipSuccess || devs == 0) {
+        set_error("no HIP device visible (algebra_linear_hip needs an MI355X / gfx950)");
+        return AL_ERR_NO_DEVICE;
+    }
+    return batch_solve_noalloc((hipStream_t)stream, count, d_entries, A_pool, B_pool, d_info, d_steps);
+}
+
 int al_batch_tune_set(int bucket, int kb, int wgs_per_cu) { return batch_tune_set(bucket, kb, wgs_per_cu); }
 
 int al_batch_tune_get(int bucket, int* kb, int* wgs_per_cu) { return batch_tune_get(bucket, kb, wgs_per_cu); }

@@ -1182,33 +1182,113 @@ int launch_bucket(int b, hipStream_t s, const al_batch_entry* d_entries, const i
 
 size_t batch_scratch_bytes(size_t count) { return 256 + (size_t)NBUCKETS * count * sizeof(int); }
 
-int batch_solve(hipStream_t s, Workspace* ws, size_t count, const al_batch_entry* d_entries, double* A_pool,
-                double* B_pool, float* W_pool, int* d_info, int* d_steps, void* scratch, bool mixed, int max_iter) {
-    if (count == 0) return AL_OK;
+namespace {
+
+// The launches of one batched call.  work: batch_scratch_bytes(count) device bytes -- [0, 64)
+// the buckets' counts, [64, 128) their queue counters, from 256 the bucket lists; the counts and
+// counters are cleared here (by a kernel, so the sequence can be captured in a graph).  With a
+// side stream (and its two events) the largest bucket runs on s and the others beside it;
+// without one every launch is on s.
+int batch_launch(hipStream_t s, hipStream_t side, hipEvent_t ev_fork, hipEvent_t ev_join, size_t count,
+                 const al_batch_entry* d_entries, double* A_pool, double* B_pool, float* W_pool, int* d_info,
+                 int* d_steps, void* work, bool mixed, int max_iter) {
     std::call_once(g_cfg_once, load_tuning_table);
-    // scratch: [0, 64) the buckets' counts, [64, 128) their queue counters, then the lists
-    int* counts = (int*)scratch;
-    unsigned* queues = (unsigned*)((char*)scratch + 64);
-    int* lists = (int*)((char*)scratch + 256);
-    AL_CHECK(fill_u32(s, (unsigned*)scratch, 64, 0u));
+    int* counts = (int*)work;
+    unsigned* queues = (unsigned*)((char*)work + 64);
+    int* lists = (int*)((char*)work + 256);
+    AL_CHECK(fill_u32(s, (unsigned*)work, 64, 0u));
     hipLaunchKernelGGL(bucket_k, dim3(1), dim3(1024), 0, s, d_entries, (int)count, lists, counts);
     AL_HIP(hipGetLastError());
-    // the largest bucket on s, the others on the side stream, joined by events
-    if (!ws->side) AL_HIP(hipStreamCreateWithFlags(&ws->side, hipStreamNonBlocking));
-    if (!ws->ev_side) AL_HIP(hipEventCreateWithFlags(&ws->ev_side, hipEventDisableTiming));
-    if (!ws->ev_main) AL_HIP(hipEventCreateWithFlags(&ws->ev_main, hipEventDisableTiming));
-    AL_HIP(hipEventRecord(ws->ev_main, s));
-    AL_HIP(hipStreamWaitEvent(ws->side, ws->ev_main, 0));
-    const int ticket = prof_begin(s);
+    if (side) {
+        AL_HIP(hipEventRecord(ev_fork, s));
+        AL_HIP(hipStreamWaitEvent(side, ev_fork, 0));
+    }
     AL_CHECK(launch_bucket(3, s, d_entries, lists, counts, count, A_pool, B_pool, W_pool, d_info, d_steps,
                            queues + 3, mixed, max_iter));
     for (int b = 2; b >= 0; --b)
-        AL_CHECK(launch_bucket(b, ws->side, d_entries, lists, counts, count, A_pool, B_pool, W_pool, d_info, d_steps,
-                               queues + b, mixed, max_iter));
-    AL_HIP(hipEventRecord(ws->ev_side, ws->side));
-    AL_HIP(hipStreamWaitEvent(s, ws->ev_side, 0));
+        AL_CHECK(launch_bucket(b, side ? side : s, d_entries, lists, counts, count, A_pool, B_pool, W_pool, d_info,
+                               d_steps, queues + b, mixed, max_iter));
+    if (side) {
+        AL_HIP(hipEventRecord(ev_join, side));
+        AL_HIP(hipStreamWaitEvent(s, ev_join, 0));
+    }
+    return AL_OK;
+}
+
+}  // namespace
+
+int batch_solve(hipStream_t s, Workspace* ws, size_t count, const al_batch_entry* d_entries, double* A_pool,
+                double* B_pool, float* W_pool, int* d_info, int* d_steps, void* work, bool mixed, int max_iter) {
+    if (count == 0) return AL_OK;
+    if (!ws->side) AL_HIP(hipStreamCreateWithFlags(&ws->side, hipStreamNonBlocking));
+    if (!ws->ev_side) AL_HIP(hipEventCreateWithFlags(&ws->ev_side, hipEventDisableTiming));
+    if (!ws->ev_main) AL_HIP(hipEventCreateWithFlags(&ws->ev_main, hipEventDisableTiming));
+    const int ticket = prof_begin(s);
+    AL_CHECK(batch_launch(s, ws->side, ws->ev_main, ws->ev_side, count, d_entries, A_pool, B_pool, W_pool, d_info,
+                          d_steps, work, mixed, max_iter));
     if (ticket >= 0) prof_end(ticket, s, mixed ? "batch_mixed" : "batch_fp64", 0.0, 0.0);
     return AL_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// the FP64 batched solve that allocates nothing
+// ------------------------------------------------------------------------------------------
+
+namespace {
+
+// Scratch of al_dev_batch_solve_noalloc: the buckets' counts and queue counters and the bucket
+// lists of up to NOALLOC_MAX_SYSTEMS systems (1 MiB + 256 B), one copy per GPU, held until the
+// process ends.  Host code never passes the array's name to a kernel: it is a host-side
+// stand-in, not a device address; noalloc_scratch() resolves each device's copy once.
+constexpr size_t NOALLOC_MAX_SYSTEMS = 65536;
+constexpr size_t NOALLOC_SCRATCH_BYTES = 256 + (size_t)NBUCKETS * NOALLOC_MAX_SYSTEMS * sizeof(int);
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The current device's copy of `scratch`, resolved with hipGetSymbolAddress on the first call on
+// that device (with the device current) and kept.
+int noalloc_scratch(void** out) {
+    int dev = 0;
+    AL_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+        set_error("batch_solve_noalloc: device index beyond 63");
+        return AL_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return fail_hip(e == hipSuccess ? hipErrorInvalidSymbol : e,
+                                                  "hipGetSymbolAddress(scratch)");
+        g_noalloc_ptr[dev] = p;
+    }
+    *out = g_noalloc_ptr[dev];
+    return AL_OK;
+}
+
+// The tuning table is read when the library is loaded, so the no-allocation entry point never
+// opens a file.
+struct LoadTableAtLoad {
+    LoadTableAtLoad() { std::call_once(g_cfg_once, load_tuning_table); }
+} g_load_table_at_load;
+
+}  // namespace
+
+int batch_solve_noalloc(hipStream_t s, size_t count, const al_batch_entry* d_entries, double* A_pool, double* B_pool,
+                        int* d_info, int* d_steps) {
+    if (count > NOALLOC_MAX_SYSTEMS) {
+        set_error("batch_solve_noalloc: more than 65536 systems (the static scratch holds the bucket lists of 65536)");
+        return AL_ERR_ARG;
+    }
+    if (count == 0) return AL_OK;
+    void* work = nullptr;
+    AL_CHECK(noalloc_scratch(&work));
+    // every bucket on the caller's stream: no side stream, no events, no workspace
+    return batch_launch(s, nullptr, nullptr, nullptr, count, d_entries, A_pool, B_pool, nullptr, d_info, d_steps, work,
+                        false, 0);
 }
 
 int batch_tune_set(int bucket, int kb, int wgs_per_cu) {

@@ -259,6 +259,17 @@ typedef struct al_batch_entry {
 int al_dev_batch_solve(size_t count, const al_batch_entry* d_entries, double* A_pool, size_t a_pool_len,
                        double* B_pool, int32_t* d_info, int32_t* d_steps, int factor, void* stream);
 
+/* No counterpart in the crate: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The FP64 form of
+ * al_dev_batch_solve (same kernels, same results bit for bit) that calls no hipMalloc/hipFree
+ * and touches no per-thread workspace: its scratch -- the size buckets' counters and lists --
+ * is a static device array of 1 MiB + 256 B per GPU, held until the process ends, so count is
+ * at most 65536 (AL_ERR_ARG beyond).  Every launch runs on `stream`.  Each call clears the
+ * counters it needs; calls that overlap on one GPU (other streams or threads) share the array
+ * and race, so serialize them. */
+int al_dev_batch_solve_noalloc(size_t count, const al_batch_entry* d_entries, double* A_pool, double* B_pool,
+                               int32_t* d_info, int32_t* d_steps, void* stream);
+
 /* The batched solver's launch table (extension): the systems are split on the device into
  * size buckets -- 0: n <= 64 (one wave per system), 1: n <= 128, 2: n <= 256, 3: n <= 512 (one
  * workgroup per system) -- each served by its own persistent kernel.  Per bucket, kb is the

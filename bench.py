@@ -73,20 +73,75 @@ def hpl_flops(n: int, nrhs: int) -> float:
     return 2.0 / 3.0 * n ** 3 + 2.0 * n * n * nrhs
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(sample_n: int) -> dict:
+    """The crate's CPU algorithms (the oracle's C restatement) on one host core: the CPU-runnable
+    configuration in full (dgemm m = n = k = 512 in all four transpose forms with the AVX2+FMA
+    micro-kernel, dgesv n = 512 with 4 right-hand sides as one dgetrf and four dgetrs), dgemm at
+    1024 and 4096, the unblocked dgetrf at 512 and 1024, and the headline-metric sample: dgesv at
+    sample_n, with its n^3 extrapolation to the GPU sizes (labelled as such)."""
     import numpy as np
 
     import oracle as O
     rng = np.random.default_rng(7)
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        r = fn()
+        return r, time.perf_counter() - t0
+
+    seed = [1988, 1989, 1990, 1991]
+    out = {"cpu_model": cpu_model(), "host_cores": os.cpu_count(), "threads_used": 1, "config1": {}, "dgemm": [],
+           "dgetrf": []}
+    # the CPU-runnable configuration, in full: uniform(-1, 1) from the larnv stream
+    n = 512
+    ab = O.larnv(O.SYMMETRIC, seed, 3 * n * n)[0]
+    a, b, c = ab[:n * n], ab[n * n:2 * n * n], ab[2 * n * n:]
+    for name, fn in (("nn", O.dgemm), ("tn", O.dgemm_tn), ("nt", O.dgemm_nt), ("tt", O.dgemm_tt)):
+        _, dt = timed(lambda: fn(1.0, a, b, 0.0, c, n, n, n))
+        out["config1"][f"dgemm_{name}_512"] = {"ms": round(dt * 1e3, 2), "gflops": round(2.0 * n ** 3 / dt / 1e9, 2)}
+    bb = O.larnv(O.SYMMETRIC, [7, 11, 13, 17], n * 4)[0]
+
+    def gesv512():
+        st, lu, ipiv, _ = O.dgetrf(a, n)
+        assert st == 0
+        return [O.dgetrs(lu, ipiv, bb[k * n:(k + 1) * n]) for k in range(4)]
+    _, dt = timed(gesv512)
+    out["config1"]["dgesv_512_nrhs4"] = {"ms": round(dt * 1e3, 2), "gflops": round(hpl_flops(n, 4) / dt / 1e9, 3)}
+    for n in (1024, 4096):
+        x = rng.uniform(-1.0, 1.0, 2 * n * n)
+        _, dt = timed(lambda: O.dgemm(1.0, x[:n * n], x[n * n:], 0.0, np.zeros(n * n), n, n, n))
+        out["dgemm"].append({"n": n, "s": round(dt, 3), "gflops": round(2.0 * n ** 3 / dt / 1e9, 2)})
+    for n in (512, 1024):
+        x = rng.uniform(-1.0, 1.0, n * n)
+        _, dt = timed(lambda: O.dgetrf(x, n))
+        out["dgetrf"].append({"n": n, "s": round(dt, 3), "gflops": round(2.0 / 3.0 * n ** 3 / dt / 1e9, 3)})
     a = rng.uniform(-1.0, 1.0, sample_n * sample_n)
     b = rng.uniform(-1.0, 1.0, sample_n)
-    t0 = time.perf_counter()
-    st, _, _ = O.dgesv(a, b)
-    dt = time.perf_counter() - t0
+    (st, _, _), dt = timed(lambda: O.dgesv(a, b))   # dgetrf + one dgetrs: the factorization's time
     assert st == 0
-    return {"value": hpl_flops(sample_n, 1) / dt / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port",
-            "sample": f"oracle dgesv (the crate's unblocked LU + substitution, C restatement) n={sample_n}, "
-                      f"nrhs=1, one core, {dt:.1f} s"}
+    out["dgetrf"].append({"n": sample_n, "s": round(dt, 3), "gflops": round(2.0 / 3.0 * sample_n ** 3 / dt / 1e9, 3),
+                          "as": "dgesv with one right-hand side"})
+    scale = lambda m: (m / sample_n) ** 3  # noqa: E731
+    out["dgetrf_extrapolated_s"] = {"note": f"n^3 extrapolation of the n = {sample_n} time, not measured (the "
+                                            "stride-n loop slows further with n, so these are lower bounds)",
+                                    "8192": round(dt * scale(8192), 1), "16384": round(dt * scale(16384), 1)}
+    out.update({"value": hpl_flops(sample_n, 1) / dt / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port",
+                "sample": f"oracle dgesv (the crate's unblocked LU + substitution, C restatement, -O2 "
+                          f"-ffp-contract=off) n={sample_n}, nrhs=1, one thread on {out['cpu_model']} "
+                          f"({out['host_cores']} host cores), {dt:.1f} s; the n = 16384 headline would take "
+                          f"about {out['dgetrf_extrapolated_s']['16384']} s (extrapolated)"})
+    return out
 
 
 def run_batched(count: int, steps: int, warmup: int, lib, _lib, stream, sp) -> dict:

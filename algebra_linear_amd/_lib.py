@@ -54,6 +54,18 @@ class BatchEntry(ctypes.Structure):
     """``al_batch_entry``: one system of a batched solve."""
     _fields_ = [("a_off", ctypes.c_size_t), ("b_off", ctypes.c_size_t), ("n", ctypes.c_int32),
                 ("nrhs", ctypes.c_int32), ("kind", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class Complex(ctypes.Structure):
+    """``al_complex``: the crate's Complex64 (num-complex ``Complex<f64>``, ``#[repr(C)]``)."""
+    _fields_ = [("re", ctypes.c_double), ("im", ctypes.c_double)]
+
+    @classmethod
+    def of(cls, v) -> "Complex":
+        v = complex(v)
+        return cls(v.real, v.imag)
+
+
 _sp = ctypes.POINTER(ctypes.c_size_t)
 _ip = ctypes.POINTER(ctypes.c_int32)
 _d, _sz, _i, _v = ctypes.c_double, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p
@@ -92,6 +104,18 @@ SIGNATURES = {
     "al_dev_dtrsm": (_i, [_i, _i, _i, _sz, _sz, _d, _v, _sz, _v, _sz, _v]),
     "al_dev_dgetrf": (_i, [_sz, _sz, _v, _sz, _v, _v, _v]),
     "al_dev_dgetrs": (_i, [_sz, _sz, _v, _sz, _v, _v, _sz, _v]),
+    "al_zgemm": (_i, [Complex, _dp, _sz, _dp, _sz, Complex, _dp, _sz, _sz, _sz, _sz]),
+    "al_zgemm_tn": (_i, [Complex, _dp, _sz, _dp, _sz, Complex, _dp, _sz, _sz, _sz, _sz]),
+    "al_zgemm_nt": (_i, [Complex, _dp, _sz, _dp, _sz, Complex, _dp, _sz, _sz, _sz, _sz]),
+    "al_zgemm_tt": (_i, [Complex, _dp, _sz, _dp, _sz, Complex, _dp, _sz, _sz, _sz, _sz]),
+    "al_zgetrf": (_i, [_dp, _sz, _sz, _sp, _sz]),
+    "al_zgetrf_rect": (_i, [_dp, _sz, _sz, _sz, _sp, _sz]),
+    "al_zgesv": (_i, [_dp, _sz, _dp, _sz]),
+    "al_zpotrf": (_i, [_dp, _sz, _sz, _i]),
+    "al_dev_zgemm": (_i, [_i, _i, _sz, _sz, _sz, Complex, _v, _sz, _v, _sz, Complex, _v, _sz, _v]),
+    "al_dev_zgetrf": (_i, [_sz, _sz, _v, _sz, _v, _v, _v]),
+    "al_dev_zgetrs": (_i, [_sz, _sz, _v, _sz, _v, _v, _sz, _v]),
+    "al_dev_zpotrf": (_i, [_i, _sz, _v, _sz, _v, _v]),
     "al_dev_dpotrf": (_i, [_sz, _v, _sz, _v, _v]),
     "al_dev_dlaswp": (_i, [_sz, _v, _sz, _sz, _sz, _v, _v]),
     "al_dev_sgemm": (_i, [_i, _i, _sz, _sz, _sz, ctypes.c_float, _v, _sz, _v, _sz, ctypes.c_float, _v, _sz, _v]),

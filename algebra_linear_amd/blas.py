@@ -106,3 +106,47 @@ def dtrsv_transpose(a, x: np.ndarray, n: int, lower: bool, unit: bool) -> None:
     """Solves A^T x = b in place (src/blas.rs:1366)."""
     a, xx = _in(a), _out(x)
     check_blas(_lib.load().al_dtrsv_transpose(_p(a), a.size, _p(xx), xx.size, n, int(lower), int(unit)))
+
+
+# --- complex twins (src/blas.rs:1023-1118): complex128 arrays, Complex64's layout ----------
+
+def _zin(x) -> np.ndarray:
+    """A read-only complex operand, flattened (complex128 = interleaved re, im doubles)."""
+    return np.ascontiguousarray(np.asarray(x, dtype=np.complex128)).reshape(-1)
+
+
+def _zout(x) -> np.ndarray:
+    """An in-place complex operand (``&mut [Complex64]``)."""
+    if not (isinstance(x, np.ndarray) and x.dtype == np.complex128 and x.flags.c_contiguous and x.flags.writeable):
+        raise TypeError("in-place operand must be a writeable, contiguous complex128 numpy array")
+    return x.reshape(-1)
+
+
+def _zp(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def _zgemm(fn, alpha, a, b, beta, c, m, n, k) -> None:
+    a, b, cc = _zin(a), _zin(b), _zout(c)
+    C = _lib.Complex.of
+    check_blas(fn(C(alpha), _zp(a), a.size, _zp(b), b.size, C(beta), _zp(cc), cc.size, m, n, k))
+
+
+def zgemm(alpha, a, b, beta, c: np.ndarray, m: int, n: int, k: int) -> None:
+    """C := alpha*A*B + beta*C for complex data; ``a`` m x k, ``b`` k x n (src/blas.rs:1023)."""
+    _zgemm(_lib.load().al_zgemm, alpha, a, b, beta, c, m, n, k)
+
+
+def zgemm_tn(alpha, a, b, beta, c: np.ndarray, m: int, n: int, k: int) -> None:
+    """C := alpha*A^T*B + beta*C, not conjugated; ``a`` k x m (src/blas.rs:1042)."""
+    _zgemm(_lib.load().al_zgemm_tn, alpha, a, b, beta, c, m, n, k)
+
+
+def zgemm_nt(alpha, a, b, beta, c: np.ndarray, m: int, n: int, k: int) -> None:
+    """C := alpha*A*B^T + beta*C, not conjugated; ``b`` n x k (src/blas.rs:1059)."""
+    _zgemm(_lib.load().al_zgemm_nt, alpha, a, b, beta, c, m, n, k)
+
+
+def zgemm_tt(alpha, a, b, beta, c: np.ndarray, m: int, n: int, k: int) -> None:
+    """C := alpha*A^T*B^T + beta*C; ``a`` k x m, ``b`` n x k (src/blas.rs:1071)."""
+    _zgemm(_lib.load().al_zgemm_tt, alpha, a, b, beta, c, m, n, k)

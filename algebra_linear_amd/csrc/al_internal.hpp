@@ -11,6 +11,7 @@
 #include <string>
 
 #include "../../include/algebra_linear_hip.h"
+#include "zcomplex.hpp"
 
 namespace al {
 
@@ -96,6 +97,8 @@ struct Workspace {
     size_t gbuf_bytes = 0;
     void* bbuf = nullptr;          // batched solver: FP32 work matrices
     size_t bbuf_bytes = 0;
+    void* zbuf = nullptr;          // complex twins: real-form staging (R(L), R(A), E, W)
+    size_t zbuf_bytes = 0;
     // a second stream (default priority) for the panel factorization that runs ahead of the
     // trailing update in the FP32 LU, and the two events that order the streams
     hipStream_t side = nullptr;
@@ -138,6 +141,11 @@ int thread_stream(hipStream_t* out);
 // C := alpha*op(A)*op(B) + beta*C; tri: 0 full, 1 lower triangle only, 2 upper only.
 int gemm_f64(hipStream_t s, bool ta, bool tb, size_t m, size_t n, size_t k, double alpha, const double* A,
              size_t lda, const double* B, size_t ldb, double beta, double* C, size_t ldc, int tri = 0);
+// The FP64 GEMM restricted to a complex triangle of C's real view (2m' x n' with row r in
+// complex row r >> 1): upper = false -> op = (N, N), complex rows >= columns kept; upper = true
+// -> op = (T, N), complex rows <= columns kept.  m is the real row count.
+int gemm_f64_ctri(hipStream_t s, bool upper, size_t m, size_t n, size_t k, double alpha, const double* A, size_t lda,
+                  const double* B, size_t ldb, double beta, double* C, size_t ldc);
 // The same on FP32 matrix cores (v_mfma_f32_16x16x4_f32), FP32 accumulation.
 int gemm_f32(hipStream_t s, bool ta, bool tb, size_t m, size_t n, size_t k, float alpha, const float* A, size_t lda,
              const float* B, size_t ldb, float beta, float* C, size_t ldc, int tri = 0);
@@ -163,6 +171,9 @@ int swap_block_f64(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, siz
 // that may run concurrently on two streams need buffers of their own.
 int swap_block_f32(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, size_t jb, float* A0, size_t n0,
                    float* A1, size_t n1, size_t lda, const int* info, int* pairs = nullptr);
+// The same for a complex matrix (lda in complex elements).
+int swap_block_z(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, size_t jb, zd* A0, size_t n0, zd* A1,
+                 size_t n1, size_t lda, const int* info);
 // Applies (destination, source) row pairs already composed (as tslu_panel writes them) to the
 // columns [0, n0) of A0 and [0, n1) of A1; nothing when *info != 0.
 int swap_apply_f32(hipStream_t s, const int* pairs, float* A0, size_t n0, float* A1, size_t n1, size_t lda,
@@ -207,6 +218,23 @@ int getrf_enqueue(hipStream_t s, Workspace* ws, size_t m, size_t n, double* A, s
 // global row indices, swaps applied inside the panel only.
 int panel_f32(hipStream_t s, Workspace* ws, float* P, size_t lda, size_t row0, size_t m, size_t ncols, int* ipiv,
               int* info);
+// The same panel for a complex matrix, ncols <= panel_z_width() (32), |Re| + |Im| pivots.
+int panel_z(hipStream_t s, Workspace* ws, zd* P, size_t lda, size_t row0, size_t m, size_t ncols, int* ipiv,
+            int* info);
+int panel_z_width();
+
+// Complex twins (complex.hip).  Complex matrices are interleaved (re, im) with leading
+// dimensions in complex elements.
+// C := alpha op(A) op(B) + beta C (never conjugated), beta*C formed for every entry.
+int zgemm_dev(hipStream_t s, Workspace* ws, bool ta, bool tb, size_t m, size_t n, size_t k, zd alpha, const zd* A,
+              size_t lda, const zd* B, size_t ldb, zd beta, zd* C, size_t ldc);
+// Complex LU with partial pivoting (m >= n for the panel form), pivots int32 global rows.
+int zgetrf_dev(hipStream_t s, Workspace* ws, size_t m, size_t n, zd* A, size_t lda, int* ipiv, int* info);
+// B (n x nrhs) := A^-1 B from zgetrf_dev's factors.
+int zgetrs_dev(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const zd* LU, size_t lda, const int* ipiv,
+               zd* B, size_t ldb, const int* info = nullptr);
+// Complex Cholesky, lower (A = L L^H) or upper (A = U^H U); only that triangle is read/written.
+int zpotrf_dev(hipStream_t s, Workspace* ws, size_t n, zd* A, size_t lda, bool upper, int* info);
 
 // Tournament-pivoted 64-column panel of an FP32 matrix F (rows J..J+m, columns J..J+64):
 // tslu_panel chooses the pivots (ipiv[J..J+64), global indices) and the LU of the pivot rows;

@@ -49,6 +49,7 @@ struct ThreadCtx {
             if (w.hnorms) (void)hipHostFree(w.hnorms);
             if (w.gbuf) (void)hipFree(w.gbuf);
             if (w.bbuf) (void)hipFree(w.bbuf);
+            if (w.zbuf) (void)hipFree(w.zbuf);
             if (w.ev_side) (void)hipEventDestroy(w.ev_side);
             if (w.ev_main) (void)hipEventDestroy(w.ev_main);
             if (w.side) (void)hipStreamDestroy(w.side);
@@ -867,6 +868,178 @@ int al_dev_dlaswp(size_t ncols, double* A, size_t lda, size_t k0, size_t k1, con
         AL_CHECK(swap_block_f64(s, ws, d_ipiv, k, jb, A, ncols, nullptr, 0, lda, nullptr, nullptr));
     }
     return AL_OK;
+}
+
+// ---- complex twins (complex.hip) --------------------------------------------------------
+
+static zd zc(al_complex v) { return zd(v.re, v.im); }
+
+static int zgemm_host(const char* name, bool ta, bool tb, al_complex alpha, const al_complex* a, size_t a_len,
+                      const al_complex* b, size_t b_len, al_complex beta, al_complex* c, size_t c_len, size_t m,
+                      size_t n, size_t k) {
+    // messages of src/blas.rs:1024-1026, :1043-1045, :1060-1062, :1072-1074
+    std::string nm(name);
+    if (!len_is(a_len, ta ? k : m, ta ? m : k)) return bad_shape((nm + (ta ? ": a must be k*m" : ": a must be m*k")).c_str());
+    if (!len_is(b_len, tb ? n : k, tb ? k : n)) return bad_shape((nm + (tb ? ": b must be n*k" : ": b must be k*n")).c_str());
+    if (!len_is(c_len, m, n)) return bad_shape((nm + ": c must be m*n").c_str());
+    if (m == 0 || n == 0) return AL_OK;
+    if ((!a && a_len) || (!b && b_len) || !c) {
+        set_error("null pointer");
+        return AL_ERR_ARG;
+    }
+    hipStream_t s;
+    AL_CHECK(thread_stream(&s));
+    Workspace* ws;
+    AL_CHECK(workspace_for(s, &ws, 1));
+    double *dA = nullptr, *dB = nullptr, *dC;
+    if (k) {
+        AL_CHECK(upload(0, (const double*)a, 2 * a_len, &dA, s));
+        AL_CHECK(upload(1, (const double*)b, 2 * b_len, &dB, s));
+    }
+    AL_CHECK(upload(2, (const double*)c, 2 * c_len, &dC, s));
+    AL_CHECK(zgemm_dev(s, ws, ta, tb, m, n, k, zc(alpha), (const zd*)dA, ta ? k : m, (const zd*)dB, tb ? n : k,
+                       zc(beta), (zd*)dC, m));
+    AL_CHECK(download((double*)c, dC, 2 * c_len, s));
+    return finish(s);
+}
+
+int al_zgemm(al_complex alpha, const al_complex* a, size_t a_len, const al_complex* b, size_t b_len, al_complex beta,
+             al_complex* c, size_t c_len, size_t m, size_t n, size_t k) {
+    return zgemm_host("zgemm", false, false, alpha, a, a_len, b, b_len, beta, c, c_len, m, n, k);
+}
+int al_zgemm_tn(al_complex alpha, const al_complex* a, size_t a_len, const al_complex* b, size_t b_len,
+                al_complex beta, al_complex* c, size_t c_len, size_t m, size_t n, size_t k) {
+    return zgemm_host("zgemm_tn", true, false, alpha, a, a_len, b, b_len, beta, c, c_len, m, n, k);
+}
+int al_zgemm_nt(al_complex alpha, const al_complex* a, size_t a_len, const al_complex* b, size_t b_len,
+                al_complex beta, al_complex* c, size_t c_len, size_t m, size_t n, size_t k) {
+    return zgemm_host("zgemm_nt", false, true, alpha, a, a_len, b, b_len, beta, c, c_len, m, n, k);
+}
+int al_zgemm_tt(al_complex alpha, const al_complex* a, size_t a_len, const al_complex* b, size_t b_len,
+                al_complex beta, al_complex* c, size_t c_len, size_t m, size_t n, size_t k) {
+    return zgemm_host("zgemm_tt", true, true, alpha, a, a_len, b, b_len, beta, c, c_len, m, n, k);
+}
+
+// Factors M x N complex host data in place on the device; pivots downloaded 0-based.
+static int zgetrf_host(al_complex* a, size_t M, size_t N, size_t* ipiv, hipStream_t s, Workspace* ws,
+                       double** dA_out) {
+    double* dA;
+    AL_CHECK(upload(0, (const double*)a, 2 * M * N, &dA, s));
+    const size_t kmax = M < N ? M : N;
+    AL_CHECK(iota_i32(s, ws->ipiv32, kmax));
+    AL_CHECK(zgetrf_dev(s, ws, M, N, (zd*)dA, M, ws->ipiv32, ws->info));
+    int info = 0;
+    std::vector<int> piv(kmax);
+    AL_CHECK(download((double*)a, dA, 2 * M * N, s));
+    if (kmax) AL_HIP(hipMemcpyAsync(piv.data(), ws->ipiv32, kmax * sizeof(int), hipMemcpyDeviceToHost, s));
+    AL_HIP(hipMemcpyAsync(&info, ws->info, sizeof(int), hipMemcpyDeviceToHost, s));
+    AL_CHECK(finish(s));
+    for (size_t i = 0; i < kmax; ++i) ipiv[i] = (size_t)piv[i];
+    if (dA_out) *dA_out = dA;
+    return status_from_info(info);
+}
+
+int al_zgetrf(al_complex* a, size_t a_len, size_t n, size_t* ipiv, size_t ipiv_len) {
+    // src/lapack/complex.rs:67-73
+    if (!len_is(a_len, n, n) || ipiv_len != n) return bad_shape("zgetrf: a must be n*n and ipiv n");
+    if (n == 0) return AL_OK;
+    hipStream_t s;
+    AL_CHECK(thread_stream(&s));
+    Workspace* ws;
+    AL_CHECK(workspace_for(s, &ws, n));
+    return zgetrf_host(a, n, n, ipiv, s, ws, nullptr);
+}
+
+int al_zgetrf_rect(al_complex* a, size_t a_len, size_t nrows, size_t ncols, size_t* ipiv, size_t ipiv_len) {
+    // src/lapack/complex.rs:169-175
+    if (nrows < ncols || !len_is(a_len, nrows, ncols) || ipiv_len != ncols)
+        return bad_shape("zgetrf_rect: need nrows >= ncols, a nrows*ncols and ipiv ncols");
+    if (ncols == 0) return AL_OK;
+    hipStream_t s;
+    AL_CHECK(thread_stream(&s));
+    Workspace* ws;
+    AL_CHECK(workspace_for(s, &ws, ncols));
+    return zgetrf_host(a, nrows, ncols, ipiv, s, ws, nullptr);
+}
+
+int al_zgesv(al_complex* a, size_t a_len, al_complex* b, size_t b_len) {
+    // src/lapack/complex.rs:279-290: n is b's length
+    const size_t n = b_len;
+    if (!len_is(a_len, n, n)) return bad_shape("zgesv: a must be n*n for n = b.len()");
+    if (n == 0) return AL_OK;
+    hipStream_t s;
+    AL_CHECK(thread_stream(&s));
+    Workspace* ws;
+    AL_CHECK(workspace_for(s, &ws, n));
+    std::vector<size_t> piv(n);
+    double* dA;
+    const int st = zgetrf_host(a, n, n, piv.data(), s, ws, &dA);
+    if (st != AL_OK) return st;  // b untouched, as the crate's early return leaves it
+    double* dB;
+    AL_CHECK(upload(1, (const double*)b, 2 * n, &dB, s));
+    AL_CHECK(zgetrs_dev(s, ws, n, 1, (const zd*)dA, n, ws->ipiv32, (zd*)dB, n, ws->info));
+    return download_solution((double*)b, dB, 2 * n, s, ws);
+}
+
+int al_zpotrf(al_complex* a, size_t a_len, size_t n, int upper) {
+    // src/lapack/complex.rs:344-347
+    if (!len_is(a_len, n, n)) return bad_shape("zpotrf: a must be n*n");
+    if (n == 0) return AL_OK;
+    hipStream_t s;
+    AL_CHECK(thread_stream(&s));
+    Workspace* ws;
+    AL_CHECK(workspace_for(s, &ws, 1));
+    double* dA;
+    AL_CHECK(upload(0, (const double*)a, 2 * a_len, &dA, s));
+    AL_CHECK(zpotrf_dev(s, ws, n, (zd*)dA, n, upper != 0, ws->info));
+    int info = 0;
+    AL_CHECK(download((double*)a, dA, 2 * a_len, s));
+    AL_HIP(hipMemcpyAsync(&info, ws->info, sizeof(int), hipMemcpyDeviceToHost, s));
+    AL_CHECK(finish(s));
+    return status_from_info(info);
+}
+
+int al_dev_zgemm(int trans_a, int trans_b, size_t m, size_t n, size_t k, al_complex alpha, const al_complex* A,
+                 size_t lda, const al_complex* B, size_t ldb, al_complex beta, al_complex* C, size_t ldc,
+                 void* stream) {
+    if (ldc < m || lda < (trans_a ? k : m) || ldb < (trans_b ? n : k)) return bad_shape("zgemm: leading dimension too small");
+    hipStream_t s;
+    AL_CHECK(dev_stream(stream, &s));
+    Workspace* ws;
+    AL_CHECK(workspace_for(s, &ws, 1));
+    return zgemm_dev(s, ws, trans_a != 0, trans_b != 0, m, n, k, zc(alpha), (const zd*)A, lda, (const zd*)B, ldb,
+                     zc(beta), (zd*)C, ldc);
+}
+
+int al_dev_zgetrf(size_t m, size_t n, al_complex* A, size_t lda, int32_t* d_ipiv, int32_t* d_info, void* stream) {
+    if (lda < m || m < n) return bad_shape("zgetrf: need lda >= m >= n");
+    hipStream_t s;
+    AL_CHECK(dev_stream(stream, &s));
+    Workspace* ws;
+    AL_CHECK(workspace_for(s, &ws, 1));
+    const size_t kmax = m < n ? m : n;
+    if (kmax == 0) return fill_i32(s, d_info, 1, 0);
+    AL_CHECK(iota_i32(s, d_ipiv, kmax));
+    return zgetrf_dev(s, ws, m, n, (zd*)A, lda, d_ipiv, d_info);
+}
+
+int al_dev_zgetrs(size_t n, size_t nrhs, const al_complex* LU, size_t lda, const int32_t* d_ipiv, al_complex* B,
+                  size_t ldb, void* stream) {
+    if (lda < n || ldb < n) return bad_shape("zgetrs: leading dimension too small");
+    hipStream_t s;
+    AL_CHECK(dev_stream(stream, &s));
+    Workspace* ws;
+    AL_CHECK(workspace_for(s, &ws, 1));
+    return zgetrs_dev(s, ws, n, nrhs, (const zd*)LU, lda, d_ipiv, (zd*)B, ldb);
+}
+
+int al_dev_zpotrf(int upper, size_t n, al_complex* A, size_t lda, int32_t* d_info, void* stream) {
+    if (lda < n) return bad_shape("zpotrf: leading dimension too small");
+    hipStream_t s;
+    AL_CHECK(dev_stream(stream, &s));
+    Workspace* ws;
+    AL_CHECK(workspace_for(s, &ws, 1));
+    return zpotrf_dev(s, ws, n, (zd*)A, lda, upper != 0, d_info);
 }
 
 }  // extern "C"

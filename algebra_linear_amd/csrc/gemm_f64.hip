@@ -1,6 +1,8 @@
 // FP64 GEMM on CDNA4 matrix cores: C := alpha*op(A)*op(B) + beta*C, column-major, with
 // the four transposition variants of blas::dgemm{,_tn,_nt,_tt} (src/blas.rs:201-257) and a
-// triangle-restricted form for dsyrk (src/blas.rs:609-626).
+// triangle-restricted form for dsyrk (src/blas.rs:609-626).  Two more restrictions serve the
+// complex Cholesky: on the real view of a complex matrix (row r holds part r & 1 of complex row
+// r >> 1) they keep the complex lower (r >> 1 >= column) or upper (r >> 1 <= column) triangle.
 //
 // Design (MI355X-first, not a translation of the crate's Goto blocking):
 //   * one 256-thread workgroup (4 wave64s, 2x2) computes a 128x128 tile of C; each wave
@@ -112,6 +114,8 @@ __global__ __launch_bounds__(NT, 2) void dgemm_mfma(int m, int n, int k, T alpha
     const int bm = tm * BM, bn = tn * BN;
     if (TRI == 1 && bm + BM - 1 < bn) return;  // tile strictly above the diagonal
     if (TRI == 2 && bm > bn + BN - 1) return;  // tile strictly below the diagonal
+    if (TRI == 3 && ((bm + BM - 1) >> 1) < bn) return;  // complex rows strictly above
+    if (TRI == 4 && (bm >> 1) > bn + BN - 1) return;    // complex rows strictly below
 
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
@@ -180,6 +184,8 @@ __global__ __launch_bounds__(NT, 2) void dgemm_mfma(int m, int n, int k, T alpha
                 if (cc >= n) continue;
                 if (TRI == 1 && row < cc) continue;
                 if (TRI == 2 && row > cc) continue;
+                if (TRI == 3 && (row >> 1) < cc) continue;
+                if (TRI == 4 && (row >> 1) > cc) continue;
                 T* cp = C + (size_t)row + (size_t)cc * ldc;
                 T v = alpha * acc[i][j][r];
                 if (beta != (T)0) v += beta * *cp;
@@ -231,7 +237,9 @@ int launch(hipStream_t s, int m, int n, int k, T alpha, const T* A, size_t lda, 
              {"ssyrk_lower", "ssyrk_lower", "ssyrk_lower", "ssyrk_lower"},
              {"ssyrk_upper", "ssyrk_upper", "ssyrk_upper", "ssyrk_upper"}}};
         const double by2 = by * (double)sizeof(T) / 8.0;
-        prof_end(ticket, s, names[sizeof(T) == 4 ? 1 : 0][TRI][(TA ? 1 : 0) + (TB ? 2 : 0)], fl, by2);
+        const char* nm = TRI >= 3 ? (TRI == 3 ? "zherk_lower_real_view" : "zherk_upper_real_view")
+                                  : names[sizeof(T) == 4 ? 1 : 0][TRI][(TA ? 1 : 0) + (TB ? 2 : 0)];
+        prof_end(ticket, s, nm, fl, by2);
     }
     return AL_OK;
 }
@@ -272,6 +280,17 @@ int gemm_t(hipStream_t s, bool ta, bool tb, size_t m, size_t n, size_t k, T alph
 int gemm_f64(hipStream_t s, bool ta, bool tb, size_t m, size_t n, size_t k, double alpha, const double* A,
              size_t lda, const double* B, size_t ldb, double beta, double* C, size_t ldc, int tri) {
     return gemm_t<double>(s, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri);
+}
+
+int gemm_f64_ctri(hipStream_t s, bool upper, size_t m, size_t n, size_t k, double alpha, const double* A, size_t lda,
+                  const double* B, size_t ldb, double beta, double* C, size_t ldc) {
+    if (m == 0 || n == 0 || k == 0) return AL_OK;
+    if (m > INT32_MAX || n > INT32_MAX || k > INT32_MAX) {
+        set_error("gemm: dimensions beyond 2^31 are not supported");
+        return AL_ERR_ARG;
+    }
+    if (upper) return launch<double, true, false, 4>(s, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc);
+    return launch<double, false, false, 3>(s, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
 int gemm_f32(hipStream_t s, bool ta, bool tb, size_t m, size_t n, size_t k, float alpha, const float* A, size_t lda,

@@ -15,6 +15,9 @@
 //   * row swaps of the columns outside a panel, the U12 triangular solve and the trailing
 //     update are separate kernels on the same stream; each checks the status word first so
 //     that a zero pivot stops the factorization like the crate's early return.
+// The panel kernel is also instantiated for float (the mixed-precision factor) and for the
+// complex element `zd` (lapack::zgetrf, src/lapack/complex.rs:67-121: |Re| + |Im| pivot keys,
+// complex division and multiply-subtract, 32-column panels).
 #include "al_internal.hpp"
 #include "coop_ledger.hpp"
 
@@ -26,12 +29,27 @@
 namespace al {
 namespace {
 
-constexpr int PNB = 64;        // panel width
+constexpr int PNB = 64;        // panel width of the real factorizations
 constexpr int PT = 256;        // threads per panel workgroup
 constexpr int CHUNK_MAX = 256; // rows per workgroup, one per thread (LDS: 256 x 64 doubles = 128 KiB)
 constexpr int MAX_WG = 128;    // cooperative workgroups per panel (one per CU at most)
-constexpr int SLOT2 = 2 + 2 * PNB;  // key, index, candidate row, row j
 constexpr int NBO = 256;       // outer block (trailing-update depth)
+
+// Per element type: the pivot key's type and the panel width that keeps a CHUNK_MAX-row
+// chunk of the panel at 128 KiB of LDS (64 real columns, 32 complex ones).
+template <typename T>
+struct PanelElem {
+    typedef T key_t;
+    static constexpr int NB = PNB;
+};
+template <>
+struct PanelElem<zd> {
+    typedef double key_t;
+    static constexpr int NB = 32;
+};
+// exchange slot: key, index, candidate row, row j
+template <typename T>
+constexpr int slot2() { return 2 + 2 * PanelElem<T>::NB; }
 
 // Candidate ordering: larger key wins, ties go to the lower row (first maximum).
 template <typename T>
@@ -47,6 +65,18 @@ __device__ __forceinline__ T pivot_key(T v, int g, int j) {
     if (v != v) return (g == j) ? (T)INFINITY : (T)-1;
     return v < (T)0 ? -v : v;
 }
+// complex entries: |Re| + |Im| (src/lapack/complex.rs:78-90), with the same NaN rule
+__device__ __forceinline__ double pivot_key(zd v, int g, int j) {
+    const double a = abs1(v);
+    if (a != a) return (g == j) ? INFINITY : -1.0;
+    return a;
+}
+
+// A slot's key and index travel as elements of the panel's type (the index is exact: rows
+// stay below 2^24); a complex slot carries them in the real part.
+__device__ __forceinline__ double slot_val(double v) { return v; }
+__device__ __forceinline__ float slot_val(float v) { return v; }
+__device__ __forceinline__ double slot_val(zd v) { return v.re; }
 
 // Order-preserving integer image of a pivot key: non-negative keys (|v| or +inf) keep their
 // IEEE order as unsigned bit patterns; the sentinels -1 (a skipped NaN) and -2 (no candidate)
@@ -117,14 +147,17 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
                                                   int chunk, int* __restrict__ ipiv, int* __restrict__ info,
                                                   unsigned* __restrict__ flags, unsigned* __restrict__ abort_flag,
                                                   T* __restrict__ slots, unsigned gen0, long long timeout) {
+    typedef typename PanelElem<T>::key_t K;
+    constexpr int NB = PanelElem<T>::NB;
+    constexpr int SLOT2 = slot2<T>();
     extern __shared__ __align__(16) unsigned char smem[];
-    T* S = (T*)smem;  // chunk x PNB, column-major with leading dimension `chunk`
+    T* S = (T*)smem;  // chunk x NB, column-major with leading dimension `chunk`
     __shared__ T lslot[2][SLOT2];  // single-workgroup exchange
-    __shared__ T prow_c[PNB];
-    __shared__ T rj_c[PNB];
+    __shared__ T prow_c[NB];
+    __shared__ T rj_c[NB];
     __shared__ ArgmaxLds red;
     __shared__ int win_lds;
-    __shared__ T wkey_lds;
+    __shared__ K wkey_lds;
     __shared__ int ok_lds;
 
     const int nwg = COOP ? (int)gridDim.x : 1;
@@ -149,18 +182,18 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
         return COOP ? slots + ((size_t)par * nwg + q) * SLOT2 : &lslot[par][0];
     };
     // wave 0 writes this workgroup's slot for column jj and raises its flag
-    auto publish = [&](int jj, T key, int idx) {
+    auto publish = [&](int jj, K key, int idx) {
         if (wave == 0) {
             T* my = slot_ptr(jj & 1, w);
-            if (key > (T)-2) {
+            if (key > (K)-2) {
                 const int lr = idx - rb;
                 for (int c = lane; c < ncols; c += 64) my[2 + c] = S[c * chunk + lr];
             }
             if (jj >= rb && jj < rb + nr)
-                for (int c = lane; c < ncols; c += 64) my[2 + PNB + c] = S[c * chunk + (jj - rb)];
+                for (int c = lane; c < ncols; c += 64) my[2 + NB + c] = S[c * chunk + (jj - rb)];
             if (lane == 0) {
-                my[0] = key;
-                my[1] = (T)idx;
+                my[0] = T(key);
+                my[1] = T((K)idx);
                 if (COOP)
                     __hip_atomic_store(&flags[w * 16], gen0 + (unsigned)jj + 1u, __ATOMIC_RELEASE,
                                        __HIP_MEMORY_SCOPE_AGENT);
@@ -198,12 +231,12 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
                 }
                 if (ok) {
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                    T bk = (T)-3;
+                    K bk = (K)-3;
                     int bi = 0x7fffffff;
                     for (int q = lane; q < nwg; q += 64) {
                         const T* sq = slot_ptr(par, q);
-                        const T kq = sq[0];
-                        const int iq = (int)sq[1];
+                        const K kq = slot_val(sq[0]);
+                        const int iq = (int)slot_val(sq[1]);
                         if (better(kq, iq, bk, bi)) {
                             bk = kq;
                             bi = iq;
@@ -211,7 +244,7 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
                     }
 #pragma unroll
                     for (int off = 32; off >= 1; off >>= 1) {
-                        const T ok2 = __shfl_xor(bk, off, 64);
+                        const K ok2 = __shfl_xor(bk, off, 64);
                         const int oi = __shfl_xor(bi, off, 64);
                         if (better(ok2, oi, bk, bi)) {
                             bk = ok2;
@@ -222,7 +255,7 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
                     const T* own = slot_ptr(par, jj / chunk);
                     for (int c = lane; c < ncols; c += 64) {
                         prow_c[c] = win[2 + c];
-                        rj_c[c] = own[2 + PNB + c];
+                        rj_c[c] = own[2 + NB + c];
                     }
                     if (lane == 0) {
                         win_lds = bi;
@@ -239,10 +272,10 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
             __syncthreads();
             const T* sl = &lslot[par][0];
             prow = sl + 2;
-            rj = sl + 2 + PNB;
+            rj = sl + 2 + NB;
             if (t == 0) {
-                win_lds = (int)sl[1];
-                wkey_lds = sl[0];
+                win_lds = (int)slot_val(sl[1]);
+                wkey_lds = slot_val(sl[0]);
             }
             __syncthreads();
             return true;
@@ -250,7 +283,7 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
     };
 
     {
-        T key = has ? pivot_key(S[t], g, 0) : (T)-2;
+        K key = has ? pivot_key(S[t], g, 0) : (K)-2;
         int idx = has ? g : 0x7fffffff;
         block_argmax(key, idx, &red, calls);
         publish(0, key, idx);
@@ -264,7 +297,7 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
             break;
         }
         const int p = win_lds;
-        if (!(wkey_lds > (T)0)) {
+        if (!(wkey_lds > (K)0)) {
             // |pivot| <= 0: singular (src/lapack/mod.rs:136-138); a NaN diagonal carries +inf
             if (w == 0 && t == 0) *info = row0 + j + 1;
             break;
@@ -283,7 +316,7 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
         const bool mine = has && g > j;
         if (j + 1 < ncols) {
             // multipliers of column j and column j+1's update, then the next candidate
-            T key = (T)-2;
+            K key = (K)-2;
             int idx = 0x7fffffff;
             if (mine) {
                 const T f = S[j * chunk + t] / d;
@@ -295,7 +328,7 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
             }
             block_argmax(key, idx, &red, calls);
             // bring the candidate's row and row j+1 up to date, publish, then finish the rest
-            const int lc = (key > (T)-2) ? idx - rb : -1;
+            const int lc = (key > (K)-2) ? idx - rb : -1;
             const int ln = (j + 1 >= rb && j + 1 < rb + nr) ? j + 1 - rb : -1;
             for (int c = j + 2 + t; c < ncols; c += PT) {
                 if (lc >= 0) S[c * chunk + lc] -= S[j * chunk + lc] * prow[c];
@@ -329,13 +362,18 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
 template <typename T>
 int launch_panel(hipStream_t s, Workspace* ws, T* Ppanel, size_t lda, size_t row0, size_t m, size_t ncols,
                  int* ipiv, int* info) {
+    constexpr int NB = PanelElem<T>::NB;
     static bool attr_set = false;
     if (!attr_set) {
         AL_HIP(hipFuncSetAttribute((const void*)panel_getf2<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)(CHUNK_MAX * PNB * sizeof(T))));
+                                   (int)(CHUNK_MAX * NB * sizeof(T))));
         AL_HIP(hipFuncSetAttribute((const void*)panel_getf2<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)(CHUNK_MAX * PNB * sizeof(T))));
+                                   (int)(CHUNK_MAX * NB * sizeof(T))));
         attr_set = true;
+    }
+    if (ncols > (size_t)NB) {
+        set_error("getrf: panel wider than the panel kernel holds");
+        return AL_ERR_ARG;
     }
     size_t nwg = (m + CHUNK_MAX - 1) / CHUNK_MAX;
     if (nwg > (size_t)MAX_WG) nwg = MAX_WG;
@@ -353,19 +391,22 @@ int launch_panel(hipStream_t s, Workspace* ws, T* Ppanel, size_t lda, size_t row
     ws->gen += (unsigned)ncols;
     const int ticket = prof_begin(s);
     if (nwg > 1) {
-        hipLaunchKernelGGL((panel_getf2<T, true>), dim3((unsigned)nwg), dim3(PT), chunk * PNB * sizeof(T), s, Ppanel,
+        hipLaunchKernelGGL((panel_getf2<T, true>), dim3((unsigned)nwg), dim3(PT), chunk * NB * sizeof(T), s, Ppanel,
                            lda, (int)row0, (int)m, (int)ncols, (int)chunk, ipiv, info, ws->flags, ws->abort_flag,
                            (T*)ws->slots, gen0, timeout);
     } else {
-        hipLaunchKernelGGL((panel_getf2<T, false>), dim3(1), dim3(PT), chunk * PNB * sizeof(T), s, Ppanel, lda,
+        hipLaunchKernelGGL((panel_getf2<T, false>), dim3(1), dim3(PT), chunk * NB * sizeof(T), s, Ppanel, lda,
                            (int)row0, (int)m, (int)ncols, (int)chunk, ipiv, info, ws->flags, ws->abort_flag,
                            (T*)ws->slots, gen0, timeout);
     }
     AL_HIP(hipGetLastError());
     if (ticket >= 0) {
-        // unblocked LU of an m x ncols panel: m*ncols^2 - ncols^3/3 flops; the panel read and written once
+        // unblocked LU of an m x ncols panel: m*ncols^2 - ncols^3/3 multiply-adds' worth of flops
+        // (four times that for complex entries); the panel read and written once
         const double dm = (double)m, db = (double)ncols;
-        prof_end(ticket, s, "panel_getf2", dm * db * db - db * db * db / 3.0, 2.0 * sizeof(T) * dm * db);
+        const bool cplx = sizeof(T) == 2 * sizeof(double);
+        prof_end(ticket, s, cplx ? "zpanel_getf2" : "panel_getf2",
+                 (cplx ? 4.0 : 1.0) * (dm * db * db - db * db * db / 3.0), 2.0 * sizeof(T) * dm * db);
     }
     return AL_OK;
 }
@@ -398,6 +439,12 @@ int panel_f32(hipStream_t s, Workspace* ws, float* P, size_t lda, size_t row0, s
               int* info) {
     return launch_panel<float>(s, ws, P, lda, row0, m, ncols, ipiv, info);
 }
+
+int panel_z(hipStream_t s, Workspace* ws, zd* P, size_t lda, size_t row0, size_t m, size_t ncols, int* ipiv,
+            int* info) {
+    return launch_panel<zd>(s, ws, P, lda, row0, m, ncols, ipiv, info);
+}
+int panel_z_width() { return PanelElem<zd>::NB; }
 
 namespace {
 
@@ -572,7 +619,11 @@ int getrs_f64(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const double*
     return AL_OK;
 }
 
-size_t panel_slots_bytes() { return (size_t)2 * MAX_WG * SLOT2 * sizeof(double); }
+size_t panel_slots_bytes() {
+    const size_t real = (size_t)2 * MAX_WG * slot2<double>() * sizeof(double);
+    const size_t cplx = (size_t)2 * MAX_WG * slot2<zd>() * sizeof(zd);
+    return real > cplx ? real : cplx;
+}
 size_t panel_flags_count() { return (size_t)MAX_WG * 16; }
 
 }  // namespace al

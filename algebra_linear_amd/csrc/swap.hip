@@ -231,6 +231,11 @@ int swap_block_f64(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, siz
     return swap_block_t<double>(s, ws, ipiv, k0, jb, A0, n0, A1, n1, lda, perm, info, nullptr);
 }
 
+int swap_block_z(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, size_t jb, zd* A0, size_t n0, zd* A1,
+                 size_t n1, size_t lda, const int* info) {
+    return swap_block_t<zd>(s, ws, ipiv, k0, jb, A0, n0, A1, n1, lda, nullptr, info, nullptr);
+}
+
 int swap_block_f32(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, size_t jb, float* A0, size_t n0,
                    float* A1, size_t n1, size_t lda, const int* info, int* pairs) {
     return swap_block_t<float>(s, ws, ipiv, k0, jb, A0, n0, A1, n1, lda, nullptr, info, pairs);

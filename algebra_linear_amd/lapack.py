@@ -11,7 +11,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from .blas import _in, _out, _p
+from .blas import _in, _out, _p, _zin, _zout, _zp
 from .errors import BAD_SHAPE, BlasPanic, LapackError, check_lapack
 
 _sp = ctypes.POINTER(ctypes.c_size_t)
@@ -120,3 +120,34 @@ def dgesv_mixed(a, b: np.ndarray, n: int, nrhs: int = 1, factor: str = "fp32", o
     check_lapack(_lib.load().al_dgesv_mixed(_p(aa), aa.size, _p(bb), bb.size, n, nrhs, FACTORS[factor],
                                             ctypes.byref(opt), ctypes.byref(st)))
     return {name: getattr(st, name) for name, _ in _lib.IrStats._fields_}
+
+
+# --- complex twins (src/lapack/complex.rs): complex128 arrays -----------------------------
+
+def zgetrf(a: np.ndarray, n: int) -> list:
+    """In-place P*A = L*U of a complex n x n matrix, |Re|+|Im| pivoting; returns the pivots, as the
+    crate's ``Vec<usize>`` (src/lapack/complex.rs:67)."""
+    aa = _zout(a)
+    ipiv = np.zeros(max(n, 1), dtype=np.uintp)
+    check_lapack(_lib.load().al_zgetrf(_zp(aa), aa.size, n, ipiv.ctypes.data_as(_sp), n))
+    return [int(v) for v in ipiv[:n]]
+
+
+def zgetrf_rect(a: np.ndarray, nrows: int, ncols: int, ipiv: np.ndarray) -> None:
+    """Complex panel LU, nrows >= ncols; ``ipiv`` (len ncols) filled (src/lapack/complex.rs:169)."""
+    aa, pv = _zout(a), _piv_out(ipiv)
+    check_lapack(_lib.load().al_zgetrf_rect(_zp(aa), aa.size, nrows, ncols, pv.ctypes.data_as(_sp), pv.size))
+
+
+def zgesv(a: np.ndarray, b: np.ndarray) -> None:
+    """Solve A x = b (one right-hand side, n = len(b)); ``a`` destroyed, ``b`` := x
+    (src/lapack/complex.rs:279)."""
+    aa, bb = _zout(a), _zout(b)
+    check_lapack(_lib.load().al_zgesv(_zp(aa), aa.size, _zp(bb), bb.size))
+
+
+def zpotrf(a: np.ndarray, n: int, upper: bool) -> None:
+    """Complex Cholesky, A = L L^H (lower) or U^H U (upper); only that triangle is read or written
+    (src/lapack/complex.rs:344)."""
+    aa = _zout(a)
+    check_lapack(_lib.load().al_zpotrf(_zp(aa), aa.size, n, int(upper)))

@@ -323,10 +323,62 @@ int al_server_release_factor(al_server* server, int64_t factor_id);
 void al_server_destroy(al_server* server);
 
 /* ------------------------------------------------------------------------------------
+ * Complex twins.  al_complex is the crate's Complex64 (num-complex 0.4 Complex<f64>, a
+ * #[repr(C)] pair {re, im}), passed by value for scalars; complex slices are arrays of it and
+ * their lengths count complex elements.  Semantics, shape checks and messages are the crate's.
+ * ------------------------------------------------------------------------------------ */
+typedef struct al_complex {
+    double re;
+    double im;
+} al_complex;
+
+/* blas::zgemm, src/blas.rs:1023-1028: C := alpha*A*B + beta*C; a m*k, b k*n, c m*n.  Every C
+ * entry becomes alpha*sum + beta*c, beta*c formed even for beta = 0 (src/blas.rs:1115-1116). */
+int al_zgemm(al_complex alpha, const al_complex* a, size_t a_len, const al_complex* b, size_t b_len, al_complex beta,
+             al_complex* c, size_t c_len, size_t m, size_t n, size_t k);
+/* blas::zgemm_tn, src/blas.rs:1042-1047: C := alpha*A^T*B + beta*C (not conjugated); a k*m. */
+int al_zgemm_tn(al_complex alpha, const al_complex* a, size_t a_len, const al_complex* b, size_t b_len,
+                al_complex beta, al_complex* c, size_t c_len, size_t m, size_t n, size_t k);
+/* blas::zgemm_nt, src/blas.rs:1059-1064: C := alpha*A*B^T + beta*C (not conjugated); b n*k. */
+int al_zgemm_nt(al_complex alpha, const al_complex* a, size_t a_len, const al_complex* b, size_t b_len,
+                al_complex beta, al_complex* c, size_t c_len, size_t m, size_t n, size_t k);
+/* blas::zgemm_tt, src/blas.rs:1071-1076: C := alpha*A^T*B^T + beta*C; a k*m, b n*k. */
+int al_zgemm_tt(al_complex alpha, const al_complex* a, size_t a_len, const al_complex* b, size_t b_len,
+                al_complex beta, al_complex* c, size_t c_len, size_t m, size_t n, size_t k);
+
+/* lapack::zgetrf, src/lapack/complex.rs:67-121: in-place P*A = L*U with |Re|+|Im| pivoting;
+ * the crate returns the pivots, here they are written to ipiv (len n).  AL_SINGULAR on a zero
+ * pivot (a then holds the partial factorization). */
+int al_zgetrf(al_complex* a, size_t a_len, size_t n, size_t* ipiv, size_t ipiv_len);
+/* lapack::zgetrf_rect, src/lapack/complex.rs:169-208: the nrows x ncols panel (nrows >= ncols). */
+int al_zgetrf_rect(al_complex* a, size_t a_len, size_t nrows, size_t ncols, size_t* ipiv, size_t ipiv_len);
+/* lapack::zgesv, src/lapack/complex.rs:279-320: solves A x = b for one right-hand side
+ * (n = b_len); a is destroyed (holds the LU factors), b receives x. */
+int al_zgesv(al_complex* a, size_t a_len, al_complex* b, size_t b_len);
+/* lapack::zpotrf, src/lapack/complex.rs:344-400: A = L L^H (upper = 0) or A = U^H U (upper = 1);
+ * only that triangle is read and written.  On AL_SINGULAR the failing pivot value (ajj, 0) is
+ * on the diagonal, as the crate leaves it. */
+int al_zpotrf(al_complex* a, size_t a_len, size_t n, int upper);
+
+/* Device-pointer forms (leading dimensions in complex elements, asynchronous on `stream`):
+ *   al_dev_zgemm: C := alpha op(A) op(B) + beta C, op = transpose without conjugation;
+ *   al_dev_zgetrf: m x n (m >= n) LU, int32 0-based pivots, *d_info as in al_dev_dgetrf;
+ *   al_dev_zgetrs: B (n x nrhs) := A^-1 B from al_dev_zgetrf's factors;
+ *   al_dev_zpotrf: Cholesky of one triangle, *d_info = 1-based failing column or 0. */
+int al_dev_zgemm(int trans_a, int trans_b, size_t m, size_t n, size_t k, al_complex alpha, const al_complex* A,
+                 size_t lda, const al_complex* B, size_t ldb, al_complex beta, al_complex* C, size_t ldc,
+                 void* stream);
+int al_dev_zgetrf(size_t m, size_t n, al_complex* A, size_t lda, int32_t* d_ipiv, int32_t* d_info, void* stream);
+int al_dev_zgetrs(size_t n, size_t nrhs, const al_complex* LU, size_t lda, const int32_t* d_ipiv, al_complex* B,
+                  size_t ldb, void* stream);
+int al_dev_zpotrf(int upper, size_t n, al_complex* A, size_t lda, int32_t* d_info, void* stream);
+
+/* ------------------------------------------------------------------------------------
  * Kernel instrumentation (extension): while enabled, every launch of an instrumented
  * kernel class is bracketed by HIP events on the stream it runs on, with its algorithmic
  * flops and bytes.  Classes: "dgemm_nn", "dgemm_tn", "dgemm_nt", "dgemm_tt",
- * "dsyrk_lower", "dsyrk_upper", "panel_getf2".  al_prof_query synchronizes the recorded
+ * "dsyrk_lower", "dsyrk_upper", "panel_getf2", "zpanel_getf2", "zherk_lower_real_view",
+ * "zherk_upper_real_view".  al_prof_query synchronizes the recorded
  * events and returns the totals for one class.
  * ------------------------------------------------------------------------------------ */
 void al_prof_enable(int on);

@@ -77,6 +77,11 @@ def _declare(L: ctypes.CDLL) -> None:
         "or_latm1": (i, [i, d, i, i, _c_u32_p, _c_double_p, sz]),
         "or_dlatms": (i, [i, i, i, d, d, i, sz, sz, _c_double_p, _c_double_p, _c_u32_p]),
         "or_backward_error": (d, [_c_double_p, _c_double_p, _c_double_p, sz, sz]),
+        "or_zgemm": (i, [i, i, _c_double_p, _c_double_p, _c_double_p, _c_double_p, _c_double_p, sz, sz, sz]),
+        "or_zgetrf": (i, [_c_double_p, sz, _c_size_p]),
+        "or_zgetrf_rect": (i, [_c_double_p, sz, sz, _c_size_p]),
+        "or_zgesv": (i, [_c_double_p, _c_double_p, sz]),
+        "or_zpotrf": (i, [_c_double_p, sz, i]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -299,3 +304,64 @@ def dlatms(dist, sym, mode, cond, dmax, pack, m, n, seed, d=None):
 def backward_error(a, x, b, n, nrhs=1) -> float:
     a, x, b = _f64(a), _f64(x), _f64(b)
     return float(lib().or_backward_error(_dp(a), _dp(x), _dp(b), n, nrhs))
+
+
+# --- complex twins: numpy complex128 arrays (interleaved re, im, exactly Complex64's layout) ---
+
+def _c128(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.complex128).ravel())
+
+
+def _zp(a: np.ndarray):
+    assert a.dtype == np.complex128 and a.flags.c_contiguous
+    return a.ctypes.data_as(_c_double_p)
+
+
+def _zscalar(v) -> np.ndarray:
+    v = complex(v)
+    return np.array([v.real, v.imag], dtype=np.float64)
+
+
+def zgemm(ta: bool, tb: bool, alpha, a, b, beta, c, m, n, k) -> np.ndarray:
+    """``zgemm_core`` of src/blas.rs:1086-1118 (``zgemm``/``_tn``/``_nt``/``_tt``); returns the new C."""
+    a, b, c = _c128(a), _c128(b), _c128(c).copy()
+    assert a.size == m * k and b.size == k * n and c.size == m * n
+    al, be = _zscalar(alpha), _zscalar(beta)
+    lib().or_zgemm(int(ta), int(tb), _dp(al), _zp(a), _zp(b), _dp(be), _zp(c), m, n, k)
+    return c
+
+
+def zgetrf(a, n):
+    """src/lapack/complex.rs:67-121: (status, LU, ipiv); status 0 Ok, 1 Singular."""
+    a = _c128(a).copy()
+    assert a.size == n * n
+    ipiv = np.zeros(max(n, 1), dtype=np.uintp)
+    st = lib().or_zgetrf(_zp(a), n, ipiv.ctypes.data_as(_c_size_p))
+    return st, a, ipiv[:n].astype(np.int64)
+
+
+def zgetrf_rect(a, nrows, ncols):
+    """src/lapack/complex.rs:169-208: (status, LU, ipiv)."""
+    a = _c128(a).copy()
+    assert a.size == nrows * ncols
+    ipiv = np.zeros(max(ncols, 1), dtype=np.uintp)
+    st = lib().or_zgetrf_rect(_zp(a), nrows, ncols, ipiv.ctypes.data_as(_c_size_p))
+    return st, a, ipiv[:ncols].astype(np.int64)
+
+
+def zgesv(a, b):
+    """src/lapack/complex.rs:279-320: (status, x); one right-hand side."""
+    b = _c128(b).copy()
+    n = b.size
+    a = _c128(a).copy()
+    assert a.size == n * n
+    st = lib().or_zgesv(_zp(a), _zp(b), n)
+    return st, b
+
+
+def zpotrf(a, n, upper: bool):
+    """src/lapack/complex.rs:344-400: (status, factor); only the requested triangle is written."""
+    a = _c128(a).copy()
+    assert a.size == n * n
+    st = lib().or_zpotrf(_zp(a), n, int(upper))
+    return st, a

@@ -1038,3 +1038,208 @@ double or_backward_error(const double* a, const double* x, const double* b, size
     free(comp);
     return worst;
 }
+
+/* ------------------------------------------------------------------------- */
+/* complex twins                                                             */
+/* ------------------------------------------------------------------------- */
+
+/*
+ * Complex64 is num-complex 0.4's Complex<f64> (Cargo.toml:32-33; the crate is not vendored
+ * under the reference): two doubles {re, im}, so a complex slice is interleaved re, im pairs.
+ * Its published operators, restated here unfused (rustc does not contract):
+ *   a * b = (a.re*b.re - a.im*b.im, a.re*b.im + a.im*b.re)
+ *   a / b = ((a.re*b.re + a.im*b.im) / q, (a.im*b.re - a.re*b.im) / q), q = b.re*b.re + b.im*b.im
+ *   a / r = (a.re / r, a.im / r) for a real r;  a +- b and a -= b componentwise;
+ *   norm_sqr(a) = a.re*a.re + a.im*a.im;  conj(a) = (a.re, -a.im).
+ * MulAssign and DivAssign expand to the same roundings as * and /.
+ */
+typedef struct {
+    double re, im;
+} zc;
+
+static inline zc z_ld(const double* p, size_t i) {
+    zc r = {p[2 * i], p[2 * i + 1]};
+    return r;
+}
+static inline void z_st(double* p, size_t i, zc v) {
+    p[2 * i] = v.re;
+    p[2 * i + 1] = v.im;
+}
+static inline zc z_mul(zc a, zc b) {
+    zc r = {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+    return r;
+}
+static inline zc z_div(zc a, zc b) {
+    double q = b.re * b.re + b.im * b.im;
+    zc r = {(a.re * b.re + a.im * b.im) / q, (a.im * b.re - a.re * b.im) / q};
+    return r;
+}
+static inline zc z_add(zc a, zc b) {
+    zc r = {a.re + b.re, a.im + b.im};
+    return r;
+}
+static inline zc z_sub(zc a, zc b) {
+    zc r = {a.re - b.re, a.im - b.im};
+    return r;
+}
+static inline zc z_conj(zc a) {
+    zc r = {a.re, -a.im};
+    return r;
+}
+/* complex.rs:52-54: IZAMAX's measure |Re| + |Im| */
+static inline double z_abs1(zc a) { return fabs(a.re) + fabs(a.im); }
+
+/* src/blas.rs:1023-1118 (zgemm, zgemm_tn, zgemm_nt, zgemm_tt -> zgemm_core): the plain triple
+ * loop, op(A) element (i, l) at a[i + l*m] or (transposed, A stored k x m) a[l + i*k], op(B)
+ * element (l, j) at b[l + j*k] or (B stored n x k) b[j + l*n]; never conjugated.  Every C
+ * element becomes alpha*sum + beta*c, beta*c evaluated even when beta is zero. */
+int or_zgemm(int trans_a, int trans_b, const double* alpha, const double* a, const double* b, const double* beta,
+             double* c, size_t m, size_t n, size_t k) {
+    zc al = {alpha[0], alpha[1]}, be = {beta[0], beta[1]};
+    size_t ada = trans_a ? k : m, adb = trans_b ? n : k;
+    for (size_t j = 0; j < n; j++) {
+        for (size_t i = 0; i < m; i++) {
+            zc sum = {0.0, 0.0};
+            for (size_t l = 0; l < k; l++) {
+                zc av = trans_a ? z_ld(a, l + i * ada) : z_ld(a, i + l * ada);
+                zc bv = trans_b ? z_ld(b, j + l * adb) : z_ld(b, l + j * adb);
+                sum = z_add(sum, z_mul(av, bv));
+            }
+            size_t idx = i + j * m;
+            z_st(c, idx, z_add(z_mul(al, sum), z_mul(be, z_ld(c, idx))));
+        }
+    }
+    return 0;
+}
+
+/* src/lapack/complex.rs:67-121 (zgetf2): first maximum of |Re|+|Im| from the diagonal down,
+ * Singular when it is <= 0 (ZERO_DETERMINANT, src/lapack/mod.rs:73), whole-row swap, the column
+ * divided by the pivot, then the rank-1 update column by column.  ipiv 0-based. */
+int or_zgetrf(double* a, size_t n, size_t* ipiv) {
+    for (size_t j = 0; j < n; j++) {
+        double best = z_abs1(z_ld(a, j + j * n));
+        size_t jp = j;
+        for (size_t i = j + 1; i < n; i++) {
+            double cand = z_abs1(z_ld(a, i + j * n));
+            if (cand > best) {
+                best = cand;
+                jp = i;
+            }
+        }
+        ipiv[j] = jp;
+        if (best <= 0.0) return 1;
+        if (jp != j) {
+            for (size_t k = 0; k < n; k++) {
+                zc t = z_ld(a, k * n + j);
+                z_st(a, k * n + j, z_ld(a, k * n + jp));
+                z_st(a, k * n + jp, t);
+            }
+        }
+        zc d = z_ld(a, j + j * n);
+        for (size_t i = j + 1; i < n; i++) z_st(a, i + j * n, z_div(z_ld(a, i + j * n), d));
+        for (size_t k = j + 1; k < n; k++) {
+            zc ajk = z_ld(a, j + k * n);
+            for (size_t i = j + 1; i < n; i++)
+                z_st(a, i + k * n, z_sub(z_ld(a, i + k * n), z_mul(z_ld(a, i + j * n), ajk)));
+        }
+    }
+    return 0;
+}
+
+/* src/lapack/complex.rs:169-208: the rectangular panel (nrows >= ncols), factor = a/d then
+ * a[i][j] -= factor * a[k][j] row by row. */
+int or_zgetrf_rect(double* a, size_t nrows, size_t ncols, size_t* ipiv) {
+    if (nrows < ncols) return 2;
+    for (size_t k = 0; k < ncols; k++) {
+        size_t pivot = k;
+        double best = z_abs1(z_ld(a, k + k * nrows));
+        for (size_t i = k + 1; i < nrows; i++) {
+            double cand = z_abs1(z_ld(a, i + k * nrows));
+            if (cand > best) {
+                best = cand;
+                pivot = i;
+            }
+        }
+        if (best <= 0.0) return 1;
+        if (pivot != k) {
+            for (size_t j = 0; j < ncols; j++) {
+                zc t = z_ld(a, j * nrows + k);
+                z_st(a, j * nrows + k, z_ld(a, j * nrows + pivot));
+                z_st(a, j * nrows + pivot, t);
+            }
+        }
+        ipiv[k] = pivot;
+        zc d = z_ld(a, k + k * nrows);
+        for (size_t i = k + 1; i < nrows; i++) {
+            zc factor = z_div(z_ld(a, i + k * nrows), d);
+            z_st(a, i + k * nrows, factor);
+            for (size_t j = k + 1; j < ncols; j++)
+                z_st(a, i + j * nrows, z_sub(z_ld(a, i + j * nrows), z_mul(factor, z_ld(a, k + j * nrows))));
+        }
+    }
+    return 0;
+}
+
+/* src/lapack/complex.rs:279-320: zgetrf, the interchanges applied to b in order, the unit-lower
+ * forward sweep and the upper back substitution (one right-hand side). */
+int or_zgesv(double* a, double* b, size_t n) {
+    if (n == 0) return 0;
+    size_t* ipiv = (size_t*)calloc(n, sizeof(size_t));
+    if (!ipiv) return -1;
+    int st = or_zgetrf(a, n, ipiv);
+    if (st == 0) {
+        for (size_t k = 0; k < n; k++) {
+            zc t = z_ld(b, k);
+            z_st(b, k, z_ld(b, ipiv[k]));
+            z_st(b, ipiv[k], t);
+        }
+        for (size_t i = 0; i < n; i++) {
+            zc sum = z_ld(b, i);
+            for (size_t j = 0; j < i; j++) sum = z_sub(sum, z_mul(z_ld(a, i + j * n), z_ld(b, j)));
+            z_st(b, i, sum);
+        }
+        for (size_t ii = n; ii-- > 0;) {
+            zc sum = z_ld(b, ii);
+            for (size_t j = ii + 1; j < n; j++) sum = z_sub(sum, z_mul(z_ld(a, ii + j * n), z_ld(b, j)));
+            z_st(b, ii, z_div(sum, z_ld(a, ii + ii * n)));
+        }
+    }
+    free(ipiv);
+    return st;
+}
+
+/* src/lapack/complex.rs:344-400 (zpotf2): upper -> A = U^H U reading and writing i <= j only;
+ * lower -> A = L L^H reading and writing i >= j only.  The diagonal is Re(a_jj) minus the
+ * squared moduli; a non-positive or NaN value is written to the diagonal (imaginary part 0)
+ * before Singular is returned. */
+int or_zpotrf(double* a, size_t n, int upper) {
+    for (size_t j = 0; j < n; j++) {
+        double ajj = a[2 * (j + j * n)];
+        for (size_t q = 0; q < j; q++) {
+            zc v = upper ? z_ld(a, q + j * n) : z_ld(a, j + q * n);
+            ajj -= v.re * v.re + v.im * v.im;
+        }
+        if (ajj <= 0.0 || isnan(ajj)) {
+            zc bad = {ajj, 0.0};
+            z_st(a, j + j * n, bad);
+            return 1;
+        }
+        ajj = sqrt(ajj);
+        zc dg = {ajj, 0.0};
+        z_st(a, j + j * n, dg);
+        for (size_t o = j + 1; o < n; o++) {
+            if (upper) {
+                zc sum = z_ld(a, j + o * n);
+                for (size_t i = 0; i < j; i++) sum = z_sub(sum, z_mul(z_ld(a, i + o * n), z_conj(z_ld(a, i + j * n))));
+                zc r = {sum.re / ajj, sum.im / ajj};
+                z_st(a, j + o * n, r);
+            } else {
+                zc sum = z_ld(a, o + j * n);
+                for (size_t q = 0; q < j; q++) sum = z_sub(sum, z_mul(z_ld(a, o + q * n), z_conj(z_ld(a, j + q * n))));
+                zc r = {sum.re / ajj, sum.im / ajj};
+                z_st(a, o + j * n, r);
+            }
+        }
+    }
+    return 0;
+}

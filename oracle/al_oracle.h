@@ -79,6 +79,15 @@ int or_dlatms(int dist, int sym, int mode, double cond, double dmax, int pack, s
  * Neumaier summation in the residual (not a reference routine). */
 double or_backward_error(const double* a, const double* x, const double* b, size_t n, size_t nrhs);
 
+/* ---- complex twins (src/blas.rs:1023-1118, src/lapack/complex.rs:67-400); complex data are
+ * interleaved (re, im) double pairs, lengths and indices count complex elements ---- */
+int or_zgemm(int trans_a, int trans_b, const double* alpha, const double* a, const double* b, const double* beta,
+             double* c, size_t m, size_t n, size_t k);
+int or_zgetrf(double* a, size_t n, size_t* ipiv);
+int or_zgetrf_rect(double* a, size_t nrows, size_t ncols, size_t* ipiv);
+int or_zgesv(double* a, double* b, size_t n);
+int or_zpotrf(double* a, size_t n, int upper);
+
 #ifdef __cplusplus
 }
 #endif

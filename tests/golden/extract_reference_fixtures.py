@@ -13,7 +13,14 @@ no source text is stored.  What is extracted, and from where:
 * ``larnv``: LAPACK ``libtmglib`` stream values for seed [1988,1989,1990,1991], n = 200
   (first twelve, last two, seed after) -- src/lapack/families.rs:1182-1258;
 * ``dlatms``: the twelve ``DLATMS`` matrices with the seed after each call --
-  src/lapack/families.rs:2013-2470 (compared at 1e-13 relative, seeds exactly).
+  src/lapack/families.rs:2013-2470 (compared at 1e-13 relative, seeds exactly);
+* ``zgemm_{tn,nt,tt}``: the 2x3x4 CBLAS-captured complex products at alpha = 2 - i,
+  beta = 0.5 + 0.25i -- src/blas.rs:2029-2195 (compared at 1e-13 relative to max(|want|, 1),
+  src/blas.rs:2386-2389); complex numbers are stored as [re, im] pairs;
+* ``zlapack``: the complex LU/Cholesky test matrices -- ``spd3`` and ``general3``
+  (src/lapack/complex.rs:422-451), the ``zgesv`` known solution (:483-494), the exactly
+  singular 2x2 (:500-508), the non-positive-definite 2x2 (:577-581) and the 3x2 ``zgetrf_rect``
+  panel (:598-608).
 """
 from __future__ import annotations
 
@@ -124,9 +131,71 @@ def dlatms_fixture(fam: str) -> dict:
     return {"seed": [1988, 1989, 1990, 1991], "cases": cases}
 
 
+CPX = re.compile(r"cpx!\(\s*(" + NUM + r")\s*,\s*(" + NUM + r")\s*\)")
+
+
+def cpx_list(text: str) -> list[list[float]]:
+    text = re.sub(r"//[^\n]*", "", text)
+    return [[float(a), float(b)] for a, b in CPX.findall(text)]
+
+
+def bracket_after(src: str, start: int, pattern: str) -> tuple[str, int]:
+    """Text inside the first ``[...]`` after the regex ``pattern`` (searched from ``start``)."""
+    m = re.compile(pattern).search(src, start)
+    if m is None:
+        raise KeyError(pattern)
+    i = src.index("[", m.end() - 1)
+    depth, j = 1, i + 1
+    while depth:
+        depth += {"[": 1, "]": -1}.get(src[j], 0)
+        j += 1
+    return src[i + 1 : j - 1], j
+
+
+def zgemm_fixture(blas: str, fn_name: str) -> dict:
+    s, e = fn_body(blas, fn_name)
+    body = blas[s:e]
+    out = {}
+    pos = 0
+    for nm in ("a", "b", "c0", "want"):
+        txt, pos = bracket_after(body, pos, r"let\s+" + nm + r"\s*:[^=]*=\s*&\[")
+        out[nm] = cpx_list(txt)
+    mnk = re.search(r"let \(m, n, k\) = \((\d+)usize, (\d+)usize, (\d+)usize\)", body)
+    out["m"], out["n"], out["k"] = (int(x) for x in mnk.groups())
+    out["alpha"] = cpx_list(re.search(r"let alpha = (cpx!\([^)]*\))", body).group(1))[0]
+    out["beta"] = cpx_list(re.search(r"let beta = (cpx!\([^)]*\))", body).group(1))[0]
+    return out
+
+
+def zlapack_fixture(cx: str) -> dict:
+    out = {}
+    for nm in ("spd3", "general3"):
+        s, e = fn_body(cx, nm)
+        out[nm] = cpx_list(cx[s:e])
+        assert len(out[nm]) == 9, nm
+    s, e = fn_body(cx, "zgesv_solves_a_system")
+    txt, _ = bracket_after(cx[s:e], 0, r"let x_true = \[")
+    out["zgesv_x_true"] = cpx_list(txt)
+    s, e = fn_body(cx, "zgetrf_detects_a_singular_matrix")
+    txt, _ = bracket_after(cx[s:e], 0, r"let mut a = vec!\[")
+    out["singular2"] = cpx_list(txt)
+    s, e = fn_body(cx, "zpotrf_rejects_a_non_positive_definite_matrix")
+    txt, _ = bracket_after(cx[s:e], 0, r"let mut a = vec!\[")
+    out["not_pd2"] = cpx_list(txt)
+    s, e = fn_body(cx, "zgetrf_rect_reconstructs_a_tall_panel")
+    txt, _ = bracket_after(cx[s:e], 0, r"let a0 = vec!\[")
+    out["rect3x2"] = cpx_list(txt)
+    shp = re.search(r"let \(nrows, ncols\) = \((\d+), (\d+)\)", cx[s:e])
+    out["rect_shape"] = [int(shp.group(1)), int(shp.group(2))]
+    for key, ln in (("zgesv_x_true", 3), ("singular2", 4), ("not_pd2", 4), ("rect3x2", 6)):
+        assert len(out[key]) == ln, (key, len(out[key]))
+    return out
+
+
 def main(root: str) -> None:
     blas = open(os.path.join(root, "src", "blas.rs")).read()
     fam = open(os.path.join(root, "src", "lapack", "families.rs")).read()
+    cx = open(os.path.join(root, "src", "lapack", "complex.rs")).read()
     out = {
         "provenance": "numeric test vectors of cpmech/algebra_linear; see the docstring of "
                       "tests/golden/extract_reference_fixtures.py for file:line origins",
@@ -136,6 +205,10 @@ def main(root: str) -> None:
         "dgemm_tn_exact": exact_fixture(blas),
         "larnv": larnv_fixture(fam),
         "dlatms": dlatms_fixture(fam),
+        "zgemm_tn": zgemm_fixture(blas, "zgemm_tn_matches_cblas"),
+        "zgemm_nt": zgemm_fixture(blas, "zgemm_nt_matches_cblas"),
+        "zgemm_tt": zgemm_fixture(blas, "zgemm_tt_matches_cblas"),
+        "zlapack": zlapack_fixture(cx),
     }
     dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_fixtures.json")
     with open(dst, "w") as f:

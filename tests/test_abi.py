@@ -86,3 +86,35 @@ def test_python_mirror_raises_the_crates_error_kinds():
     assert e.value.kind == al.BAD_SHAPE
     with pytest.raises(TypeError):
         al.lapack.dgesv([1.0, 2.0, 3.0, 4.0], np.zeros(2))  # outputs must be ndarrays (&mut [f64])
+
+
+def test_complex_shape_errors_and_empty_inputs():
+    # src/blas.rs:1024-1074 (panics with these texts), src/lapack/complex.rs:67-347 (BadShape)
+    import algebra_linear_amd as al
+    lib = _lib.load()
+    C = _lib.Complex.of
+    z = np.zeros(6, dtype=np.complex128)
+    zp = z.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    assert lib.al_zgemm(C(1), zp, 5, zp, 6, C(0), zp, 4, 2, 2, 3) == _lib.AL_BAD_SHAPE
+    assert _lib.last_error() == "zgemm: a must be m*k"
+    assert lib.al_zgemm_tn(C(1), zp, 6, zp, 5, C(0), zp, 4, 2, 2, 3) == _lib.AL_BAD_SHAPE
+    assert _lib.last_error() == "zgemm_tn: b must be k*n"
+    assert lib.al_zgemm_nt(C(1), zp, 6, zp, 5, C(0), zp, 4, 2, 2, 3) == _lib.AL_BAD_SHAPE
+    assert _lib.last_error() == "zgemm_nt: b must be n*k"
+    assert lib.al_zgemm_tt(C(1), zp, 6, zp, 6, C(0), zp, 5, 2, 2, 3) == _lib.AL_BAD_SHAPE
+    assert _lib.last_error() == "zgemm_tt: c must be m*n"
+    with pytest.raises(al.BlasPanic, match="zgemm_tn: a must be k\\*m"):
+        al.blas.zgemm_tn(1.0, np.zeros(5, complex), np.zeros(6, complex), 0.0, np.zeros(4, complex), 2, 2, 3)
+    for call in (lambda: al.lapack.zgetrf(np.zeros(3, complex), 2),
+                 lambda: al.lapack.zgesv(np.zeros(3, complex), np.zeros(2, complex)),
+                 lambda: al.lapack.zpotrf(np.zeros(3, complex), 2, False),
+                 lambda: al.lapack.zgetrf_rect(np.zeros(4, complex), 1, 2, np.zeros(2, dtype=np.uintp))):
+        with pytest.raises(al.LapackError) as e:
+            call()
+        assert e.value.kind == al.BAD_SHAPE
+    # n = 0 is a no-op everywhere (src/lapack/complex.rs:611-619)
+    assert al.lapack.zgetrf(np.zeros(0, complex), 0) == []
+    al.lapack.zgesv(np.zeros(0, complex), np.zeros(0, complex))
+    al.lapack.zpotrf(np.zeros(0, complex), 0, True)
+    with pytest.raises(TypeError):
+        al.lapack.zgesv(np.zeros(4, complex), np.zeros(2))  # b must be complex128 (&mut [Complex64])

@@ -7,10 +7,14 @@ are generated on the host with numpy.  Workloads (--what, comma-separated):
             A uniform(-1, 1) (the kernels' launch shapes do not depend on the values);
   dgemm  -- al_dev_dgemm nn, m = n = k = 8192, uniform(-1, 1);
   batch  -- al_dev_batch_solve in FP64 on `--systems` systems of the batched workload's size
-            distribution (orders log-uniform in [8, 512], half dgesv, half dposv, 1..8 rhs).
+            distribution (orders log-uniform in [8, 512], half dgesv, half dposv, 1..8 rhs);
+  zgemm  -- al_dev_zgemm nn, m = n = k = 4096, entries uniform in the unit square;
+  zgetrf -- al_dev_zgetrf, n = 8192, entries uniform in the unit square;
+  zpotrf -- al_dev_zpotrf lower, n = 8192, a Hermitian matrix with entries in the unit square
+            plus 2n I (positive definite by Gershgorin's theorem).
 Each runs --reps times after one warm-up.
 
-  python -m algebra_linear_amd.profile_kernels --what mixed,dgemm,batch
+  python -m algebra_linear_amd.profile_kernels --what mixed,dgemm,batch,zgemm,zgetrf,zpotrf
 """
 from __future__ import annotations
 
@@ -149,6 +153,54 @@ def main() -> int:
         info = dev.download(dInfo, np.zeros(count, dtype=np.int32))
         if np.any(info != 0):
             raise RuntimeError("a batched system failed")
+        dev.close()
+    one, zero = _lib.Complex.of(1.0), _lib.Complex.of(0.0)
+
+    def cunit(count):
+        return rng.uniform(-1.0, 1.0, count) + 1j * rng.uniform(-1.0, 1.0, count)
+    if "zgemm" in what:
+        n = 4096
+        dA, dB = dev.upload(cunit(n * n)), dev.upload(cunit(n * n))
+        dC = dev.alloc(16 * n * n)
+
+        def zgemm():
+            if lib.al_dev_zgemm(0, 0, n, n, n, one, dA, n, dB, n, zero, dC, n, None) != 0:
+                raise RuntimeError(_lib.last_error())
+        run(zgemm, args.reps, dev, f"zgemm nn {n}")
+        dev.close()
+    if "zgetrf" in what or "zpotrf" in what:
+        n = 8192
+        dInfo = dev.alloc(4)
+        dPiv = dev.alloc(4 * n)
+        if "zgetrf" in what:
+            A = cunit(n * n)
+            dA0, dA = dev.upload(A), dev.alloc(A.nbytes)
+
+            def zgetrf():
+                if dev.hip.hipMemcpy(dA, dA0, A.nbytes, 3) != 0:
+                    raise RuntimeError("hipMemcpy")
+                if lib.al_dev_zgetrf(n, n, dA, n, dPiv, dInfo, None) != 0:
+                    raise RuntimeError(_lib.last_error())
+            run(zgetrf, args.reps, dev, f"zgetrf {n}")
+            if dev.download(dInfo, np.zeros(1, dtype=np.int32))[0] != 0:
+                raise RuntimeError("zgetrf reported a zero pivot")
+            del A
+        if "zpotrf" in what:
+            H = cunit(n * n).reshape(n, n)
+            H = 0.5 * (H + H.conj().T) + 2.0 * n * np.eye(n)
+            A = np.ascontiguousarray(H.T).reshape(-1)
+            del H
+            dA0, dA = dev.upload(A), dev.alloc(A.nbytes)
+
+            def zpotrf():
+                if dev.hip.hipMemcpy(dA, dA0, A.nbytes, 3) != 0:
+                    raise RuntimeError("hipMemcpy")
+                if lib.al_dev_zpotrf(0, n, dA, n, dInfo, None) != 0:
+                    raise RuntimeError(_lib.last_error())
+            run(zpotrf, args.reps, dev, f"zpotrf lower {n}")
+            if dev.download(dInfo, np.zeros(1, dtype=np.int32))[0] != 0:
+                raise RuntimeError("zpotrf reported a non-positive pivot")
+            del A
         dev.close()
     return 0
 

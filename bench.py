@@ -39,6 +39,9 @@ second of wall time of a step.  Prints ONE JSON line; extra objects:
                   request stream solved one drop-in call at a time;
   dropin_fp64  -- the plain FP64 drop-in configuration: dgemm nn/tn/nt/tt at 1024, 4096 and
                   8192, dgetrf at 8192, dgetrf_rect on 16384 x 2048, dpotrf at 8192;
+  dropin_complex -- the complex twins at the same sizes: zgemm nn/tn/nt/tt at 4096 and nn at
+                  8192, zgetrf at 8192, zgetrf_rect on 16384 x 2048, zpotrf lower and upper at
+                  8192, zgesv at 8192 (complex flops: 4x the real counts);
   sweep        -- the mixed configuration's grid: n in {4096, 16384}, kappa in {1e2, 1e5, 1e8,
                   1e11}, nrhs in {1, 16}, FP32 and bf16 factors, beside the FP64 path;
   cpu_baseline -- the CPU oracle (a C restatement of the crate's unblocked dgesv) timed on a
@@ -102,7 +105,7 @@ def cpu_baseline(sample_n: int) -> dict:
 
     seed = [1988, 1989, 1990, 1991]
     out = {"cpu_model": cpu_model(), "host_cores": os.cpu_count(), "threads_used": 1, "config1": {}, "dgemm": [],
-           "dgetrf": []}
+           "dgetrf": [], "complex": {}}
     # the CPU-runnable configuration, in full: uniform(-1, 1) from the larnv stream
     n = 512
     ab = O.larnv(O.SYMMETRIC, seed, 3 * n * n)[0]
@@ -126,6 +129,17 @@ def cpu_baseline(sample_n: int) -> dict:
         x = rng.uniform(-1.0, 1.0, n * n)
         _, dt = timed(lambda: O.dgetrf(x, n))
         out["dgetrf"].append({"n": n, "s": round(dt, 3), "gflops": round(2.0 / 3.0 * n ** 3 / dt / 1e9, 3)})
+    # the complex twins' naive loops (src/blas.rs:1086-1118, src/lapack/complex.rs:67-121, :344-400)
+    n = 512
+    z = rng.uniform(-1.0, 1.0, (3, n * n)) + 1j * rng.uniform(-1.0, 1.0, (3, n * n))
+    _, dt = timed(lambda: O.zgemm(False, False, 1.0, z[0], z[1], 0.0, z[2], n, n, n))
+    out["complex"]["zgemm_512"] = {"s": round(dt, 3), "gflops": round(8.0 * n ** 3 / dt / 1e9, 3)}
+    (st, _, _), dt = timed(lambda: O.zgetrf(z[0], n))
+    out["complex"]["zgetrf_512"] = {"s": round(dt, 3), "gflops": round(8.0 / 3.0 * n ** 3 / dt / 1e9, 3)}
+    H = z[0].reshape(n, n)
+    H = H @ H.conj().T + n * np.eye(n)
+    (st, _), dt = timed(lambda: O.zpotrf(H.ravel(), n, False))
+    out["complex"]["zpotrf_512"] = {"s": round(dt, 3), "gflops": round(4.0 / 3.0 * n ** 3 / dt / 1e9, 3)}
     a = rng.uniform(-1.0, 1.0, sample_n * sample_n)
     b = rng.uniform(-1.0, 1.0, sample_n)
     (st, _, _), dt = timed(lambda: O.dgesv(a, b))   # dgetrf + one dgetrs: the factorization's time
@@ -533,6 +547,108 @@ def run_dropin(lib, stream, sp) -> dict:
     return out
 
 
+def run_dropin_complex(lib, _lib, stream, sp) -> dict:
+    """The complex twins on device-resident complex128 data (interleaved re, im = Complex64):
+    zgemm nn/tn/nt/tt at m = n = k = 4096 and nn at 8192 (entries uniform in the unit square),
+    zgetrf of an order-8192 matrix, zgetrf_rect of a 16384 x 2048 panel, zpotrf (both triangles)
+    of an order-8192 Hermitian positive definite G G^H + n I, and zgesv (zgetrf + the solve) at
+    8192 with one right-hand side; one warm-up and two timed calls each, inputs restored inside
+    the timed region.  Flops are the complex counts: 8mnk for a product, 8/3 n^3 for the LU,
+    4/3 n^3 for the Cholesky."""
+    import torch
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(11)
+    C1 = _lib.Complex.of(1.0)
+    C0 = _lib.Complex.of(0.0)
+
+    def crand(count):
+        re = torch.rand(count, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+        im = torch.rand(count, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+        return torch.complex(re, im)
+
+    def timed(step, reps=2) -> float:
+        step()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            step()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    out = {"zgemm": []}
+    for n, ops in ((4096, ((0, 0, "nn"), (1, 0, "tn"), (0, 1, "nt"), (1, 1, "tt"))), (8192, ((0, 0, "nn"),))):
+        A, B = crand(n * n), crand(n * n)
+        C = torch.zeros(n * n, dtype=torch.complex128, device=dev)
+        for ta, tb, name in ops:
+            def step():
+                st = lib.al_dev_zgemm(ta, tb, n, n, n, C1, A.data_ptr(), n, B.data_ptr(), n, C0, C.data_ptr(), n, sp)
+                if st != 0:
+                    raise RuntimeError(f"zgemm {st}")
+            ms = timed(step)
+            out["zgemm"].append({"n": n, "op": name, "ms": round(ms, 3),
+                                 "tflops": round(8.0 * n ** 3 / (ms * 1e-3) / 1e12, 2)})
+        del A, B, C
+    ipiv = torch.empty(16384, dtype=torch.int32, device=dev)
+    info = torch.zeros(1, dtype=torch.int32, device=dev)
+    for m, n, name in ((8192, 8192, "zgetrf"), (16384, 2048, "zgetrf_rect")):
+        A0 = crand(m * n)
+        A = torch.empty_like(A0)
+
+        def step():
+            A.copy_(A0)
+            st = lib.al_dev_zgetrf(m, n, A.data_ptr(), m, ipiv.data_ptr(), info.data_ptr(), sp)
+            if st != 0:
+                raise RuntimeError(f"{name} {st}")
+        ms = timed(step)
+        k = min(m, n)
+        fl = 8.0 * (m * n * k - (m + n) * k * k / 2.0 + k ** 3 / 3.0)  # 4x LAWN 41's real count
+        out[name] = {"m": m, "n": n, "ms": round(ms, 3), "tflops": round(fl / (ms * 1e-3) / 1e12, 2),
+                     "status": int(info.item())}
+        if name == "zgetrf":
+            b0 = crand(m)
+            b = torch.empty_like(b0)
+
+            def solve():
+                A.copy_(A0)
+                b.copy_(b0)
+                st = lib.al_dev_zgetrf(m, m, A.data_ptr(), m, ipiv.data_ptr(), info.data_ptr(), sp)
+                st = st or lib.al_dev_zgetrs(m, 1, A.data_ptr(), m, ipiv.data_ptr(), b.data_ptr(), m, sp)
+                if st != 0:
+                    raise RuntimeError(f"zgesv {st}")
+            ms = timed(solve)
+            torch.cuda.synchronize()
+            Am = A0.view(m, m).T  # column-major storage: element (i, j) at [i + j*m]
+            r = (b0 - Am @ b).abs().max().item()
+            berr = r / (Am.abs().sum(dim=1).max().item() * b.abs().max().item() + b0.abs().max().item())
+            out["zgesv"] = {"n": m, "nrhs": 1, "ms": round(ms, 3),
+                            "tflops": round((8.0 / 3.0 * m ** 3 + 8.0 * m * m) / (ms * 1e-3) / 1e12, 2),
+                            "status": int(info.item()), "backward_error": berr}
+            del b0, b, Am
+        del A0, A
+    n = 8192
+    G = crand(n * n).view(n, n)
+    S0 = (G @ G.conj().T + n * torch.eye(n, dtype=torch.complex128, device=dev)).T.contiguous().reshape(-1)
+    del G
+    S = torch.empty_like(S0)
+    for upper in (0, 1):
+        def step():
+            S.copy_(S0)
+            st = lib.al_dev_zpotrf(upper, n, S.data_ptr(), n, info.data_ptr(), sp)
+            if st != 0:
+                raise RuntimeError(f"zpotrf {st}")
+        ms = timed(step)
+        out["zpotrf_upper" if upper else "zpotrf_lower"] = {
+            "n": n, "ms": round(ms, 3), "tflops": round(4.0 / 3.0 * n ** 3 / (ms * 1e-3) / 1e12, 2),
+            "status": int(info.item())}
+    del S0, S
+    torch.cuda.empty_cache()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -736,6 +852,7 @@ def main() -> int:
     torch.cuda.empty_cache()
     if not args.no_dropin:
         out["dropin_fp64"] = run_dropin(lib, stream, sp)
+        out["dropin_complex"] = run_dropin_complex(lib, _lib, stream, sp)
     if args.sweep:
         out["sweep"] = run_sweep([int(v) for v in args.sweep.split(",")], [1e2, 1e5, 1e8, 1e11], [1, 16],
                                  ["fp32", "bf16"], lib, _lib, lapack, stream, sp)

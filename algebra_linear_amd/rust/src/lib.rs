@@ -51,7 +51,7 @@ fn blas_status(st: c_int) {
 
 pub mod blas {
     //! `algebra_linear::blas` routines of the solve path (src/blas.rs).
-    use super::{blas_status, sys};
+    use super::{blas_status, sys, Complex64};
     use std::os::raw::c_int;
 
     /// `C := alpha A B + beta C`; `a` m x k, `b` k x n, `c` m x n (src/blas.rs:201-206).
@@ -139,11 +139,46 @@ pub mod blas {
             sys::al_dtrsv_transpose(a.as_ptr(), a.len(), x.as_mut_ptr(), x.len(), n, lower as c_int, unit as c_int)
         });
     }
+
+    /// `C := alpha A B + beta C` for complex data; `a` m x k, `b` k x n (src/blas.rs:1023-1028).
+    pub fn zgemm(alpha: Complex64, a: &[Complex64], b: &[Complex64], beta: Complex64, c: &mut [Complex64], m: usize,
+                 n: usize, k: usize) {
+        blas_status(unsafe {
+            sys::al_zgemm(alpha, a.as_ptr(), a.len(), b.as_ptr(), b.len(), beta, c.as_mut_ptr(), c.len(), m, n, k)
+        });
+    }
+
+    /// `C := alpha A^T B + beta C`, not conjugated; `a` k x m (src/blas.rs:1042-1047).
+    pub fn zgemm_tn(alpha: Complex64, a: &[Complex64], b: &[Complex64], beta: Complex64, c: &mut [Complex64],
+                    m: usize, n: usize, k: usize) {
+        blas_status(unsafe {
+            sys::al_zgemm_tn(alpha, a.as_ptr(), a.len(), b.as_ptr(), b.len(), beta, c.as_mut_ptr(), c.len(), m, n, k)
+        });
+    }
+
+    /// `C := alpha A B^T + beta C`, not conjugated; `b` n x k (src/blas.rs:1059-1064).
+    pub fn zgemm_nt(alpha: Complex64, a: &[Complex64], b: &[Complex64], beta: Complex64, c: &mut [Complex64],
+                    m: usize, n: usize, k: usize) {
+        blas_status(unsafe {
+            sys::al_zgemm_nt(alpha, a.as_ptr(), a.len(), b.as_ptr(), b.len(), beta, c.as_mut_ptr(), c.len(), m, n, k)
+        });
+    }
+
+    /// `C := alpha A^T B^T + beta C`; `a` k x m, `b` n x k (src/blas.rs:1071-1076).
+    pub fn zgemm_tt(alpha: Complex64, a: &[Complex64], b: &[Complex64], beta: Complex64, c: &mut [Complex64],
+                    m: usize, n: usize, k: usize) {
+        blas_status(unsafe {
+            sys::al_zgemm_tt(alpha, a.as_ptr(), a.len(), b.as_ptr(), b.len(), beta, c.as_mut_ptr(), c.len(), m, n, k)
+        });
+    }
 }
 
+/// The crate's complex scalar (`algebra_linear::Complex64`, num-complex 0.4).
+pub use num_complex::Complex64;
+
 pub mod lapack {
-    //! `algebra_linear::lapack` drivers of the solve path (src/lapack/mod.rs).
-    use super::{last_error, sys};
+    //! `algebra_linear::lapack` drivers of the solve path (src/lapack/mod.rs, src/lapack/complex.rs).
+    use super::{last_error, sys, Complex64};
     use std::os::raw::c_int;
 
     /// The crate's error kinds (src/lapack/mod.rs:77-96).
@@ -266,5 +301,33 @@ pub mod lapack {
                                 &mut stats)
         })?;
         Ok(stats)
+    }
+
+    /// Complex LU with |Re| + |Im| partial pivoting in place; returns the 0-based pivots
+    /// (src/lapack/complex.rs:67-121).
+    pub fn zgetrf(a: &mut [Complex64], n: usize) -> Result<Vec<usize>, LapackError> {
+        if a.len() != n * n {
+            return Err(LapackError::BadShape);
+        }
+        let mut ipiv = vec![0usize; n];
+        result(unsafe { sys::al_zgetrf(a.as_mut_ptr(), a.len(), n, ipiv.as_mut_ptr(), ipiv.len()) })?;
+        Ok(ipiv)
+    }
+
+    /// Complex panel LU, nrows >= ncols (src/lapack/complex.rs:169-208).
+    pub fn zgetrf_rect(a: &mut [Complex64], nrows: usize, ncols: usize, ipiv: &mut [usize]) -> Result<(), LapackError> {
+        result(unsafe { sys::al_zgetrf_rect(a.as_mut_ptr(), a.len(), nrows, ncols, ipiv.as_mut_ptr(), ipiv.len()) })
+    }
+
+    /// `A x = b` for one complex right-hand side, `n = b.len()`; `a` is destroyed
+    /// (src/lapack/complex.rs:279-320).
+    pub fn zgesv(a: &mut [Complex64], b: &mut [Complex64]) -> Result<(), LapackError> {
+        result(unsafe { sys::al_zgesv(a.as_mut_ptr(), a.len(), b.as_mut_ptr(), b.len()) })
+    }
+
+    /// Complex Cholesky, `A = L L^H` (`upper == false`) or `A = U^H U`; only that triangle is read
+    /// or written (src/lapack/complex.rs:344-400).
+    pub fn zpotrf(a: &mut [Complex64], n: usize, upper: bool) -> Result<(), LapackError> {
+        result(unsafe { sys::al_zpotrf(a.as_mut_ptr(), a.len(), n, upper as c_int) })
     }
 }

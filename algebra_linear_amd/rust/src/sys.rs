@@ -37,6 +37,9 @@ pub struct al_ir_stats {
     pub backward_error_low: f64,
 }
 
+/// `al_complex`: layout-identical to `num_complex::Complex64` (`#[repr(C)]` {re, im}).
+pub type al_complex = num_complex::Complex64;
+
 extern "C" {
     pub fn al_last_error() -> *const c_char;
     pub fn al_version() -> *const c_char;
@@ -78,6 +81,20 @@ extern "C" {
                          ipiv: *mut usize) -> c_int;
     pub fn al_dgetrs_nrhs(lu: *const f64, lu_len: usize, ipiv: *const usize, ipiv_len: usize, b: *mut f64,
                           b_len: usize, n: usize, nrhs: usize) -> c_int;
+
+    pub fn al_zgemm(alpha: al_complex, a: *const al_complex, a_len: usize, b: *const al_complex, b_len: usize,
+                    beta: al_complex, c: *mut al_complex, c_len: usize, m: usize, n: usize, k: usize) -> c_int;
+    pub fn al_zgemm_tn(alpha: al_complex, a: *const al_complex, a_len: usize, b: *const al_complex, b_len: usize,
+                       beta: al_complex, c: *mut al_complex, c_len: usize, m: usize, n: usize, k: usize) -> c_int;
+    pub fn al_zgemm_nt(alpha: al_complex, a: *const al_complex, a_len: usize, b: *const al_complex, b_len: usize,
+                       beta: al_complex, c: *mut al_complex, c_len: usize, m: usize, n: usize, k: usize) -> c_int;
+    pub fn al_zgemm_tt(alpha: al_complex, a: *const al_complex, a_len: usize, b: *const al_complex, b_len: usize,
+                       beta: al_complex, c: *mut al_complex, c_len: usize, m: usize, n: usize, k: usize) -> c_int;
+    pub fn al_zgetrf(a: *mut al_complex, a_len: usize, n: usize, ipiv: *mut usize, ipiv_len: usize) -> c_int;
+    pub fn al_zgetrf_rect(a: *mut al_complex, a_len: usize, nrows: usize, ncols: usize, ipiv: *mut usize,
+                          ipiv_len: usize) -> c_int;
+    pub fn al_zgesv(a: *mut al_complex, a_len: usize, b: *mut al_complex, b_len: usize) -> c_int;
+    pub fn al_zpotrf(a: *mut al_complex, a_len: usize, n: usize, upper: c_int) -> c_int;
 
     pub fn al_ir_default_options(opt: *mut al_ir_options);
     pub fn al_dgesv_mixed(a: *const f64, a_len: usize, b: *mut f64, b_len: usize, n: usize, nrhs: usize,

@@ -69,3 +69,76 @@ fn bad_shape_is_an_error_not_a_panic() {
     let mut b = vec![0.0; 3];
     assert_eq!(lapack::dgesv(&mut a, &mut b).unwrap_err(), LapackError::BadShape);
 }
+
+// ---- complex twins: src/lapack/complex.rs:422-451 (test matrices), :483-494, :500-508, :532-575
+
+use algebra_linear_hip::Complex64;
+
+fn cpx(re: f64, im: f64) -> Complex64 {
+    Complex64::new(re, im)
+}
+
+fn general3() -> Vec<Complex64> {
+    vec![cpx(4.0, 1.0), cpx(1.0, 3.0), cpx(2.0, -1.0), cpx(1.0, -2.0), cpx(3.0, 0.0), cpx(1.0, 1.0), cpx(2.0, 0.0),
+         cpx(0.0, -1.0), cpx(5.0, 2.0)]
+}
+
+fn spd3() -> Vec<Complex64> {
+    vec![cpx(5.0, 0.0), cpx(1.0, -1.0), cpx(0.0, 2.0), cpx(1.0, 1.0), cpx(6.0, 0.0), cpx(1.0, 2.0), cpx(0.0, -2.0),
+         cpx(1.0, -2.0), cpx(7.0, 0.0)]
+}
+
+#[test]
+fn zgesv_recovers_the_known_solution() {
+    let a = general3();
+    let x_true = [cpx(1.0, -2.0), cpx(3.0, 0.5), cpx(-1.0, 1.0)];
+    let mut b = vec![cpx(0.0, 0.0); 3];
+    for i in 0..3 {
+        for j in 0..3 {
+            b[i] += a[i + j * 3] * x_true[j];
+        }
+    }
+    let mut aw = a.clone();
+    lapack::zgesv(&mut aw, &mut b).unwrap();
+    for i in 0..3 {
+        assert!((b[i] - x_true[i]).norm() < 1e-12);
+    }
+}
+
+#[test]
+fn zgetrf_reports_the_singular_matrix() {
+    let mut a = vec![cpx(1.0, 1.0), cpx(3.0, 0.0), cpx(2.0, 2.0), cpx(6.0, 0.0)];
+    assert_eq!(lapack::zgetrf(&mut a, 2), Err(LapackError::Singular));
+}
+
+#[test]
+fn zpotrf_factorises_both_triangles() {
+    let a = spd3();
+    let mut lower = a.clone();
+    lapack::zpotrf(&mut lower, 3, false).unwrap();
+    let mut upper = a.clone();
+    lapack::zpotrf(&mut upper, 3, true).unwrap();
+    for i in 0..3 {
+        for j in 0..=i {
+            let mut s = cpx(0.0, 0.0);
+            for k in 0..=j {
+                s += lower[i + k * 3] * lower[j + k * 3].conj();
+            }
+            assert!((s - a[i + j * 3]).norm() < 1e-12);
+        }
+        for j in i..3 {
+            let mut s = cpx(0.0, 0.0);
+            for k in 0..=i {
+                s += upper[k + i * 3].conj() * upper[k + j * 3];
+            }
+            assert!((s - a[i + j * 3]).norm() < 1e-12);
+        }
+    }
+}
+
+#[test]
+#[should_panic(expected = "zgemm_tn: a must be k*m")]
+fn zgemm_shape_panic_has_the_crates_text() {
+    let mut c = vec![cpx(0.0, 0.0); 4];
+    blas::zgemm_tn(cpx(1.0, 0.0), &[cpx(0.0, 0.0); 5], &[cpx(0.0, 0.0); 6], cpx(0.0, 0.0), &mut c, 2, 2, 3);
+}

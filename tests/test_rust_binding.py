@@ -41,5 +41,10 @@ def test_safe_wrappers_cover_the_crate_signatures():
                 "pub fn dgetrf_rect(a: &mut [f64], nrows: usize, ncols: usize, ipiv: &mut [usize]) -> Result<(), LapackError>",
                 "pub fn dgesv(a: &mut [f64], b: &mut [f64]) -> Result<(), LapackError>",
                 "pub fn dpotrf(a: &mut [f64], n: usize) -> Result<(), LapackError>",
-                "pub fn dgetri(lu: &mut [f64], n: usize, ipiv: &[usize]) -> Result<(), LapackError>"]:
+                "pub fn dgetri(lu: &mut [f64], n: usize, ipiv: &[usize]) -> Result<(), LapackError>",
+                "pub fn zgemm(alpha: Complex64, a: &[Complex64], b: &[Complex64], beta: Complex64, c: &mut [Complex64], m: usize, n: usize, k: usize)",
+                "pub fn zgetrf(a: &mut [Complex64], n: usize) -> Result<Vec<usize>, LapackError>",
+                "pub fn zgetrf_rect(a: &mut [Complex64], nrows: usize, ncols: usize, ipiv: &mut [usize]) -> Result<(), LapackError>",
+                "pub fn zgesv(a: &mut [Complex64], b: &mut [Complex64]) -> Result<(), LapackError>",
+                "pub fn zpotrf(a: &mut [Complex64], n: usize, upper: bool) -> Result<(), LapackError>"]:
         assert sig in " ".join(lib_rs.split()), sig

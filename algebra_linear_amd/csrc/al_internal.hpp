@@ -80,6 +80,7 @@ struct Workspace {
     int* ipiv32 = nullptr;         // int32 pivots on the device
     size_t ipiv_len = 0;
     int* pairs = nullptr;          // composed row permutation of one pivot block
+    int* pairs2 = nullptr;         // the same for the main stream while the panel stream runs ahead
     double* perm = nullptr;        // row origins of a whole factorization (exact integers)
     size_t perm_len = 0;
     double* wbuf = nullptr;        // right-hand-side staging for the solves
@@ -164,11 +165,11 @@ int trsm_f64(hipStream_t s, bool left, bool lower, bool trans, bool unit, size_t
 // Row interchanges ipiv[k0..k0+jb) (int32, global row indices, jb <= 256) applied to the n0
 // columns at A0 and the n1 columns at A1 (leading dimension lda) and, if perm is not null, to
 // the row-origin vector perm -- with the result of the sequential swaps, as one gather.
-int swap_block_f64(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, size_t jb, double* A0, size_t n0,
-                   double* A1, size_t n1, size_t lda, double* perm, const int* info);
-// The same for an FP32 matrix (no perm vector).
 // pairs: the composed-permutation buffer (1 + 2 * 512 ints); ws->pairs when null.  Swaps
 // that may run concurrently on two streams need buffers of their own.
+int swap_block_f64(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, size_t jb, double* A0, size_t n0,
+                   double* A1, size_t n1, size_t lda, double* perm, const int* info, int* pairs = nullptr);
+// The same for an FP32 matrix (no perm vector).
 int swap_block_f32(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, size_t jb, float* A0, size_t n0,
                    float* A1, size_t n1, size_t lda, const int* info, int* pairs = nullptr);
 // The same for a complex matrix (lda in complex elements).
@@ -186,6 +187,10 @@ int perm_init(hipStream_t s, double* perm, size_t n);                          /
 int perm_from_ipiv(hipStream_t s, const int* ipiv, size_t n, double* perm);    // compose all pivots
 int gather_rows(hipStream_t s, const double* B, size_t ldb, size_t n, size_t nrhs, const double* perm, double* W);
 int copy_block(hipStream_t s, const double* W, size_t n, size_t nrhs, double* B, size_t ldb);
+
+// Creates ws->side (a second stream, default priority) and the two events that order it with
+// the caller's stream, once per workspace.
+int ensure_side_stream(Workspace* ws);
 
 // Compute units of the current device (256 on MI355X).
 int cu_count();

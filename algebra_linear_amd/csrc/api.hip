@@ -41,6 +41,7 @@ struct ThreadCtx {
             if (w.flags) (void)hipFree(w.flags);
             if (w.ipiv32) (void)hipFree(w.ipiv32);
             if (w.pairs) (void)hipFree(w.pairs);
+            if (w.pairs2) (void)hipFree(w.pairs2);
             if (w.perm) (void)hipFree(w.perm);
             if (w.wbuf) (void)hipFree(w.wbuf);
             if (w.trsv_flags) (void)hipFree(w.trsv_flags);
@@ -189,6 +190,7 @@ int workspace_for(hipStream_t s, Workspace** out, size_t ipiv_len) {
         w.slots_bytes = panel_slots_bytes();
         AL_HIP(hipMalloc((void**)&w.slots, w.slots_bytes));
         AL_HIP(hipMalloc((void**)&w.pairs, (1 + 2 * 512) * sizeof(int)));
+        AL_HIP(hipMalloc((void**)&w.pairs2, (1 + 2 * 512) * sizeof(int)));
         w.device = g_ctx.device;
     }
     if (w.ipiv_len < ipiv_len) {

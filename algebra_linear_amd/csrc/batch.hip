@@ -1220,9 +1220,7 @@ int batch_launch(hipStream_t s, hipStream_t side, hipEvent_t ev_fork, hipEvent_t
 int batch_solve(hipStream_t s, Workspace* ws, size_t count, const al_batch_entry* d_entries, double* A_pool,
                 double* B_pool, float* W_pool, int* d_info, int* d_steps, void* work, bool mixed, int max_iter) {
     if (count == 0) return AL_OK;
-    if (!ws->side) AL_HIP(hipStreamCreateWithFlags(&ws->side, hipStreamNonBlocking));
-    if (!ws->ev_side) AL_HIP(hipEventCreateWithFlags(&ws->ev_side, hipEventDisableTiming));
-    if (!ws->ev_main) AL_HIP(hipEventCreateWithFlags(&ws->ev_main, hipEventDisableTiming));
+    AL_CHECK(ensure_side_stream(ws));
     const int ticket = prof_begin(s);
     AL_CHECK(batch_launch(s, ws->side, ws->ev_main, ws->ev_side, count, d_entries, A_pool, B_pool, W_pool, d_info,
                           d_steps, work, mixed, max_iter));

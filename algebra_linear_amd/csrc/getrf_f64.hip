@@ -98,6 +98,51 @@ __device__ __forceinline__ float key_value<float>(unsigned long long c) {
     return c >= 2ull ? __uint_as_float((unsigned)(c - 2ull)) : (c == 1ull ? -1.0f : -2.0f);
 }
 
+// Agent-scope coherent element accesses for the cooperative panel's exchange area: relaxed
+// atomics, which gfx950 issues as write-through stores and L2-missing loads (the sc1 bit), so
+// the exchange needs no L2 writeback or invalidate (a concurrent trailing update's dirty lines
+// in L2 never have to be flushed for it).  Complex elements travel as their two doubles.
+__device__ __forceinline__ void st_agent(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(float* p, float v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(zd* p, zd v) {
+    st_agent(&p->re, v.re);
+    st_agent(&p->im, v.im);
+}
+__device__ __forceinline__ double ld_agent(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float ld_agent(const float* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ zd ld_agent(const zd* p) {
+    zd v;
+    v.re = ld_agent(&p->re);
+    v.im = ld_agent(&p->im);
+    return v;
+}
+__device__ __forceinline__ unsigned long long ld_flag(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The 32-bit head of a pivot key's integer image: orders keys as key_code does except that keys
+// differing only below their top 32 bits compare equal (such ties are settled from the full
+// keys).  The sentinels -1 and -2 map to 1 and 0.
+__device__ __forceinline__ unsigned key_head(double k) {
+    return k >= 0.0 ? (unsigned)((unsigned long long)__double_as_longlong(k) >> 32) + 2u : (k == -1.0 ? 1u : 0u);
+}
+__device__ __forceinline__ unsigned key_head(float k) {  // the whole key: no ties beyond equal keys
+    return k >= 0.0f ? __float_as_uint(k) + 2u : (k == -1.0f ? 1u : 0u);
+}
+// A publication word: the column's 16-bit generation, the candidate's panel row (16 bits; rows
+// stay below 32768) and its key head.
+__device__ __forceinline__ unsigned long long flag_word(unsigned gen, int idx, unsigned head) {
+    return ((unsigned long long)(gen & 0xFFFFu) << 48) | ((unsigned long long)(idx & 0xFFFF) << 32) | head;
+}
+
 // Workgroup-wide first maximum: the largest key, ties to the lowest row.  Two LDS atomics
 // (max of the key's integer image, then min of the row among the maximal keys) instead of a
 // shuffle tree.  `red` alternates between two slots per call so the next call's reset needs
@@ -130,14 +175,18 @@ __device__ __forceinline__ void block_argmax(T& key, int& idx, ArgmaxLds* red, i
 // (chunk <= 256, one row per thread) and keeps them in LDS for the whole panel.
 //
 // Column exchange (COOP, several workgroups): wave 0 writes the workgroup's slot -- its best
-// candidate (key, row index) with that row, and row j if it owns it -- and raises the
-// workgroup's flag to the column's generation with one release store.  Wave 0 polls every
-// flag with relaxed loads (no contended counter), takes one acquire fence, and copies the
-// winner's row and row j into LDS.  Slots alternate by column parity: a workgroup overwrites
-// parity p only after every workgroup has published the next column, which each does only
-// after it has finished reading parity p.  Every spin is bounded by `timeout` ticks of the
-// 100 MHz wall clock; on expiry the abort flag stops every workgroup and info is set to -1.
-// A single workgroup exchanges through LDS only.
+// candidate (key, row index) with that row, and row j if it owns it -- with write-through
+// stores, waits until they are acknowledged, and then publishes one 64-bit word for the column
+// (its generation, the candidate's row and the top 32 bits of its key).  Words and slots
+// alternate by column parity: a workgroup overwrites parity p only after every workgroup has
+// published the next column, which each does only after it has finished reading parity p.
+// Wave 0 of every workgroup polls the words of the column's parity (L2-missing loads) until
+// all carry the column's generation; the largest key head names the winner, and only when
+// several workgroups share it are their full keys read from the slots.  Then the winner's row,
+// its full key and row j come from the slots in one round trip.  No L2 writeback or invalidate
+// is involved.  Every spin is bounded by `timeout` ticks of the 100 MHz wall clock; on expiry
+// the abort flag stops every workgroup and info is set to -1.  A single workgroup exchanges
+// through LDS only.
 //
 // Pipelining: after the swap, column j+1 is eliminated first, the next candidate is chosen,
 // its row (and row j+1) are brought up to date, and the slot is published before the rest of
@@ -145,7 +194,8 @@ __device__ __forceinline__ void block_argmax(T& key, int& idx, ArgmaxLds* red, i
 template <typename T, bool COOP>
 __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda, int row0, int m, int ncols,
                                                   int chunk, int* __restrict__ ipiv, int* __restrict__ info,
-                                                  unsigned* __restrict__ flags, unsigned* __restrict__ abort_flag,
+                                                  unsigned long long* __restrict__ flags,
+                                                  unsigned* __restrict__ abort_flag,
                                                   T* __restrict__ slots, unsigned gen0, long long timeout) {
     typedef typename PanelElem<T>::key_t K;
     constexpr int NB = PanelElem<T>::NB;
@@ -181,22 +231,37 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
     auto slot_ptr = [&](int par, int q) -> T* {
         return COOP ? slots + ((size_t)par * nwg + q) * SLOT2 : &lslot[par][0];
     };
-    // wave 0 writes this workgroup's slot for column jj and raises its flag
+    // wave 0 writes this workgroup's slot for column jj and publishes its word
     auto publish = [&](int jj, K key, int idx) {
         if (wave == 0) {
             T* my = slot_ptr(jj & 1, w);
-            if (key > (K)-2) {
-                const int lr = idx - rb;
-                for (int c = lane; c < ncols; c += 64) my[2 + c] = S[c * chunk + lr];
-            }
-            if (jj >= rb && jj < rb + nr)
-                for (int c = lane; c < ncols; c += 64) my[2 + NB + c] = S[c * chunk + (jj - rb)];
-            if (lane == 0) {
-                my[0] = T(key);
-                my[1] = T((K)idx);
-                if (COOP)
-                    __hip_atomic_store(&flags[w * 16], gen0 + (unsigned)jj + 1u, __ATOMIC_RELEASE,
-                                       __HIP_MEMORY_SCOPE_AGENT);
+            const bool cand = key > (K)-2;
+            const int lr = idx - rb;
+            const bool ownj = jj >= rb && jj < rb + nr;
+            if (COOP) {
+                if (cand)
+                    for (int c = lane; c < ncols; c += 64) st_agent(&my[2 + c], S[c * chunk + lr]);
+                if (ownj)
+                    for (int c = lane; c < ncols; c += 64) st_agent(&my[2 + NB + c], S[c * chunk + (jj - rb)]);
+                if (lane == 0) {
+                    st_agent(&my[0], T(key));
+                    st_agent(&my[1], T((K)idx));
+                }
+                // the slot's stores are acknowledged (write-through) before the word goes out
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_s_waitcnt(0);
+                if (lane == 0)
+                    __hip_atomic_store(&flags[w * 8 + (jj & 1)], flag_word(gen0 + (unsigned)jj + 1u, idx, key_head(key)),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                if (cand)
+                    for (int c = lane; c < ncols; c += 64) my[2 + c] = S[c * chunk + lr];
+                if (ownj)
+                    for (int c = lane; c < ncols; c += 64) my[2 + NB + c] = S[c * chunk + (jj - rb)];
+                if (lane == 0) {
+                    my[0] = T(key);
+                    my[1] = T((K)idx);
+                }
             }
         }
     };
@@ -206,13 +271,13 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
         if (COOP) {
             if (wave == 0) {
                 int ok = 1;
-                const unsigned want = gen0 + (unsigned)jj + 1u;
+                const unsigned long long want = (unsigned long long)((gen0 + (unsigned)jj + 1u) & 0xFFFFu);
                 const long long t0 = wall_clock64();
+                unsigned long long f0 = 0, f1 = 0;
                 for (;;) {
-                    bool mine = true;
-                    for (int q = lane; q < nwg; q += 64)
-                        mine = mine && (__hip_atomic_load(&flags[q * 16], __ATOMIC_RELAXED,
-                                                          __HIP_MEMORY_SCOPE_AGENT) >= want);
+                    f0 = lane < nwg ? ld_flag(&flags[lane * 8 + par]) : 0ull;
+                    f1 = lane + 64 < nwg ? ld_flag(&flags[(lane + 64) * 8 + par]) : 0ull;
+                    const bool mine = (lane >= nwg || (f0 >> 48) == want) && (lane + 64 >= nwg || (f1 >> 48) == want);
                     if (__all(mine)) break;
                     int stop = 0;
                     if (lane == 0) {
@@ -230,36 +295,55 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
                     __builtin_amdgcn_s_sleep(1);
                 }
                 if (ok) {
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                    K bk = (K)-3;
-                    int bi = 0x7fffffff;
-                    for (int q = lane; q < nwg; q += 64) {
-                        const T* sq = slot_ptr(par, q);
-                        const K kq = slot_val(sq[0]);
-                        const int iq = (int)slot_val(sq[1]);
-                        if (better(kq, iq, bk, bi)) {
-                            bk = kq;
-                            bi = iq;
-                        }
-                    }
+                    // nothing read below may be issued before the words were seen
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    const unsigned h0 = lane < nwg ? (unsigned)f0 : 0u;
+                    const unsigned h1 = lane + 64 < nwg ? (unsigned)f1 : 0u;
+                    unsigned hmax = h0 > h1 ? h0 : h1;
 #pragma unroll
                     for (int off = 32; off >= 1; off >>= 1) {
-                        const K ok2 = __shfl_xor(bk, off, 64);
-                        const int oi = __shfl_xor(bi, off, 64);
-                        if (better(ok2, oi, bk, bi)) {
-                            bk = ok2;
-                            bi = oi;
-                        }
+                        const unsigned o = (unsigned)__shfl_xor((int)hmax, off, 64);
+                        hmax = o > hmax ? o : hmax;
                     }
-                    const T* win = slot_ptr(par, bi / chunk);
+                    const bool t0m = lane < nwg && h0 == hmax, t1m = lane + 64 < nwg && h1 == hmax;
+                    const unsigned long long b0 = __ballot(t0m), b1 = __ballot(t1m);
+                    int bq;  // the winning workgroup
+                    if (__popcll(b0) + __popcll(b1) == 1) {
+                        bq = b0 ? __ffsll((long long)b0) - 1 : 64 + __ffsll((long long)b1) - 1;
+                    } else {
+                        // several workgroups share the largest key head: compare their full keys
+                        K bk = (K)-3;
+                        int bi = 0x7fffffff;
+                        for (int q = lane; q < nwg; q += 64) {
+                            if (!(q < 64 ? t0m : t1m)) continue;
+                            const T* sq = slot_ptr(par, q);
+                            const K kq = slot_val(ld_agent(&sq[0]));
+                            const int iq = (int)slot_val(ld_agent(&sq[1]));
+                            if (better(kq, iq, bk, bi)) {
+                                bk = kq;
+                                bi = iq;
+                            }
+                        }
+#pragma unroll
+                        for (int off = 32; off >= 1; off >>= 1) {
+                            const K ok2 = __shfl_xor(bk, off, 64);
+                            const int oi = __shfl_xor(bi, off, 64);
+                            if (better(ok2, oi, bk, bi)) {
+                                bk = ok2;
+                                bi = oi;
+                            }
+                        }
+                        bq = bi / chunk;
+                    }
+                    const T* win = slot_ptr(par, bq);
                     const T* own = slot_ptr(par, jj / chunk);
                     for (int c = lane; c < ncols; c += 64) {
-                        prow_c[c] = win[2 + c];
-                        rj_c[c] = own[2 + NB + c];
+                        prow_c[c] = ld_agent(&win[2 + c]);
+                        rj_c[c] = ld_agent(&own[2 + NB + c]);
                     }
                     if (lane == 0) {
-                        win_lds = bi;
-                        wkey_lds = bk;
+                        win_lds = (int)slot_val(ld_agent(&win[1]));
+                        wkey_lds = slot_val(ld_agent(&win[0]));
                     }
                 }
                 if (lane == 0) ok_lds = ok;
@@ -382,7 +466,9 @@ int launch_panel(hipStream_t s, Workspace* ws, T* Ppanel, size_t lda, size_t row
         set_error("getrf: panel taller than the cooperative kernel holds");
         return AL_ERR_ARG;
     }
-    if (ws->gen > 0xF0000000u) {
+    // publication words carry 16-bit generations: start a new epoch (words cleared) before one
+    // would wrap, so a word left from an earlier panel never matches a current generation
+    if (ws->gen + (unsigned)ncols >= 0xFFFFu) {
         AL_CHECK(fill_u32(s, ws->flags, (size_t)MAX_WG * 16, 0u));
         ws->gen = 0;
     }
@@ -392,11 +478,11 @@ int launch_panel(hipStream_t s, Workspace* ws, T* Ppanel, size_t lda, size_t row
     const int ticket = prof_begin(s);
     if (nwg > 1) {
         hipLaunchKernelGGL((panel_getf2<T, true>), dim3((unsigned)nwg), dim3(PT), chunk * NB * sizeof(T), s, Ppanel,
-                           lda, (int)row0, (int)m, (int)ncols, (int)chunk, ipiv, info, ws->flags, ws->abort_flag,
+                           lda, (int)row0, (int)m, (int)ncols, (int)chunk, ipiv, info, (unsigned long long*)ws->flags, ws->abort_flag,
                            (T*)ws->slots, gen0, timeout);
     } else {
         hipLaunchKernelGGL((panel_getf2<T, false>), dim3(1), dim3(PT), chunk * NB * sizeof(T), s, Ppanel, lda,
-                           (int)row0, (int)m, (int)ncols, (int)chunk, ipiv, info, ws->flags, ws->abort_flag,
+                           (int)row0, (int)m, (int)ncols, (int)chunk, ipiv, info, (unsigned long long*)ws->flags, ws->abort_flag,
                            (T*)ws->slots, gen0, timeout);
     }
     AL_HIP(hipGetLastError());
@@ -567,30 +653,85 @@ int getrf_f64(hipStream_t s, Workspace* ws, size_t M, size_t N, double* A, size_
     if (st != AL_OK) {
         // kernels already enqueued may still run: let them finish before the room is returned
         (void)hipStreamSynchronize(s);
+        if (ws->side) (void)hipStreamSynchronize(ws->side);
         coop_release(units);
         return st;
     }
     return coop_hold(s, units);
 }
 
+int ensure_side_stream(Workspace* ws) {
+    if (!ws->side) AL_HIP(hipStreamCreateWithFlags(&ws->side, hipStreamNonBlocking));
+    if (!ws->ev_side) AL_HIP(hipEventCreateWithFlags(&ws->ev_side, hipEventDisableTiming));
+    if (!ws->ev_main) AL_HIP(hipEventCreateWithFlags(&ws->ev_main, hipEventDisableTiming));
+    return AL_OK;
+}
+
+// Blocked right-looking LU with look-ahead on two streams (when there is more than one outer
+// panel).  The side stream carries the chain that bounds the run: it factors an outer panel
+// (256 columns as four cooperative 64-column panels with their in-panel updates), then applies
+// that panel's interchanges, U12 solve and trailing product to the next panel's columns only,
+// and factors the next panel.  The caller's stream applies the same interchanges to every other
+// column (and to perm), solves U12 there and runs the large trailing product, while the side
+// stream factors the next panel.  The two streams touch disjoint columns; two events order
+// them: the side stream's preparation of a panel waits for the main stream's previous update
+// (that update covers the panel's columns), and the main stream's update waits for the panel.
+// Each stream composes the interchanges in its own pair buffer.  The pivots, and every
+// operation applied to any column, are those of the single-stream order.
 int getrf_enqueue(hipStream_t s, Workspace* ws, size_t M, size_t N, double* A, size_t lda, int* ipiv, int* info,
                   double* perm) {
     const size_t kmax = M < N ? M : N;
     AL_CHECK(fill_i32(s, info, 1, 0));
     AL_CHECK(fill_u32(s, ws->abort_flag, 1, 0u));
     if (perm) AL_CHECK(perm_init(s, perm, M));
+    auto update = [&](hipStream_t q, size_t k, size_t jb, size_t c0, size_t c1) -> int {
+        // U12 and the trailing product of the outer panel [k, k + jb) on columns [c0, c1)
+        if (c1 <= c0) return AL_OK;
+        double* A12 = A + k + c0 * lda;
+        AL_CHECK(trsm_f64(q, true, true, false, true, jb, c1 - c0, 1.0, A + k + k * lda, lda, A12, lda, info));
+        if (M > k + jb)
+            AL_CHECK(gemm_f64(q, false, false, M - k - jb, c1 - c0, jb, -1.0, A + (k + jb) + k * lda, lda, A12, lda,
+                              1.0, A + (k + jb) + c0 * lda, lda));
+        return AL_OK;
+    };
+    if (kmax <= (size_t)NBO) {
+        for (size_t k = 0; k < kmax; k += NBO) {
+            const size_t jb = (kmax - k) < (size_t)NBO ? (kmax - k) : (size_t)NBO;
+            AL_CHECK(panel_blocked(s, ws, A, lda, M, k, jb, ipiv, info));
+            const size_t right = (k + jb < N) ? N - k - jb : 0;
+            AL_CHECK(swap_block_f64(s, ws, ipiv, k, jb, A, k, A + (k + jb) * lda, right, lda, perm, info));
+            AL_CHECK(update(s, k, jb, k + jb, N));
+        }
+        return AL_OK;
+    }
+    AL_CHECK(ensure_side_stream(ws));
+    hipStream_t side = ws->side;
+    AL_HIP(hipEventRecord(ws->ev_main, s));
+    AL_HIP(hipStreamWaitEvent(side, ws->ev_main, 0));
+    AL_CHECK(panel_blocked(side, ws, A, lda, M, 0, NBO, ipiv, info));
     for (size_t k = 0; k < kmax; k += NBO) {
         const size_t jb = (kmax - k) < (size_t)NBO ? (kmax - k) : (size_t)NBO;
-        AL_CHECK(panel_blocked(s, ws, A, lda, M, k, jb, ipiv, info));
-        const size_t right = (k + jb < N) ? N - k - jb : 0;
-        AL_CHECK(swap_block_f64(s, ws, ipiv, k, jb, A, k, A + (k + jb) * lda, right, lda, perm, info));
-        if (right > 0) {
-            double* A12 = A + k + (k + jb) * lda;
-            AL_CHECK(trsm_f64(s, true, true, false, true, jb, right, 1.0, A + k + k * lda, lda, A12, lda, info));
-            if (M > k + jb)
-                AL_CHECK(gemm_f64(s, false, false, M - k - jb, right, jb, -1.0, A + (k + jb) + k * lda, lda, A12, lda,
-                                  1.0, A + (k + jb) + (k + jb) * lda, lda));
+        const size_t ke = k + jb;
+        if (ke >= kmax) {  // the last panel: the columns left of it (and right of kmax when N > M)
+            AL_HIP(hipEventRecord(ws->ev_side, side));
+            AL_HIP(hipStreamWaitEvent(s, ws->ev_side, 0));
+            AL_CHECK(swap_block_f64(s, ws, ipiv, k, jb, A, k, A + ke * lda, N - ke, lda, perm, info, ws->pairs2));
+            AL_CHECK(update(s, k, jb, ke, N));
+            break;
         }
+        const size_t ke2 = (kmax - ke) < (size_t)NBO ? kmax : ke + NBO;
+        // side: the next panel's columns, once the main stream's last update of them is done
+        AL_HIP(hipStreamWaitEvent(side, ws->ev_main, 0));
+        AL_CHECK(swap_block_f64(side, ws, ipiv, k, jb, A + ke * lda, ke2 - ke, nullptr, 0, lda, nullptr, info,
+                                 ws->pairs));
+        AL_CHECK(update(side, k, jb, ke, ke2));
+        AL_HIP(hipEventRecord(ws->ev_side, side));
+        AL_CHECK(panel_blocked(side, ws, A, lda, M, ke, ke2 - ke, ipiv, info));
+        // main: every other column
+        AL_HIP(hipStreamWaitEvent(s, ws->ev_side, 0));
+        AL_CHECK(swap_block_f64(s, ws, ipiv, k, jb, A, k, A + ke2 * lda, N - ke2, lda, perm, info, ws->pairs2));
+        AL_CHECK(update(s, k, jb, ke2, N));
+        AL_HIP(hipEventRecord(ws->ev_main, s));
     }
     return AL_OK;
 }

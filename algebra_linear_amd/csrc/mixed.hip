@@ -441,13 +441,6 @@ int trailing_update(hipStream_t s, Arena& a, bool bf16, int K, int KE, int c0, i
     return gemm_f32(s, false, false, m2, c1 - c0, kb, -1.f, at(KE, K), ld, at(K, c0), ld, 1.f, at(KE, c0), ld);
 }
 
-int ensure_side_stream(Workspace* ws) {
-    if (!ws->side) AL_HIP(hipStreamCreateWithFlags(&ws->side, hipStreamNonBlocking));
-    if (!ws->ev_side) AL_HIP(hipEventCreateWithFlags(&ws->ev_side, hipEventDisableTiming));
-    if (!ws->ev_main) AL_HIP(hipEventCreateWithFlags(&ws->ev_main, hipEventDisableTiming));
-    return AL_OK;
-}
-
 // Blocked right-looking LU with partial pivoting of the np x np FP32 matrix a.F
 // (np % 64 == 0); pivots as global row indices in a.ipiv, status in a.minfo[0].
 //

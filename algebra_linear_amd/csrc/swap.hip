@@ -227,8 +227,8 @@ int swap_block_t(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, size_
 }
 
 int swap_block_f64(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, size_t jb, double* A0, size_t n0,
-                   double* A1, size_t n1, size_t lda, double* perm, const int* info) {
-    return swap_block_t<double>(s, ws, ipiv, k0, jb, A0, n0, A1, n1, lda, perm, info, nullptr);
+                   double* A1, size_t n1, size_t lda, double* perm, const int* info, int* pairs) {
+    return swap_block_t<double>(s, ws, ipiv, k0, jb, A0, n0, A1, n1, lda, perm, info, pairs);
 }
 
 int swap_block_z(hipStream_t s, Workspace* ws, const int* ipiv, size_t k0, size_t jb, zd* A0, size_t n0, zd* A1,

@@ -11,7 +11,12 @@ are generated on the host with numpy.  Workloads (--what, comma-separated):
   zgemm  -- al_dev_zgemm nn, m = n = k = 4096, entries uniform in the unit square;
   zgetrf -- al_dev_zgetrf, n = 8192, entries uniform in the unit square;
   zpotrf -- al_dev_zpotrf lower, n = 8192, a Hermitian matrix with entries in the unit square
-            plus 2n I (positive definite by Gershgorin's theorem).
+            plus 2n I (positive definite by Gershgorin's theorem);
+  dgetrf -- al_dev_dgetrf, n = 8192, uniform(-1, 1), the matrix restored by a copy per call;
+  dgetrf_rect -- al_dev_dgetrf on a 16384 x 2048 panel, uniform(-1, 1), restored per call;
+  dpotrf -- al_dev_dpotrf, n = 8192, A = G G^T + n I with G uniform(-1, 1), restored per call;
+  dgesv64 -- the plain FP64 solve, al_dev_dgetrf + al_dev_dgetrs, n = 16384, one right-hand
+            side, uniform(-1, 1), restored per call.
 Each runs --reps times after one warm-up.
 
   python -m algebra_linear_amd.profile_kernels --what mixed,dgemm,batch,zgemm,zgetrf,zpotrf
@@ -153,6 +158,48 @@ def main() -> int:
         info = dev.download(dInfo, np.zeros(count, dtype=np.int32))
         if np.any(info != 0):
             raise RuntimeError("a batched system failed")
+        dev.close()
+    for name, (m, n) in (("dgetrf", (8192, 8192)), ("dgetrf_rect", (16384, 2048)), ("dgesv64", (16384, 16384))):
+        if name not in what:
+            continue
+        A = rng.uniform(-1.0, 1.0, m * n)
+        dA0, dA = dev.upload(A), dev.alloc(A.nbytes)
+        dPiv, dInfo = dev.alloc(4 * n), dev.alloc(4)
+        dB0 = dev.upload(rng.standard_normal(m))
+        dB = dev.alloc(8 * m)
+        del A
+
+        def lu(m=m, n=n, dA0=dA0, dA=dA, dPiv=dPiv, dInfo=dInfo, dB0=dB0, dB=dB, solve=name == "dgesv64"):
+            if dev.hip.hipMemcpy(dA, dA0, 8 * m * n, 3) != 0:
+                raise RuntimeError("hipMemcpy")
+            if lib.al_dev_dgetrf(m, n, dA, m, dPiv, dInfo, None) != 0:
+                raise RuntimeError(_lib.last_error())
+            if solve:
+                if dev.hip.hipMemcpy(dB, dB0, 8 * m, 3) != 0:
+                    raise RuntimeError("hipMemcpy")
+                if lib.al_dev_dgetrs(n, 1, dA, m, dPiv, dB, n, None) != 0:
+                    raise RuntimeError(_lib.last_error())
+        run(lu, args.reps, dev, f"{name} {m}x{n}")
+        if dev.download(dInfo, np.zeros(1, dtype=np.int32))[0] != 0:
+            raise RuntimeError(f"{name} reported a zero pivot")
+        dev.close()
+    if "dpotrf" in what:
+        n = 8192
+        G = rng.uniform(-1.0, 1.0, (n, n))
+        A = (G @ G.T + n * np.eye(n)).reshape(-1)
+        del G
+        dA0, dA = dev.upload(A), dev.alloc(A.nbytes)
+        dInfo = dev.alloc(4)
+        del A
+
+        def potrf():
+            if dev.hip.hipMemcpy(dA, dA0, 8 * n * n, 3) != 0:
+                raise RuntimeError("hipMemcpy")
+            if lib.al_dev_dpotrf(n, dA, n, dInfo, None) != 0:
+                raise RuntimeError(_lib.last_error())
+        run(potrf, args.reps, dev, f"dpotrf {n}")
+        if dev.download(dInfo, np.zeros(1, dtype=np.int32))[0] != 0:
+            raise RuntimeError("dpotrf reported a non-positive pivot")
         dev.close()
     one, zero = _lib.Complex.of(1.0), _lib.Complex.of(0.0)
 

@@ -51,6 +51,7 @@ struct ThreadCtx {
             if (w.gbuf) (void)hipFree(w.gbuf);
             if (w.bbuf) (void)hipFree(w.bbuf);
             if (w.zbuf) (void)hipFree(w.zbuf);
+            if (w.cbuf) (void)hipFree(w.cbuf);
             if (w.ev_side) (void)hipEventDestroy(w.ev_side);
             if (w.ev_main) (void)hipEventDestroy(w.ev_main);
             if (w.side) (void)hipStreamDestroy(w.side);

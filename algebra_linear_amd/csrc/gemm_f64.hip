@@ -6,7 +6,8 @@
 //
 // Design (MI355X-first, not a translation of the crate's Goto blocking):
 //   * one 256-thread workgroup (4 wave64s, 2x2) computes a 128x128 tile of C; each wave
-//     owns 64x64 = 4x4 tiles of v_mfma_f64_16x16x4_f64;
+//     owns 64x64 = 4x4 tiles of v_mfma_f64_16x16x4_f64.  Products with fewer 128-tiles than
+//     two per CU use 64x64 workgroup tiles (2x2 MFMA tiles per wave) instead;
 //   * K advances in steps of 16 through double-buffered LDS (66 KiB) with at most 256
 //     registers per lane, so two workgroups share a CU and one's loads and epilogue overlap
 //     the other's MFMAs; the next step's global loads
@@ -49,73 +50,81 @@ struct Mfma<float> {
     static __device__ __forceinline__ int drow(int kq, int r) { return 4 * kq + r; }
 };
 
-constexpr int BM = 128, BN = 128, BK = 16;
+constexpr int BK = 16;
 constexpr int NT = 256;
-constexpr int LDS_ROW = BM + 1;  // one-double pad breaks the strided stores' bank conflicts
-constexpr int TILE_DOUBLES = BK * LDS_ROW;
-template <typename T>
-constexpr size_t lds_bytes() { return 4 * TILE_DOUBLES * sizeof(T); }  // A and B, double-buffered
+// Workgroup tile TM x TM (128, or 64 for products too small to fill the chip with 128-tiles):
+// 2 x 2 waves of (TM/2) x (TM/2), i.e. (TM/32) x (TM/32) MFMA blocks of 16 x 16 per wave.
+template <int TM>
+constexpr int lds_row() { return TM + 1; }  // one-element pad breaks the strided stores' bank conflicts
+template <int TM>
+constexpr int tile_elems() { return BK * lds_row<TM>(); }
+template <typename T, int TM>
+constexpr size_t lds_bytes() { return 4 * tile_elems<TM>() * sizeof(T); }  // A and B, double-buffered
+template <int TM>
+constexpr int lpt() { return TM * BK / NT; }  // operand elements per thread per tile
 
-// Loads the 128 x BK operand tile whose element (r, p) is base[r + p*ld] (ROWCONTIG false:
+// Loads the TM x BK operand tile whose element (r, p) is base[r + p*ld] (ROWCONTIG false:
 // the tile's columns are contiguous, e.g. a non-transposed A) or base[p + r*ld] (true: the
-// tile's rows are contiguous, e.g. a transposed A) into LPT registers per thread, zero-filling
+// tile's rows are contiguous, e.g. a transposed A) into lpt registers per thread, zero-filling
 // outside [0, r_lim) x [0, kdim).
-constexpr int LPT = BM * BK / NT;  // operand elements per thread per tile
-
-template <bool ROWCONTIG, typename T>
+template <bool ROWCONTIG, int TM, typename T>
 __device__ __forceinline__ void load_tile(const T* __restrict__ base, size_t ld, int r0, int r_lim, int k0,
-                                          int kdim, T (&reg)[LPT], int t) {
+                                          int kdim, T (&reg)[lpt<TM>()], int t) {
     if (!ROWCONTIG) {
-        const int r = t & 127, p0 = t >> 7;
+        const int r = t & (TM - 1), p0 = t / TM;
         const bool rok = (r0 + r) < r_lim;
 #pragma unroll
-        for (int j = 0; j < LPT; ++j) {
-            const int p = k0 + p0 + 2 * j;
+        for (int j = 0; j < lpt<TM>(); ++j) {
+            const int p = k0 + p0 + (NT / TM) * j;
             reg[j] = (rok && p < kdim) ? base[(size_t)(r0 + r) + (size_t)p * ld] : (T)0;
         }
     } else {
         const int p = t % BK, rr = t / BK;
         const bool pok = (k0 + p) < kdim;
 #pragma unroll
-        for (int j = 0; j < LPT; ++j) {
+        for (int j = 0; j < lpt<TM>(); ++j) {
             const int r = r0 + rr + (NT / BK) * j;
             reg[j] = (pok && r < r_lim) ? base[(size_t)(k0 + p) + (size_t)r * ld] : (T)0;
         }
     }
 }
 
-template <bool ROWCONTIG, typename T>
-__device__ __forceinline__ void store_tile(T* __restrict__ S, const T (&reg)[LPT], int t) {
+template <bool ROWCONTIG, int TM, typename T>
+__device__ __forceinline__ void store_tile(T* __restrict__ S, const T (&reg)[lpt<TM>()], int t) {
+    constexpr int LR = lds_row<TM>();
     if (!ROWCONTIG) {
-        const int r = t & 127, p0 = t >> 7;
+        const int r = t & (TM - 1), p0 = t / TM;
 #pragma unroll
-        for (int j = 0; j < LPT; ++j) S[(p0 + 2 * j) * LDS_ROW + r] = reg[j];
+        for (int j = 0; j < lpt<TM>(); ++j) S[(p0 + (NT / TM) * j) * LR + r] = reg[j];
     } else {
         const int p = t % BK, rr = t / BK;
 #pragma unroll
-        for (int j = 0; j < LPT; ++j) S[p * LDS_ROW + rr + (NT / BK) * j] = reg[j];
+        for (int j = 0; j < lpt<TM>(); ++j) S[p * LR + rr + (NT / BK) * j] = reg[j];
     }
 }
 
-template <typename T, bool TA, bool TB, int TRI>
+template <typename T, bool TA, bool TB, int TRI, int TM>
 __global__ __launch_bounds__(NT, 2) void dgemm_mfma(int m, int n, int k, T alpha, const T* __restrict__ A,
                                                  size_t lda, const T* __restrict__ B, size_t ldb, T beta,
                                                  T* __restrict__ C, size_t ldc, int tiles_m, int tiles_n) {
+    constexpr int LR = lds_row<TM>();
+    constexpr int TE = tile_elems<TM>();
+    constexpr int WT = TM / 32;  // MFMA blocks per wave and dimension
     extern __shared__ __align__(16) unsigned char lds_raw[];
     T* lds = (T*)lds_raw;
     T* As0 = lds;
-    T* As1 = lds + TILE_DOUBLES;
-    T* Bs0 = lds + 2 * TILE_DOUBLES;
-    T* Bs1 = lds + 3 * TILE_DOUBLES;
+    T* As1 = lds + TE;
+    T* Bs0 = lds + 2 * TE;
+    T* Bs1 = lds + 3 * TE;
     typedef typename Mfma<T>::acc_t acc_t;
 
     int tm, tn;
     tile_coords(tiles_m, tiles_n, tm, tn);
-    const int bm = tm * BM, bn = tn * BN;
-    if (TRI == 1 && bm + BM - 1 < bn) return;  // tile strictly above the diagonal
-    if (TRI == 2 && bm > bn + BN - 1) return;  // tile strictly below the diagonal
-    if (TRI == 3 && ((bm + BM - 1) >> 1) < bn) return;  // complex rows strictly above
-    if (TRI == 4 && (bm >> 1) > bn + BN - 1) return;    // complex rows strictly below
+    const int bm = tm * TM, bn = tn * TM;
+    if (TRI == 1 && bm + TM - 1 < bn) return;  // tile strictly above the diagonal
+    if (TRI == 2 && bm > bn + TM - 1) return;  // tile strictly below the diagonal
+    if (TRI == 3 && ((bm + TM - 1) >> 1) < bn) return;  // complex rows strictly above
+    if (TRI == 4 && (bm >> 1) > bn + TM - 1) return;    // complex rows strictly below
 
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
@@ -127,45 +136,45 @@ __global__ __launch_bounds__(NT, 2) void dgemm_mfma(int m, int n, int k, T alpha
     constexpr bool A_RC = TA;
     constexpr bool B_RC = !TB;
 
-    acc_t acc[4][4];
+    acc_t acc[WT][WT];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < WT; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+        for (int j = 0; j < WT; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-    T ra[LPT], rb[LPT];
+    T ra[lpt<TM>()], rb[lpt<TM>()];
     const int ktiles = (k + BK - 1) / BK;
-    load_tile<A_RC>(A, lda, bm, m, 0, k, ra, t);
-    load_tile<B_RC>(B, ldb, bn, n, 0, k, rb, t);
-    store_tile<A_RC>(As0, ra, t);
-    store_tile<B_RC>(Bs0, rb, t);
+    load_tile<A_RC, TM>(A, lda, bm, m, 0, k, ra, t);
+    load_tile<B_RC, TM>(B, ldb, bn, n, 0, k, rb, t);
+    store_tile<A_RC, TM>(As0, ra, t);
+    store_tile<B_RC, TM>(Bs0, rb, t);
     __syncthreads();
 
     const int col = lane & 15, kq = lane >> 4;
     for (int kt = 0; kt < ktiles; ++kt) {
         const bool more = (kt + 1) < ktiles;
         if (more) {
-            load_tile<A_RC>(A, lda, bm, m, (kt + 1) * BK, k, ra, t);
-            load_tile<B_RC>(B, ldb, bn, n, (kt + 1) * BK, k, rb, t);
+            load_tile<A_RC, TM>(A, lda, bm, m, (kt + 1) * BK, k, ra, t);
+            load_tile<B_RC, TM>(B, ldb, bn, n, (kt + 1) * BK, k, rb, t);
         }
         const T* As = (kt & 1) ? As1 : As0;
         const T* Bs = (kt & 1) ? Bs1 : Bs0;
 #pragma unroll
         for (int s = 0; s < BK / 4; ++s) {
             const int kk = 4 * s + kq;
-            T af[4], bf[4];
+            T af[WT], bf[WT];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) af[i] = As[kk * LDS_ROW + wm * 64 + i * 16 + col];
+            for (int i = 0; i < WT; ++i) af[i] = As[kk * LR + wm * (TM / 2) + i * 16 + col];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) bf[j] = Bs[kk * LDS_ROW + wn * 64 + j * 16 + col];
+            for (int j = 0; j < WT; ++j) bf[j] = Bs[kk * LR + wn * (TM / 2) + j * 16 + col];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < WT; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::op(bf[j], af[i], acc[i][j]);
+                for (int j = 0; j < WT; ++j) acc[i][j] = Mfma<T>::op(bf[j], af[i], acc[i][j]);
         }
         if (more) {
-            store_tile<A_RC>((kt & 1) ? As0 : As1, ra, t);
-            store_tile<B_RC>((kt & 1) ? Bs0 : Bs1, rb, t);
+            store_tile<A_RC, TM>((kt & 1) ? As0 : As1, ra, t);
+            store_tile<B_RC, TM>((kt & 1) ? Bs0 : Bs1, rb, t);
         }
         __syncthreads();
     }
@@ -173,14 +182,14 @@ __global__ __launch_bounds__(NT, 2) void dgemm_mfma(int m, int n, int k, T alpha
     // with the operands fed as (B^T, A^T), D = C^T: accumulator r of a lane holds C[row][col],
     // row = .. + (lane & 15), col = .. + Mfma<T>::drow(lane >> 4, r)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int row = bm + wm * 64 + i * 16 + col;
+    for (int i = 0; i < WT; ++i) {
+        const int row = bm + wm * (TM / 2) + i * 16 + col;
         if (row >= m) continue;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < WT; ++j) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int cc = bn + wn * 64 + j * 16 + Mfma<T>::drow(kq, r);
+                const int cc = bn + wn * (TM / 2) + j * 16 + Mfma<T>::drow(kq, r);
                 if (cc >= n) continue;
                 if (TRI == 1 && row < cc) continue;
                 if (TRI == 2 && row > cc) continue;
@@ -207,21 +216,38 @@ __global__ void scale_c(int m, int n, T beta, T* C, size_t ldc, int tri) {
     *cp = (beta == (T)0) ? (T)0 : beta * *cp;
 }
 
-template <typename T, bool TA, bool TB, int TRI>
-int launch(hipStream_t s, int m, int n, int k, T alpha, const T* A, size_t lda, const T* B, size_t ldb, T beta, T* C,
-           size_t ldc) {
+// 128-tiles unless that leaves the chip short of workgroups (fewer than two per CU).
+inline bool small_tiles(int m, int n, int tri) {
+    const long long t128 = (long long)((m + 127) / 128) * ((n + 127) / 128);
+    const long long eff = tri ? (t128 + (long long)((m + 127) / 128)) / 2 : t128;
+    return eff < 2LL * cu_count();
+}
+
+template <typename T, bool TA, bool TB, int TRI, int TM>
+int launch_tm(hipStream_t s, int m, int n, int k, T alpha, const T* A, size_t lda, const T* B, size_t ldb, T beta,
+              T* C, size_t ldc) {
     static bool attr_set = false;  // benign race: every thread sets the same value
-    auto fn = dgemm_mfma<T, TA, TB, TRI>;
-    constexpr size_t LDS_BYTES = lds_bytes<T>();
+    auto fn = dgemm_mfma<T, TA, TB, TRI, TM>;
+    constexpr size_t LDS_BYTES = lds_bytes<T, TM>();
     if (!attr_set) {
         AL_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_BYTES));
         attr_set = true;
     }
-    const int tiles_m = (m + BM - 1) / BM, tiles_n = (n + BN - 1) / BN;
-    const int ticket = prof_begin(s);
+    const int tiles_m = (m + TM - 1) / TM, tiles_n = (n + TM - 1) / TM;
     hipLaunchKernelGGL(fn, dim3(tiles_m * tiles_n), dim3(NT), LDS_BYTES, s, m, n, k, alpha, A, lda, B, ldb, beta,
                        C, ldc, tiles_m, tiles_n);
     AL_HIP(hipGetLastError());
+    return AL_OK;
+}
+
+template <typename T, bool TA, bool TB, int TRI>
+int launch(hipStream_t s, int m, int n, int k, T alpha, const T* A, size_t lda, const T* B, size_t ldb, T beta, T* C,
+           size_t ldc) {
+    const int ticket = prof_begin(s);
+    if (small_tiles(m, n, TRI))
+        AL_CHECK((launch_tm<T, TA, TB, TRI, 64>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc)));
+    else
+        AL_CHECK((launch_tm<T, TA, TB, TRI, 128>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc)));
     if (ticket >= 0) {
         // algorithmic work: 2mnk flops (half, plus the diagonal, for a triangle); bytes: A and B
         // once, C written (and read when beta != 0)

@@ -219,6 +219,9 @@ int getrf_f64(hipStream_t s, Workspace* ws, size_t m, size_t n, double* A, size_
 // The same without the cooperative-panel reservation (the caller holds one).
 int getrf_enqueue(hipStream_t s, Workspace* ws, size_t m, size_t n, double* A, size_t lda, int* ipiv, int* info,
                   double* perm);
+// LU without any row interchange of the n x n matrix A (already in a good row order), with the
+// same blocking and look-ahead; info > 0: a zero or non-finite pivot at that 1-based column.
+int getrf_nopiv_f64(hipStream_t s, Workspace* ws, size_t n, double* A, size_t lda, int* info);
 
 // One cooperative 64-wide (ncols <= 64) partial-pivoting panel of an FP32 matrix: the m x
 // ncols block at P (rows row0.. of the whole matrix), pivots written to ipiv[row0..] as

@@ -519,6 +519,61 @@ int panel_blocked(hipStream_t s, Workspace* ws, double* A, size_t lda, size_t M,
     return AL_OK;
 }
 
+// LU without pivoting of the nb x nb (nb <= 64) diagonal block at A, in place, in one
+// workgroup with the block in LDS: per column, the multipliers below the diagonal, then the
+// rank-1 update of the block right of and below it (fused multiply-adds).  A zero or
+// non-finite pivot sets info to its 1-based global column (col0 + j + 1) and stops; the
+// caller then factors with pivoting instead.  Used by the factorization in a given row order.
+__global__ __launch_bounds__(256) void getf2_nopiv_k(double* __restrict__ A, size_t lda, int nb, int col0,
+                                                     int* __restrict__ info) {
+    if (*info != 0) return;
+    __shared__ double S[PNB][PNB + 1];
+    const int t = threadIdx.x;
+    for (int idx = t; idx < nb * nb; idx += 256) S[idx % nb][idx / nb] = A[(size_t)(idx % nb) + (size_t)(idx / nb) * lda];
+    __syncthreads();
+    int done = nb;
+    for (int j = 0; j < nb; ++j) {
+        const double piv = S[j][j];
+        if (!(fabs(piv) > 0.0) || !(fabs(piv) < INFINITY)) {  // uniform: every thread reads the same value
+            if (t == 0) *info = col0 + j + 1;
+            done = j;
+            break;
+        }
+        if (t > j && t < nb) S[t][j] = S[t][j] / piv;
+        __syncthreads();
+        const int w = nb - 1 - j;
+        for (int idx = t; idx < w * w; idx += 256) {
+            const int i = j + 1 + idx % w, c = j + 1 + idx / w;
+            S[i][c] = fma(-S[i][j], S[j][c], S[i][c]);
+        }
+        __syncthreads();
+    }
+    (void)done;
+    for (int idx = t; idx < nb * nb; idx += 256) A[(size_t)(idx % nb) + (size_t)(idx / nb) * lda] = S[idx % nb][idx / nb];
+}
+
+// The m x n outer panel at (k0, k0) of A factored without pivoting, 64 columns at a time: the
+// diagonal block's LU, L21 := A21 U11^{-1}, then U12 and the in-panel trailing product on the
+// outer panel's later columns.
+int panel_nopiv(hipStream_t s, double* A, size_t lda, size_t M, size_t k0, size_t n, int* info) {
+    for (size_t kk = k0; kk < k0 + n; kk += PNB) {
+        const size_t ib = (k0 + n - kk) < (size_t)PNB ? (k0 + n - kk) : (size_t)PNB;
+        double* Akk = A + kk + kk * lda;
+        hipLaunchKernelGGL(getf2_nopiv_k, dim3(1), dim3(256), 0, s, Akk, lda, (int)ib, (int)kk, info);
+        AL_HIP(hipGetLastError());
+        if (M > kk + ib) AL_CHECK(trsm_f64(s, false, false, false, false, M - kk - ib, ib, 1.0, Akk, lda, Akk + ib, lda, info));
+        const size_t rest = k0 + n - kk - ib;
+        if (rest > 0) {
+            double* A12 = Akk + ib * lda;
+            AL_CHECK(trsm_f64(s, true, true, false, true, ib, rest, 1.0, Akk, lda, A12, lda, info));
+            if (M > kk + ib)
+                AL_CHECK(gemm_f64(s, false, false, M - kk - ib, rest, ib, -1.0, Akk + ib, lda, A12, lda, 1.0, A12 + ib,
+                                  lda));
+        }
+    }
+    return AL_OK;
+}
+
 }  // namespace
 
 int panel_f32(hipStream_t s, Workspace* ws, float* P, size_t lda, size_t row0, size_t m, size_t ncols, int* ipiv,
@@ -678,12 +733,24 @@ int ensure_side_stream(Workspace* ws) {
 // (that update covers the panel's columns), and the main stream's update waits for the panel.
 // Each stream composes the interchanges in its own pair buffer.  The pivots, and every
 // operation applied to any column, are those of the single-stream order.
-int getrf_enqueue(hipStream_t s, Workspace* ws, size_t M, size_t N, double* A, size_t lda, int* ipiv, int* info,
-                  double* perm) {
+namespace {
+
+// The blocked factorization with look-ahead; pivot false: no row interchanges at all (ipiv and
+// perm untouched), the panels factored by panel_nopiv.
+int getrf_la(hipStream_t s, Workspace* ws, size_t M, size_t N, double* A, size_t lda, int* ipiv, int* info,
+             double* perm, bool pivot) {
     const size_t kmax = M < N ? M : N;
     AL_CHECK(fill_i32(s, info, 1, 0));
     AL_CHECK(fill_u32(s, ws->abort_flag, 1, 0u));
-    if (perm) AL_CHECK(perm_init(s, perm, M));
+    if (perm && pivot) AL_CHECK(perm_init(s, perm, M));
+    auto panel = [&](hipStream_t q, size_t k, size_t jb) -> int {
+        return pivot ? panel_blocked(q, ws, A, lda, M, k, jb, ipiv, info) : panel_nopiv(q, A, lda, M, k, jb, info);
+    };
+    auto swaps = [&](hipStream_t q, size_t k, size_t jb, double* A0, size_t n0, double* A1, size_t n1, double* pv,
+                     int* pairs) -> int {
+        if (!pivot) return AL_OK;
+        return swap_block_f64(q, ws, ipiv, k, jb, A0, n0, A1, n1, lda, pv, info, pairs);
+    };
     auto update = [&](hipStream_t q, size_t k, size_t jb, size_t c0, size_t c1) -> int {
         // U12 and the trailing product of the outer panel [k, k + jb) on columns [c0, c1)
         if (c1 <= c0) return AL_OK;
@@ -697,9 +764,9 @@ int getrf_enqueue(hipStream_t s, Workspace* ws, size_t M, size_t N, double* A, s
     if (kmax <= (size_t)NBO) {
         for (size_t k = 0; k < kmax; k += NBO) {
             const size_t jb = (kmax - k) < (size_t)NBO ? (kmax - k) : (size_t)NBO;
-            AL_CHECK(panel_blocked(s, ws, A, lda, M, k, jb, ipiv, info));
+            AL_CHECK(panel(s, k, jb));
             const size_t right = (k + jb < N) ? N - k - jb : 0;
-            AL_CHECK(swap_block_f64(s, ws, ipiv, k, jb, A, k, A + (k + jb) * lda, right, lda, perm, info));
+            AL_CHECK(swaps(s, k, jb, A, k, A + (k + jb) * lda, right, perm, nullptr));
             AL_CHECK(update(s, k, jb, k + jb, N));
         }
         return AL_OK;
@@ -708,32 +775,42 @@ int getrf_enqueue(hipStream_t s, Workspace* ws, size_t M, size_t N, double* A, s
     hipStream_t side = ws->side;
     AL_HIP(hipEventRecord(ws->ev_main, s));
     AL_HIP(hipStreamWaitEvent(side, ws->ev_main, 0));
-    AL_CHECK(panel_blocked(side, ws, A, lda, M, 0, NBO, ipiv, info));
+    AL_CHECK(panel(side, 0, NBO));
     for (size_t k = 0; k < kmax; k += NBO) {
         const size_t jb = (kmax - k) < (size_t)NBO ? (kmax - k) : (size_t)NBO;
         const size_t ke = k + jb;
         if (ke >= kmax) {  // the last panel: the columns left of it (and right of kmax when N > M)
             AL_HIP(hipEventRecord(ws->ev_side, side));
             AL_HIP(hipStreamWaitEvent(s, ws->ev_side, 0));
-            AL_CHECK(swap_block_f64(s, ws, ipiv, k, jb, A, k, A + ke * lda, N - ke, lda, perm, info, ws->pairs2));
+            AL_CHECK(swaps(s, k, jb, A, k, A + ke * lda, N - ke, perm, ws->pairs2));
             AL_CHECK(update(s, k, jb, ke, N));
             break;
         }
         const size_t ke2 = (kmax - ke) < (size_t)NBO ? kmax : ke + NBO;
         // side: the next panel's columns, once the main stream's last update of them is done
         AL_HIP(hipStreamWaitEvent(side, ws->ev_main, 0));
-        AL_CHECK(swap_block_f64(side, ws, ipiv, k, jb, A + ke * lda, ke2 - ke, nullptr, 0, lda, nullptr, info,
-                                 ws->pairs));
+        AL_CHECK(swaps(side, k, jb, A + ke * lda, ke2 - ke, nullptr, 0, nullptr, ws->pairs));
         AL_CHECK(update(side, k, jb, ke, ke2));
         AL_HIP(hipEventRecord(ws->ev_side, side));
-        AL_CHECK(panel_blocked(side, ws, A, lda, M, ke, ke2 - ke, ipiv, info));
+        AL_CHECK(panel(side, ke, ke2 - ke));
         // main: every other column
         AL_HIP(hipStreamWaitEvent(s, ws->ev_side, 0));
-        AL_CHECK(swap_block_f64(s, ws, ipiv, k, jb, A, k, A + ke2 * lda, N - ke2, lda, perm, info, ws->pairs2));
+        AL_CHECK(swaps(s, k, jb, A, k, A + ke2 * lda, N - ke2, perm, ws->pairs2));
         AL_CHECK(update(s, k, jb, ke2, N));
         AL_HIP(hipEventRecord(ws->ev_main, s));
     }
     return AL_OK;
+}
+
+}  // namespace
+
+int getrf_enqueue(hipStream_t s, Workspace* ws, size_t M, size_t N, double* A, size_t lda, int* ipiv, int* info,
+                  double* perm) {
+    return getrf_la(s, ws, M, N, A, lda, ipiv, info, perm, true);
+}
+
+int getrf_nopiv_f64(hipStream_t s, Workspace* ws, size_t n, double* A, size_t lda, int* info) {
+    return getrf_la(s, ws, n, n, A, lda, nullptr, info, nullptr, false);
 }
 
 int getrs_f64(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const double* LU, size_t lda, const int* ipiv,

@@ -74,6 +74,58 @@ __global__ __launch_bounds__(256) void gather_perm_k(int n, int np, int nr, cons
     Y[(size_t)i + (size_t)k * np] = src < n ? R[(size_t)src + (size_t)k * ldr] : 0.0;
 }
 
+// Af (np x np, FP64, ld np) := P [A 0; 0 pad I] with P the FP32 factor's row origins (perm)
+// and pad as in to_f32_pad_k: the padded system in the low-precision factorization's row order.
+// One workgroup per column, so a column's scattered reads stay in one L2.
+__global__ __launch_bounds__(256) void gather_pad_k(int n, int np, const double* __restrict__ A, size_t lda,
+                                                    const double* __restrict__ perm, const u64* __restrict__ norms,
+                                                    double* __restrict__ Af) {
+    const int c = blockIdx.x;
+    const double anorm = __longlong_as_double((long long)norms[0]);
+    const double pad = (anorm > 0.0 && anorm <= DBL_MAX) ? anorm / n : 1.0;
+    for (int i = threadIdx.x; i < np; i += 256) {
+        const int src = (int)perm[i];
+        double v;
+        if (c < n)
+            v = src < n ? A[(size_t)src + (size_t)c * lda] : 0.0;
+        else
+            v = src == c ? pad : 0.0;
+        Af[(size_t)i + (size_t)c * np] = v;
+    }
+}
+
+// out[0] := min_k |U[k][k]|, out[1] := max_k |U[k][k]| over the n x n factor at U, as the bit
+// patterns of non-negative doubles (which order as integers); one workgroup.
+__global__ __launch_bounds__(1024) void diag_minmax_k(int n, const double* __restrict__ U, size_t ldu,
+                                                      u64* __restrict__ out) {
+    __shared__ u64 lo[16], hi[16];
+    u64 mn = ~0ull, mx = 0ull;
+    for (int k = threadIdx.x; k < n; k += 1024) {
+        const u64 b = (u64)__double_as_longlong(fabs(U[(size_t)k * (ldu + 1)]));
+        mn = b < mn ? b : mn;
+        mx = b > mx ? b : mx;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const u64 a = (u64)__shfl_xor((long long)mn, off, 64), c = (u64)__shfl_xor((long long)mx, off, 64);
+        mn = a < mn ? a : mn;
+        mx = c > mx ? c : mx;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        lo[threadIdx.x >> 6] = mn;
+        hi[threadIdx.x >> 6] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 16; ++w) {
+            mn = lo[w] < mn ? lo[w] : mn;
+            mx = hi[w] > mx ? hi[w] : mx;
+        }
+        out[0] = mn;
+        out[1] = mx;
+    }
+}
+
 // X (n x nr, ld ldx) := (accumulate ? X : 0) + Y[0:n).
 __global__ __launch_bounds__(256) void add_k(int n, int np, int nr, const double* __restrict__ Y,
                                              double* __restrict__ X, size_t ldx, int accumulate) {
@@ -321,7 +373,7 @@ struct Arena {
     double* dinv = nullptr;   // FP64 inverses of the factor's 128-row diagonal blocks: L's, then U's
 };
 
-constexpr int NORM_A = 0, NORM_B = 1, NORM_R = 17, NORM_X = 33, NORM_ABORT = 63, NORM_WORDS = 64;
+constexpr int NORM_A = 0, NORM_B = 1, NORM_R = 17, NORM_X = 33, NORM_DIAG = 49, NORM_ABORT = 63, NORM_WORDS = 64;
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
@@ -617,6 +669,82 @@ int fallback_fp64(hipStream_t s, Workspace* ws, Arena& a, size_t n, size_t nrhs,
     return AL_OK;
 }
 
+// The first fallback: the padded system in the low-precision factor's row order (tournament
+// pivots, chosen for A itself) factored in FP64 without any pivot search -- the row order keeps
+// the growth as small as the low-precision factorization's, so no column needs a grid-wide
+// exchange -- then FP64 refinement with that factor.  *ok reports a result within opt.tol; it is
+// false (and X unspecified) when a pivot in this order is zero or refinement stalls, and the
+// caller then runs the pivoted FP64 LU.
+int reordered_fp64(hipStream_t s, Workspace* ws, Arena& a, size_t n, size_t nrhs, const double* A, size_t lda,
+                   const double* B, size_t ldb, double* X, size_t ldx, const al_ir_options& opt, u64* hn, bool* ok,
+                   double* berr_out, int* iters_out) {
+    *ok = false;
+    const size_t np = a.np;
+    AL_CHECK(grow(&ws->fbuf, &ws->fbuf_bytes, align256(np * np * sizeof(double)) + align256(np * sizeof(int))));
+    double* Af = (double*)ws->fbuf;
+    hipLaunchKernelGGL(gather_pad_k, dim3((unsigned)np), dim3(256), 0, s, (int)n, (int)np, A, lda, a.perm, a.norms, Af);
+    AL_HIP(hipGetLastError());
+    int* info = a.minfo + 3;
+    AL_CHECK(getrf_nopiv_f64(s, ws, np, Af, np, info));
+    hipLaunchKernelGGL(diag_minmax_k, dim3(1), dim3(1024), 0, s, (int)np, Af, np, a.norms + NORM_DIAG);
+    AL_HIP(hipGetLastError());
+    int hinfo = 0;
+    AL_HIP(hipMemcpyAsync(&hinfo, info, sizeof hinfo, hipMemcpyDeviceToHost, s));
+    AL_HIP(hipMemcpyAsync(hn + NORM_DIAG, a.norms + NORM_DIAG, 2 * sizeof(u64), hipMemcpyDeviceToHost, s));
+    AL_HIP(hipStreamSynchronize(s));
+    if (hinfo != 0) return AL_OK;
+    // a pivot at rounding level in this order: the matrix may be singular, which only the
+    // pivoted factorization decides exactly (the crate's zero-pivot test)
+    if (!(as_double(hn[NORM_DIAG]) > (double)np * DBL_EPSILON * as_double(hn[NORM_DIAG + 1]))) return AL_OK;
+    double worst = 0.0;
+    int iters = 0;
+    bool all_ok = true;
+    const int max_it = opt.max_iter < 10 ? opt.max_iter : 10;
+    for (size_t c0 = 0; c0 < nrhs; c0 += NR_MAX) {
+        const size_t nr = (nrhs - c0) < (size_t)NR_MAX ? (nrhs - c0) : (size_t)NR_MAX;
+        const double* Bc = B + c0 * ldb;
+        double* Xc = X + c0 * ldx;
+        AL_CHECK(fill_u32(s, (unsigned*)(a.norms + NORM_B), 2 * (NORM_WORDS - NORM_B), 0u));
+        hipLaunchKernelGGL(colmax_k, dim3(ceil_div(n, 256)), dim3(256), 0, s, (int)n, (int)nr, Bc, ldb, a.norms + NORM_B);
+        AL_HIP(hipGetLastError());
+        hipLaunchKernelGGL(copy2d_k, dim3(ceil_div(n * nr, 256)), dim3(256), 0, s, (int)n, (int)nr, Bc, ldb, a.R, n);
+        AL_HIP(hipGetLastError());
+        double prev = INFINITY, berr = INFINITY;
+        int it = 0;
+        while (it < max_it) {
+            ++it;
+            hipLaunchKernelGGL(gather_perm_k, dim3(ceil_div(np * nr, 256)), dim3(256), 0, s, (int)n, (int)np, (int)nr,
+                               a.perm, a.R, n, a.Y);
+            AL_HIP(hipGetLastError());
+            AL_CHECK(trsv_solve<double>(s, ws, true, true, np, nr, Af, np, a.Y, np));
+            AL_CHECK(trsv_solve<double>(s, ws, false, false, np, nr, Af, np, a.Y, np));
+            hipLaunchKernelGGL(add_k, dim3(ceil_div(n * nr, 256)), dim3(256), 0, s, (int)n, (int)np, (int)nr, a.Y, Xc,
+                               ldx, it > 1 ? 1 : 0);
+            AL_HIP(hipGetLastError());
+            AL_CHECK(fill_u32(s, (unsigned*)(a.norms + NORM_R), 2 * (NORM_WORDS - NORM_R), 0u));
+            AL_CHECK(residual(s, a, nr, A, lda, Bc, ldb, Xc, ldx));
+            AL_CHECK(read_norms(s, ws, a, hn));
+            const double anorm = as_double(hn[NORM_A]);
+            berr = 0.0;
+            for (size_t k = 0; k < nr; ++k) {
+                const double e = berr_of(as_double(hn[NORM_R + k]), anorm, as_double(hn[NORM_X + k]),
+                                         as_double(hn[NORM_B + k]));
+                berr = (e > berr || std::isnan(e)) ? e : berr;
+            }
+            if (berr <= opt.stop_tol) break;
+            if (!(berr < 0.5 * prev)) break;
+            prev = berr;
+        }
+        iters += it;
+        if (!(berr <= opt.tol)) all_ok = false;
+        worst = (berr > worst || std::isnan(berr)) ? berr : worst;
+    }
+    *ok = all_ok;
+    *berr_out = worst;
+    *iters_out = iters;
+    return AL_OK;
+}
+
 }  // namespace
 
 int gesv_mixed(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const double* A, size_t lda, const double* B,
@@ -753,11 +881,24 @@ int gesv_mixed(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const double
     stats.backward_error = berr_low;
     stats.converged = ok_low ? 1 : 0;
     int status = AL_OK;
-    if (!ok_low) {
+    bool reordered_ok = false;
+    if (!ok_low && factor != AL_FACTOR_FP64 && opt.allow_fallback && stats.factor_breakdown == 0) {
+        // the low-precision factor's row order is complete: FP64 LU in that order first
+        double be2 = INFINITY;
+        int it2 = 0;
+        AL_CHECK(reordered_fp64(s, ws, a, n, nrhs, A, lda, B, ldb, X, ldx, opt, hn, &reordered_ok, &be2, &it2));
+        if (reordered_ok) {
+            stats.fell_back = 1;
+            stats.iterations += it2;
+            stats.backward_error = be2;
+            stats.converged = 1;
+        }
+    }
+    if (!ok_low && !reordered_ok) {
         if (factor == AL_FACTOR_FP64 || opt.allow_fallback) {
             int info = 0;
             AL_CHECK(fallback_fp64(s, ws, a, n, nrhs, A, lda, B, ldb, X, ldx, &info));
-            stats.fell_back = factor == AL_FACTOR_FP64 ? 0 : 1;
+            stats.fell_back = factor == AL_FACTOR_FP64 ? 0 : 2;
             if (info > 0) {
                 status = AL_SINGULAR;
                 stats.converged = 0;

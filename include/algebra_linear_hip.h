@@ -201,14 +201,16 @@ enum { AL_FACTOR_FP64 = 0, AL_FACTOR_FP32 = 1, AL_FACTOR_BF16 = 2 };
 
 typedef struct al_ir_options {
     int max_iter;        /* refinement steps (0: 30) */
-    int allow_fallback;  /* nonzero: fall back to the FP64 pivoted LU when refinement fails */
+    int allow_fallback;  /* nonzero: fall back to FP64 factorizations when refinement fails */
     double tol;          /* accepted backward error (0: 1e-14) */
     double stop_tol;     /* refinement stops once every column is below it (0: 1e-15) */
 } al_ir_options;
 
 typedef struct al_ir_stats {
     int iterations;         /* refinement steps taken (the first solve counts as one) */
-    int fell_back;          /* 1 if the FP64 pivoted LU produced the result */
+    int fell_back;          /* 0: the low-precision factor's refinement produced the result; 1: an
+                               FP64 LU in that factor's row order (no pivot search) and FP64
+                               refinement; 2: the FP64 partially pivoted LU */
     int factor_breakdown;   /* 1-based column of an exactly zero pivot of the FP32 LU, else 0 */
     int converged;          /* 1 if the returned solution meets tol */
     double backward_error;  /* max over columns of the normwise backward error of the result */

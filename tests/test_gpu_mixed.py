@@ -77,7 +77,24 @@ def test_ill_conditioned_reaches_fp64_accuracy_with_fallback_if_needed(kappa):
     assert st["converged"] == 1, st
     assert O.backward_error(a, x, b, n) <= TOL
     if kappa >= 1e11:
-        assert st["fell_back"] == 1  # kappa * u_fp32 >> 1: refinement cannot converge
+        # kappa * u_fp32 >> 1: FP32 refinement cannot converge, and the FP64 LU in the FP32
+        # factor's row order (no pivot search) with FP64 refinement produces the result
+        assert st["fell_back"] == 1
+
+
+def test_fp32_breakdown_falls_back_to_the_pivoted_fp64_lu():
+    # rows 0 and 1 differ only below FP32's resolution: the FP32 factor meets an exact zero
+    # pivot, its row order is incomplete, and the partially pivoted FP64 LU solves the system
+    n = 130
+    A = np.eye(n)
+    A[0, 0] = A[0, 1] = A[1, 0] = 1.0
+    A[1, 1] = 1.0 + 2.0 ** -30
+    A[5:, :5] = 0.25
+    a = cm(A)
+    b = normal(n, [3, 5, 7, 9])
+    x, st = solve(a, b, n, 1, "fp32")
+    assert st["factor_breakdown"] > 0 and st["fell_back"] == 2 and st["converged"] == 1, st
+    assert O.backward_error(a, x, b, n) <= TOL
 
 
 def test_refinement_failure_without_fallback_raises_convergence():

@@ -529,27 +529,25 @@ __global__ __launch_bounds__(256) void getf2_nopiv_k(double* __restrict__ A, siz
     if (*info != 0) return;
     __shared__ double S[PNB][PNB + 1];
     const int t = threadIdx.x;
-    for (int idx = t; idx < nb * nb; idx += 256) S[idx % nb][idx / nb] = A[(size_t)(idx % nb) + (size_t)(idx / nb) * lda];
+    const int i = t & (PNB - 1), q = t >> 6;  // this thread: row i, columns q, q + 4, q + 8, ..
+    for (int c = q; c < nb; c += 4) S[i][c] = i < nb ? A[(size_t)i + (size_t)c * lda] : 0.0;
     __syncthreads();
-    int done = nb;
     for (int j = 0; j < nb; ++j) {
         const double piv = S[j][j];
         if (!(fabs(piv) > 0.0) || !(fabs(piv) < INFINITY)) {  // uniform: every thread reads the same value
             if (t == 0) *info = col0 + j + 1;
-            done = j;
             break;
         }
-        if (t > j && t < nb) S[t][j] = S[t][j] / piv;
-        __syncthreads();
-        const int w = nb - 1 - j;
-        for (int idx = t; idx < w * w; idx += 256) {
-            const int i = j + 1 + idx % w, c = j + 1 + idx / w;
-            S[i][c] = fma(-S[i][j], S[j][c], S[i][c]);
+        const double l = S[i][j] / piv;  // row i's multiplier (used only below the diagonal)
+        __syncthreads();                  // every thread has read column j
+        if (i > j && i < nb) {
+            if (q == 0) S[i][j] = l;
+            for (int c = j + 1 + ((q - j - 1) & 3); c < nb; c += 4) S[i][c] = fma(-l, S[j][c], S[i][c]);
         }
         __syncthreads();
     }
-    (void)done;
-    for (int idx = t; idx < nb * nb; idx += 256) A[(size_t)(idx % nb) + (size_t)(idx / nb) * lda] = S[idx % nb][idx / nb];
+    for (int c = q; c < nb; c += 4)
+        if (i < nb) A[(size_t)i + (size_t)c * lda] = S[i][c];
 }
 
 // The m x n outer panel at (k0, k0) of A factored without pivoting, 64 columns at a time: the

@@ -156,7 +156,10 @@ inline const double* opT_block(const double* T, size_t ldt, bool trans, size_t r
 int diag_solve(hipStream_t s, int nb, int nvec, const double* T, size_t t_rs, size_t t_cs, double* X, size_t x_es,
                size_t x_vs, bool forward, bool unit, const int* info) {
     if (nvec <= 0 || nb <= 0) return AL_OK;
-    if (forward) {
+    // one wave per four vectors, except for many vectors strided across columns (the rows of a
+    // tall right-hand side): those go one per thread, which keeps the loads coalesced across
+    // threads and still fills the chip
+    if (forward && (x_es == 1 || nvec < 4096)) {
         const unsigned wgs = (unsigned)std::min<size_t>(ceil_div((size_t)nvec, (size_t)16), (size_t)512);
         if (unit)
             hipLaunchKernelGGL(trsm_fwd_wave<true>, dim3(wgs), dim3(256), 0, s, nb, nvec, T, t_rs, t_cs, X, x_es, x_vs,

@@ -5,6 +5,9 @@ Device memory comes from the HIP runtime itself (hipMalloc / hipMemcpy through c
 are generated on the host with numpy.  Workloads (--what, comma-separated):
   mixed  -- al_dev_dgesv_mixed, FP32 factor + FP64 refinement, n = 16384, one right-hand side,
             A uniform(-1, 1) (the kernels' launch shapes do not depend on the values);
+  mixedfb -- the same with row n-1 replaced by row 0 plus 1e-9 uniform(-1, 1): kappa near 1e9,
+            beyond FP32 refinement, so the solve ends in the FP64 LU in the FP32 factor's row
+            order;
   dgemm  -- al_dev_dgemm nn, m = n = k = 8192, uniform(-1, 1);
   batch  -- al_dev_batch_solve in FP64 on `--systems` systems of the batched workload's size
             distribution (orders log-uniform in [8, 512], half dgesv, half dposv, 1..8 rhs);
@@ -96,9 +99,16 @@ def main() -> int:
     dev = Device()
     rng = np.random.default_rng(1988)
     what = set(args.what.split(","))
-    if "mixed" in what:
+    for label in ("mixed", "mixedfb"):
+        if label not in what:
+            continue
         n = 16384
-        dA = dev.upload(rng.uniform(-1.0, 1.0, n * n))
+        A = rng.uniform(-1.0, 1.0, n * n)
+        if label == "mixedfb":
+            Am = A.reshape(n, n)  # Am[j, i] = A[i, j] (column-major)
+            Am[:, n - 1] = Am[:, 0] + 1e-9 * rng.uniform(-1.0, 1.0, n)
+        dA = dev.upload(A)
+        del A
         dB = dev.upload(rng.standard_normal(n))
         dX = dev.alloc(8 * n)
         opt = _lib.IrOptions()
@@ -110,7 +120,7 @@ def main() -> int:
                                         ctypes.byref(stats), None)
             if st != 0:
                 raise RuntimeError(f"dgesv_mixed {st}: {_lib.last_error()}")
-        run(mixed, args.reps, dev, f"mixed fp32 n={n}")
+        run(mixed, args.reps, dev, f"{label} fp32 n={n}")
         print(f"  refinement steps {stats.iterations}, fell back {stats.fell_back}, backward error "
               f"{stats.backward_error:.2e}", flush=True)
         dev.close()

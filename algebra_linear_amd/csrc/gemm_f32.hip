@@ -5,15 +5,17 @@
 // FP32 matrix cores do a 16x16x4 product in half the time of FP64, so per flop the operand
 // path must move twice the bytes; this kernel is built around 16-byte accesses everywhere:
 //   * one 256-thread workgroup (4 wave64s, 2x2) computes a 128x128 tile of C, K in steps of
-//     32 through double-buffered LDS (66 KiB, two workgroups per CU), one barrier per step;
-//   * global loads are float4: four rows of a column of A, four k of a column of B; A's
-//     float4 lands in LDS as one 16-byte store, B's as four scalar stores into its k-major
-//     layout (rows padded to 132 floats: 16-byte aligned, at most 2-way conflicts);
-//   * the MFMA rows and columns are permuted so a lane's operands for its wave's four 16-row
-//     (16-column) MFMA tiles are four consecutive rows (columns): one ds_read_b128 of A and
-//     one of B feed 16 MFMAs.  With that permutation, accumulator r of tile (i, j) in lane
-//     (c = l & 15, q = l >> 4) is C[16q + 4r + i][4c + j] of the wave's 64x64 block, so the
-//     epilogue reads and writes C as float4 runs down columns;
+//     32 through double-buffered LDS (69 KiB, two workgroups per CU), one barrier per step;
+//   * global loads are float4: four rows of a column of A, four k of a column of B.  A's float4
+//     lands in LDS k-major (rows padded to 132 floats); B's stays column-major in LDS (columns
+//     padded to 36 floats, an odd number of 16-byte units, so a 16-lane b128 access is
+//     conflict-free): both are single 16-byte stores, nothing is transposed;
+//   * the MFMA's k index is permuted per lane group: lane group q of a K step covers k = 8q ..
+//     8q + 7, so one ds_read_b128 of B gives a lane four consecutive k of one column (four MFMA
+//     steps), and one ds_read_b128 of A four consecutive rows (its four 16-row tiles).  Lane
+//     (c, q)'s columns are c, c + 16, c + 32, c + 48 of its wave's 64-column block, and with the
+//     row permutation accumulator r of tile (i, j) is C[16q + 4r + i][c + 16j], so the epilogue
+//     reads and writes C as float4 runs down columns;
 //   * workgroup ids are remapped so each XCD walks a contiguous, grouped range of tiles.
 // Requirements (checked by the caller, which otherwise uses the general kernel): m % 4 == 0,
 // k % 4 == 0, lda, ldb, ldc multiples of 4 and A, B, C 16-byte aligned.
@@ -25,9 +27,10 @@ namespace {
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int BM = 128, BN = 128, BK = 32, NT = 256;
-constexpr int LROW = 132;  // floats per k-row of a tile in LDS
-constexpr int TILE = BK * LROW;
-constexpr size_t LDS_BYTES = 4 * TILE * sizeof(float);  // A and B, double-buffered
+constexpr int LROW = 132;  // floats per k-row of the A tile in LDS
+constexpr int LDB = 36;    // floats per column of the B tile in LDS
+constexpr int ATILE = BK * LROW, BTILE = BN * LDB;
+constexpr size_t LDS_BYTES = 2 * (ATILE + BTILE) * sizeof(float);  // A and B, double-buffered
 
 __global__ __launch_bounds__(NT, 2) void sgemm_nn_k(int m, int n, int k, float alpha, const float* __restrict__ A,
                                                     size_t lda, const float* __restrict__ B, size_t ldb, float beta,
@@ -73,7 +76,7 @@ __global__ __launch_bounds__(NT, 2) void sgemm_nn_k(int m, int n, int k, float a
         for (int j = 0; j < 4; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int col = bn + wn * 64 + 4 * c + j, row = bm + wm * 64 + 16 * q + 4 * r;
+                const int col = bn + wn * 64 + c + 16 * j, row = bm + wm * 64 + 16 * q + 4 * r;
                 const bool ok = col < n && row < m;
                 const f4 v = *(const f4*)(C + (ok ? (size_t)row + (size_t)col * ldc : 0));
                 cv[j][r] = ok ? v : f4{0.f, 0.f, 0.f, 0.f};
@@ -88,11 +91,7 @@ __global__ __launch_bounds__(NT, 2) void sgemm_nn_k(int m, int n, int k, float a
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int f = t + NT * j;
-            float* d = Bs + (4 * (f & 7)) * LROW + (f >> 3);
-            d[0] = rb[j].x;
-            d[LROW] = rb[j].y;
-            d[2 * LROW] = rb[j].z;
-            d[3 * LROW] = rb[j].w;
+            *(f4*)(Bs + (f >> 3) * LDB + 4 * (f & 7)) = rb[j];
         }
     };
 
@@ -103,29 +102,33 @@ __global__ __launch_bounds__(NT, 2) void sgemm_nn_k(int m, int n, int k, float a
         for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 
     auto mfma_step = [&](int kt) {
-        const float* As = lds + (kt & 1) * TILE;
-        const float* Bs = lds + 2 * TILE + (kt & 1) * TILE;
+        const float* As = lds + (kt & 1) * ATILE;
+        const float* Bs = lds + 2 * ATILE + (kt & 1) * BTILE;
 #pragma unroll
-        for (int ks = 0; ks < BK / 4; ++ks) {
-            const int kk = 4 * ks + q;
-            const f4 av = *(const f4*)(As + kk * LROW + wm * 64 + 4 * c);
-            const f4 bv = *(const f4*)(Bs + kk * LROW + wn * 64 + 4 * c);
+        for (int h = 0; h < 2; ++h) {
+            f4 bq[4];  // four consecutive k (8q + 4h ..) of each of the lane's four columns
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) bq[j] = *(const f4*)(Bs + (wn * 64 + c + 16 * j) * LDB + 8 * q + 4 * h);
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i], bv[j], acc[i][j], 0, 0, 0);
+            for (int s4 = 0; s4 < 4; ++s4) {
+                const f4 av = *(const f4*)(As + (8 * q + 4 * h + s4) * LROW + wm * 64 + 4 * c);
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i], bq[j][s4], acc[i][j], 0, 0, 0);
+            }
         }
     };
 
     const int ktiles = (k + BK - 1) / BK;
     load(0);
-    store(lds, lds + 2 * TILE);
+    store(lds, lds + 2 * ATILE);
     __syncthreads();
     for (int kt = 0; kt + 1 < ktiles; ++kt) {
         load((kt + 1) * BK);
         mfma_step(kt);
-        store(lds + ((kt + 1) & 1) * TILE, lds + 2 * TILE + ((kt + 1) & 1) * TILE);
+        store(lds + ((kt + 1) & 1) * ATILE, lds + 2 * ATILE + ((kt + 1) & 1) * BTILE);
         __syncthreads();
     }
     // the last K step, peeled so C's registers are live only here
@@ -134,7 +137,7 @@ __global__ __launch_bounds__(NT, 2) void sgemm_nn_k(int m, int n, int k, float a
 
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        const int col = bn + wn * 64 + 4 * c + j;
+        const int col = bn + wn * 64 + c + 16 * j;
         if (col >= n) continue;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {

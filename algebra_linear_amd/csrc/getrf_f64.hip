@@ -201,7 +201,7 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
     constexpr int NB = PanelElem<T>::NB;
     constexpr int SLOT2 = slot2<T>();
     extern __shared__ __align__(16) unsigned char smem[];
-    T* S = (T*)smem;  // chunk x NB, column-major with leading dimension `chunk`
+    T* S = (T*)smem;  // chunk x NB, column-major with leading dimension chunk | 1
     __shared__ T lslot[2][SLOT2];  // single-workgroup exchange
     __shared__ T prow_c[NB];
     __shared__ T rj_c[NB];
@@ -215,6 +215,7 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     if (*info != 0) return;  // an earlier panel hit a zero pivot or failed
     const int rb = w * chunk;
+    const int ls = chunk | 1;  // LDS column stride: odd, so a row's entries (stride ls) fall in different banks
     const int nr = (m - rb) < chunk ? ((m - rb) > 0 ? (m - rb) : 0) : chunk;
     const bool has = t < nr;  // this thread's row: panel row rb + t
     const int g = rb + t;
@@ -225,7 +226,7 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
     }
 
     for (int c = 0; c < ncols; ++c)
-        if (has) S[c * chunk + t] = P[(size_t)g + (size_t)c * lda];
+        if (has) S[c * ls + t] = P[(size_t)g + (size_t)c * lda];
     __syncthreads();
 
     auto slot_ptr = [&](int par, int q) -> T* {
@@ -240,9 +241,9 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
             const bool ownj = jj >= rb && jj < rb + nr;
             if (COOP) {
                 if (cand)
-                    for (int c = lane; c < ncols; c += 64) st_agent(&my[2 + c], S[c * chunk + lr]);
+                    for (int c = lane; c < ncols; c += 64) st_agent(&my[2 + c], S[c * ls + lr]);
                 if (ownj)
-                    for (int c = lane; c < ncols; c += 64) st_agent(&my[2 + NB + c], S[c * chunk + (jj - rb)]);
+                    for (int c = lane; c < ncols; c += 64) st_agent(&my[2 + NB + c], S[c * ls + (jj - rb)]);
                 if (lane == 0) {
                     st_agent(&my[0], T(key));
                     st_agent(&my[1], T((K)idx));
@@ -255,9 +256,9 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             } else {
                 if (cand)
-                    for (int c = lane; c < ncols; c += 64) my[2 + c] = S[c * chunk + lr];
+                    for (int c = lane; c < ncols; c += 64) my[2 + c] = S[c * ls + lr];
                 if (ownj)
-                    for (int c = lane; c < ncols; c += 64) my[2 + NB + c] = S[c * chunk + (jj - rb)];
+                    for (int c = lane; c < ncols; c += 64) my[2 + NB + c] = S[c * ls + (jj - rb)];
                 if (lane == 0) {
                     my[0] = T(key);
                     my[1] = T((K)idx);
@@ -389,9 +390,9 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
         // swap rows j and p across the panel, record the pivot
         if (p != j) {
             if (p >= rb && p < rb + nr)
-                for (int c = t; c < ncols; c += PT) S[c * chunk + (p - rb)] = rj[c];
+                for (int c = t; c < ncols; c += PT) S[c * ls + (p - rb)] = rj[c];
             if (j >= rb && j < rb + nr)
-                for (int c = t; c < ncols; c += PT) S[c * chunk + (j - rb)] = prow[c];
+                for (int c = t; c < ncols; c += PT) S[c * ls + (j - rb)] = prow[c];
         }
         if (w == 0 && t == 0) ipiv[row0 + j] = row0 + p;
         __syncthreads();
@@ -403,10 +404,10 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
             K key = (K)-2;
             int idx = 0x7fffffff;
             if (mine) {
-                const T f = S[j * chunk + t] / d;
-                S[j * chunk + t] = f;
-                const T v = S[(j + 1) * chunk + t] - f * prow[j + 1];
-                S[(j + 1) * chunk + t] = v;
+                const T f = S[j * ls + t] / d;
+                S[j * ls + t] = f;
+                const T v = S[(j + 1) * ls + t] - f * prow[j + 1];
+                S[(j + 1) * ls + t] = v;
                 key = pivot_key(v, g, j + 1);
                 idx = g;
             }
@@ -415,31 +416,31 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
             const int lc = (key > (K)-2) ? idx - rb : -1;
             const int ln = (j + 1 >= rb && j + 1 < rb + nr) ? j + 1 - rb : -1;
             for (int c = j + 2 + t; c < ncols; c += PT) {
-                if (lc >= 0) S[c * chunk + lc] -= S[j * chunk + lc] * prow[c];
-                if (ln >= 0 && ln != lc) S[c * chunk + ln] -= S[j * chunk + ln] * prow[c];
+                if (lc >= 0) S[c * ls + lc] -= S[j * ls + lc] * prow[c];
+                if (ln >= 0 && ln != lc) S[c * ls + ln] -= S[j * ls + ln] * prow[c];
             }
             __syncthreads();
             publish(j + 1, key, idx);
             if (mine && t != lc && t != ln) {
-                const T f = S[j * chunk + t];
+                const T f = S[j * ls + t];
                 int c = j + 2;
                 for (; c + 8 <= ncols; c += 8) {
                     T v[8];
 #pragma unroll
-                    for (int q = 0; q < 8; ++q) v[q] = S[(c + q) * chunk + t];
+                    for (int q = 0; q < 8; ++q) v[q] = S[(c + q) * ls + t];
 #pragma unroll
-                    for (int q = 0; q < 8; ++q) S[(c + q) * chunk + t] = v[q] - f * prow[c + q];
+                    for (int q = 0; q < 8; ++q) S[(c + q) * ls + t] = v[q] - f * prow[c + q];
                 }
-                for (; c < ncols; ++c) S[c * chunk + t] -= f * prow[c];
+                for (; c < ncols; ++c) S[c * ls + t] -= f * prow[c];
             }
             __syncthreads();
         } else {
-            if (mine) S[j * chunk + t] = S[j * chunk + t] / d;
+            if (mine) S[j * ls + t] = S[j * ls + t] / d;
             __syncthreads();
         }
     }
     for (int c = 0; c < ncols; ++c)
-        if (has) P[(size_t)g + (size_t)c * lda] = S[c * chunk + t];
+        if (has) P[(size_t)g + (size_t)c * lda] = S[c * ls + t];
 }
 #pragma clang fp contract(on)
 
@@ -450,9 +451,9 @@ int launch_panel(hipStream_t s, Workspace* ws, T* Ppanel, size_t lda, size_t row
     static bool attr_set = false;
     if (!attr_set) {
         AL_HIP(hipFuncSetAttribute((const void*)panel_getf2<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)(CHUNK_MAX * NB * sizeof(T))));
+                                   (int)((CHUNK_MAX + 1) * NB * sizeof(T))));
         AL_HIP(hipFuncSetAttribute((const void*)panel_getf2<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)(CHUNK_MAX * NB * sizeof(T))));
+                                   (int)((CHUNK_MAX + 1) * NB * sizeof(T))));
         attr_set = true;
     }
     if (ncols > (size_t)NB) {
@@ -477,11 +478,11 @@ int launch_panel(hipStream_t s, Workspace* ws, T* Ppanel, size_t lda, size_t row
     ws->gen += (unsigned)ncols;
     const int ticket = prof_begin(s);
     if (nwg > 1) {
-        hipLaunchKernelGGL((panel_getf2<T, true>), dim3((unsigned)nwg), dim3(PT), chunk * NB * sizeof(T), s, Ppanel,
+        hipLaunchKernelGGL((panel_getf2<T, true>), dim3((unsigned)nwg), dim3(PT), (chunk | 1) * NB * sizeof(T), s, Ppanel,
                            lda, (int)row0, (int)m, (int)ncols, (int)chunk, ipiv, info, (unsigned long long*)ws->flags, ws->abort_flag,
                            (T*)ws->slots, gen0, timeout);
     } else {
-        hipLaunchKernelGGL((panel_getf2<T, false>), dim3(1), dim3(PT), chunk * NB * sizeof(T), s, Ppanel, lda,
+        hipLaunchKernelGGL((panel_getf2<T, false>), dim3(1), dim3(PT), (chunk | 1) * NB * sizeof(T), s, Ppanel, lda,
                            (int)row0, (int)m, (int)ncols, (int)chunk, ipiv, info, (unsigned long long*)ws->flags, ws->abort_flag,
                            (T*)ws->slots, gen0, timeout);
     }

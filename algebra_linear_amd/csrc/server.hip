@@ -13,13 +13,16 @@
 // resident factorization involved) also takes every other waiting small request of the same
 // precision that fits the budget, and solves them all with one persistent batched launch.
 // A request may keep its factorization resident (its footprint stays reserved) and later
-// requests may solve new right-hand sides with it.
+// requests may solve new right-hand sides with it.  A released factor's device block goes to a
+// pool that later keep requests draw from (held blocks are capped at a quarter of the budget),
+// and the workers' buffers only grow, so a steady request stream allocates no device memory.
 //
 // Every path is deterministic: a request's result is bit-for-bit the same alone or in a mix.
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -53,7 +56,50 @@ struct Resident {
     int* ipiv = nullptr;     // FP64 LU pivots
     MixedFactor mf;          // mixed: FP32 LU and row origins
     void* mem = nullptr;     // one allocation holding all of the above
+    size_t mem_bytes = 0;    // its size
     size_t footprint = 0;
+};
+
+// Device blocks of released resident factors, kept for later keep requests so that steady
+// keep/release traffic allocates nothing: a request takes the smallest kept block of at least
+// its size (and at most twice it); blocks beyond a quarter of the HBM budget are freed.
+struct FactorPool {
+    std::mutex mu;
+    std::multimap<size_t, void*> blocks;
+    size_t held = 0, cap = 0;
+    hipError_t take(size_t bytes, void** p, size_t* got) {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            auto it = blocks.lower_bound(bytes);
+            if (it != blocks.end() && it->first <= 2 * bytes) {
+                *p = it->second;
+                *got = it->first;
+                held -= it->first;
+                blocks.erase(it);
+                return hipSuccess;
+            }
+        }
+        *got = bytes;
+        return hipMalloc(p, bytes);
+    }
+    void give(void* p, size_t bytes) {
+        if (!p) return;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            if (held + bytes <= cap) {
+                blocks.emplace(bytes, p);
+                held += bytes;
+                return;
+            }
+        }
+        (void)hipFree(p);
+    }
+    void clear() {
+        std::lock_guard<std::mutex> lk(mu);
+        for (auto& kv : blocks) (void)hipFree(kv.second);
+        blocks.clear();
+        held = 0;
+    }
 };
 
 // Grow-only device buffer.
@@ -100,6 +146,7 @@ struct al_server {
     std::deque<al::Pending> queue;
     std::unordered_map<uint64_t, al_result> results;
     std::unordered_map<int64_t, al::Resident> factors;
+    al::FactorPool pool;
     uint64_t next_ticket = 1;
     int64_t next_factor = 1;
     int running = 0;
@@ -301,7 +348,7 @@ struct Worker {
                         const size_t n = r.n, np = mixed_padded(n);
                         size_t bytes = n * n * 8 + ((n * 4 + 255) / 256) * 256;
                         if (r.factor == AL_FACTOR_FP32) bytes += np * np * 4 + np * 8 + 256;
-                        hipError_t e = hipMalloc(&kept.mem, bytes);
+                        hipError_t e = S->pool.take(bytes, &kept.mem, &kept.mem_bytes);
                         if (e != hipSuccess) {
                             (void)hipGetLastError();
                             set_error("server: device allocation failed for a resident factor");
@@ -327,7 +374,7 @@ struct Worker {
                     if (rc == AL_OK) rc = solve_one(r, out[0], reuse, keep);
                     // only a factorization that completed (status AL_OK) may be reused
                     kept_ok = keep && rc == AL_OK && out[0].status == AL_OK;
-                    if (keep && !kept_ok) (void)hipFree(kept.mem);
+                    if (keep && !kept_ok) S->pool.give(kept.mem, kept.mem_bytes);
                 }
             }
             const double t_end = now_ms();
@@ -378,6 +425,7 @@ extern "C" al_server* al_server_create(size_t hbm_budget_bytes, int slots) {
     al_server* S = new al_server;
     (void)hipGetDevice(&S->device);
     S->budget = hbm_budget_bytes;
+    S->pool.cap = hbm_budget_bytes / 4;
     const int k = slots < 1 ? 1 : (slots > 16 ? 16 : slots);
     for (int i = 0; i < k; ++i)
         S->workers.emplace_back([S] {
@@ -438,6 +486,7 @@ extern "C" int al_server_release_factor(al_server* S, int64_t factor_id) {
         return AL_ERR_ARG;
     }
     void* mem = nullptr;
+    size_t bytes = 0;
     {
         std::lock_guard<std::mutex> lk(S->mu);
         auto it = S->factors.find(factor_id);
@@ -446,10 +495,11 @@ extern "C" int al_server_release_factor(al_server* S, int64_t factor_id) {
             return AL_ERR_ARG;
         }
         mem = it->second.mem;
+        bytes = it->second.mem_bytes;
         S->in_use -= it->second.footprint;
         S->factors.erase(it);
     }
-    (void)hipFree(mem);
+    S->pool.give(mem, bytes);
     S->cv_work.notify_all();
     return AL_OK;
 }
@@ -463,5 +513,6 @@ extern "C" void al_server_destroy(al_server* S) {
     S->cv_work.notify_all();
     for (auto& t : S->workers) t.join();
     for (auto& kv : S->factors) (void)hipFree(kv.second.mem);
+    S->pool.clear();
     delete S;
 }

@@ -109,6 +109,31 @@ def test_resident_factor_serves_new_right_hand_sides(factor, kind):
         assert r3["status"] < 0  # released: unknown factor
 
 
+@pytest.mark.parametrize("factor", [serve.FP64, serve.FP32])
+def test_released_factor_blocks_serve_later_keeps(factor):
+    # keep, release, keep a slightly smaller system (its block comes from the released one),
+    # keep a much smaller one (a fresh block): every factor solves its own system, and the
+    # reused block holds nothing of the earlier factor
+    with serve.Server(slots=1) as srv:
+        ids = []
+        for n, seed in ((600, 21), (560, 22), (200, 23)):
+            a, b = system(n, 1, serve.GESV, seed)
+            x, res = srv.wait(srv.submit(a, b, n, 1, serve.GESV, factor, keep=True))
+            assert res["status"] == 0 and res["factor_id"] >= 0
+            check(a, x, b, n, 1, serve.GESV)
+            rng = np.random.default_rng(seed)
+            b2 = rng.standard_normal(n * 2)
+            x2, r2 = srv.wait(srv.submit(None, b2, n, 2, serve.GESV, factor, factor_id=res["factor_id"]))
+            assert r2["status"] == 0
+            check(a, x2, b2, n, 2, serve.GESV)
+            if n == 600:
+                srv.release(res["factor_id"])
+            else:
+                ids.append(res["factor_id"])
+        for fid in ids:
+            srv.release(fid)
+
+
 def test_requests_larger_than_the_budget_still_run_alone():
     with serve.Server(hbm_budget_bytes=1 << 20, slots=3) as srv:
         specs = [(800, 1), (20, 2), (900, 1)]

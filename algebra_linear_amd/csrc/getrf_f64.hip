@@ -143,30 +143,36 @@ __device__ __forceinline__ unsigned long long flag_word(unsigned gen, int idx, u
     return ((unsigned long long)(gen & 0xFFFFu) << 48) | ((unsigned long long)(idx & 0xFFFF) << 32) | head;
 }
 
-// Workgroup-wide first maximum: the largest key, ties to the lowest row.  Two LDS atomics
-// (max of the key's integer image, then min of the row among the maximal keys) instead of a
-// shuffle tree.  `red` alternates between two slots per call so the next call's reset needs
-// no extra barrier (callers always pass a workgroup barrier between two calls); `call`
-// counts the calls.
-struct ArgmaxLds {
-    unsigned long long key[2];
-    int idx[2];
-};
-template <typename T>
-__device__ __forceinline__ void block_argmax(T& key, int& idx, ArgmaxLds* red, int& call) {
-    const int s = call & 1;
-    const unsigned long long code = key_code(key);
-    atomicMax(&red->key[s], code);
-    __syncthreads();
-    if (code == red->key[s]) atomicMin(&red->idx[s], idx);
-    __syncthreads();
-    key = key_value<T>(red->key[s]);
-    idx = red->idx[s];
-    if (threadIdx.x == 0) {  // the other slot is no longer read by anyone: reset it
-        red->key[s ^ 1] = 0ull;
-        red->idx[s ^ 1] = 0x7fffffff;
+// Workgroup-wide first maximum in every thread: the largest key, ties to the lowest row.  A
+// shuffle tree per wave, then the four waves' winners in LDS
+// behind one barrier; `par` alternates between calls (callers pass a workgroup barrier between
+// two calls, so a call's writes never meet the previous call's reads).
+template <typename K>
+__device__ __forceinline__ void wg_argmax(K& key, int& idx, K (*wk)[PT / 64], int (*wi)[PT / 64], int& par) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const K ok = __shfl_xor(key, off, 64);
+        const int oi = __shfl_xor(idx, off, 64);
+        if (better(ok, oi, key, idx)) {
+            key = ok;
+            idx = oi;
+        }
     }
-    ++call;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        wk[par][wave] = key;
+        wi[par][wave] = idx;
+    }
+    __syncthreads();
+    key = wk[par][0];
+    idx = wi[par][0];
+#pragma unroll
+    for (int q = 1; q < PT / 64; ++q)
+        if (better(wk[par][q], wi[par][q], key, idx)) {
+            key = wk[par][q];
+            idx = wi[par][q];
+        }
+    par ^= 1;
 }
 
 #pragma clang fp contract(off)
@@ -205,7 +211,8 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
     __shared__ T lslot[2][SLOT2];  // single-workgroup exchange
     __shared__ T prow_c[NB];
     __shared__ T rj_c[NB];
-    __shared__ ArgmaxLds red;
+    __shared__ K wk[2][PT / 64];
+    __shared__ int wi[2][PT / 64];
     __shared__ int win_lds;
     __shared__ K wkey_lds;
     __shared__ int ok_lds;
@@ -219,11 +226,7 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
     const int nr = (m - rb) < chunk ? ((m - rb) > 0 ? (m - rb) : 0) : chunk;
     const bool has = t < nr;  // this thread's row: panel row rb + t
     const int g = rb + t;
-    int calls = 0;
-    if (t == 0) {
-        red.key[0] = red.key[1] = 0ull;
-        red.idx[0] = red.idx[1] = 0x7fffffff;
-    }
+    int argpar = 0;
 
     for (int c = 0; c < ncols; ++c)
         if (has) S[c * ls + t] = P[(size_t)g + (size_t)c * lda];
@@ -370,7 +373,7 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
     {
         K key = has ? pivot_key(S[t], g, 0) : (K)-2;
         int idx = has ? g : 0x7fffffff;
-        block_argmax(key, idx, &red, calls);
+        wg_argmax(key, idx, wk, wi, argpar);
         publish(0, key, idx);
     }
 
@@ -411,7 +414,7 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
                 key = pivot_key(v, g, j + 1);
                 idx = g;
             }
-            block_argmax(key, idx, &red, calls);
+            wg_argmax(key, idx, wk, wi, argpar);
             // bring the candidate's row and row j+1 up to date, publish, then finish the rest
             const int lc = (key > (K)-2) ? idx - rb : -1;
             const int ln = (j + 1 >= rb && j + 1 < rb + nr) ? j + 1 - rb : -1;

@@ -148,12 +148,12 @@ def test_dpotrf_at_config_size():
         x = rng.uniform(-1, 1, n)
         r = A @ x - L @ (L.T @ x)
         assert np.abs(r).max() <= 4 * n * EPS * anorm * np.abs(x).max()
-    # the leading 64 x 64 block is factored with the crate's own arithmetic
-    st, l64 = O.dpotrf(np.ascontiguousarray(a_low[:64, :64].T).ravel(), 64)
+    # the leading 128 x 128 block is factored with the crate's own arithmetic
+    st, l128 = O.dpotrf(np.ascontiguousarray(a_low[:128, :128].T).ravel(), 128)
     assert st == 0
-    got = lf[:64, :64].T.ravel()                # column-major
-    low = np.tril(np.ones((64, 64), bool)).T.ravel()
-    assert np.array_equal(got[low], l64[low])
+    got = lf[:128, :128].T.ravel()              # column-major
+    low = np.tril(np.ones((128, 128), bool)).T.ravel()
+    assert np.array_equal(got[low], l128[low])
 
 
 # ------------------------------------------------------------------------------------------

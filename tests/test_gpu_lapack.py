@@ -158,7 +158,7 @@ def test_dgetrf_rect_is_bit_exact_within_one_panel(nrows, ncols):
     assert np.array_equal(a, lu)
 
 
-@pytest.mark.parametrize("n", [1, 2, 5, 9, 33, 64])
+@pytest.mark.parametrize("n", [1, 2, 5, 9, 33, 64, 65, 100, 128])
 def test_dpotrf_is_bit_exact_within_one_block(n):
     a0 = spd(n) if n < 10 else spd_gg(n)
     a = a0.copy()
@@ -265,7 +265,7 @@ def test_dpotrf_not_spd_in_a_later_block():
     n = 200
     a0 = spd_gg(n)
     A = mat(a0, n, n)
-    A[150, 150] = -1e6  # the failing pivot is beyond the first 64-block
+    A[150, 150] = -1e6  # the failing pivot is beyond the first 128-block
     with pytest.raises(al.LapackError) as e:
         lapack.dpotrf(cm(A), n)
     assert e.value.kind == al.SINGULAR

@@ -45,40 +45,26 @@ __device__ __forceinline__ double rsqrt_nr(double d) {
     return r;
 }
 
-// Row j = j0 + jj of W (its entries s <= j) from the slots of one lane: a whole group of columns
-// before j0, or the slots <= jj of the group j0 / 16.
-__device__ __forceinline__ void publish_w_row(double* c, const double (&x)[NS], int s0, int j0, int jj) {
-    if (s0 < j0) {  // uniform
-#pragma unroll
-        for (int m = 0; m < NS; ++m) c[s0 + m] = x[m];
-    } else if (s0 == j0) {  // uniform
-#pragma unroll
-        for (int m = 0; m < NS; ++m)
-            if (m <= jj) c[s0 + m] = x[m];
-    }
-}
-
 // The nb x nb (nb <= 128) diagonal block at A (global columns col0..), factored right-looking with
 // its elements in registers; with INV the same sweep forms the inverse W = L^{-1} (nb x nb, zeros
-// above the diagonal, leading dimension ldw).  Thread (p, g) = (t & 63, t >> 6) owns rows p and
-// p + 64 at the columns s = 16g + m, m < 16.  A slot holds L[i][s] until column s is factored and
-// W[i][s] from then on: at step j every slot of a row i > j loses l_ij u_s, where u_s is l_sj for
-// s > j (the factor's update), 1 / l_jj for s = j (the slot was zeroed when column j was handed
-// over, so it becomes W[i][j] = -l_ij W[j][j]) and W[j][s] for s < j (forward substitution of
-// L W = I).
-// The only values a step exchanges are in one LDS array `c`: row j of W before the division by
-// l_jj at s < j, 1 at s = j, column j of L before the division at s > j (and the pivot's sum in
-// `dg`).  Each wave reads the 16 values of its columns once (one lane each), scales them and
-// broadcasts them through scalar registers, so a step has one barrier; two copies of the array
-// alternate between steps so that handing the next column over cannot overwrite what a slower
-// wave still reads.  The step loop is unrolled by 16 so that the slot of the column handed over is
-// known at compile time.  Column j of L and row j of W go to memory as soon as they are known, so
-// the rows <= j are dead in registers and the update runs on every lane without a test.
+// above the diagonal, leading dimension ldw).  Thread (p, g) = (t & 63, t >> 6) owns rows 2p and
+// 2p + 1 at the columns s = 16g + m, m < 16.  A slot holds L[i][s] until column s is factored and
+// then l_ii W[i][s]: at step j every slot of a row i > j loses l_ij u_s, where u_s is l_sj for s > j
+// (the factor's update), 1 / l_jj for s = j (the slot was zeroed when column j was handed over)
+// and W[j][s] for s < j (forward substitution of L W = I); rows <= j keep their slots, which at
+// the end are scaled by their 1 / l_ii into W.
+// Per step the waves exchange only column j of L before the division by l_jj, in one LDS array
+// `c` (1 at s = j; the pivot's sum in `dg`); two copies alternate between steps so that handing
+// the next column over cannot overwrite what a slower wave still reads, and a step has one
+// barrier.  A wave reads the values of its 16 columns once (one lane each) and broadcasts them
+// through scalar registers; l_jj W[j][s] it takes from its own lane j / 2.  Without EXACT both
+// kinds enter unscaled and the row's multiplier is c_i / l_jj^2 = l_ij / l_jj.  The step loop is
+// unrolled by 16 so that the slot handed over and the register array of row j are known at
+// compile time.
 // With EXACT every element of L gets the crate's operations in the crate's order -- the
 // subtractions l_ij l_sj for j = 0, 1, .. as a separate multiply and subtract, then the division by
 // l_ss (the diagonal: the same subtractions, then the square root) -- so a block that no update
-// has touched factors bit for bit as lapack::dpotrf factors it.  Without, the update is one FMA
-// and the divisions are products with 1 / l_jj.
+// has touched factors bit for bit as lapack::dpotrf factors it.  Without, the update is one FMA.
 // A non-positive pivot at column j sets info to col0 + j + 1 and leaves A's columns j.. as given
 // (those before j factored).
 template <bool EXACT, bool INV>
@@ -86,23 +72,24 @@ __global__ __launch_bounds__(DT) void potrf_diag_k(double* __restrict__ A, size_
                                                   double* __restrict__ W, size_t ldw, int* __restrict__ info) {
     if (*info != 0) return;
     __shared__ double c[2][DB], dg[2];
-    const int t = threadIdx.x, p = t & 63, ph = p + 64;
+    const int t = threadIdx.x, p = t & 63, r0 = 2 * p, r1 = r0 + 1;
     const int g = __builtin_amdgcn_readfirstlane(t >> 6);  // column group, uniform per wave
     const int s0 = 16 * g;
-    double xl[NS], xh[NS];  // rows p and p + 64 (row p has no columns past 63: groups >= 4 keep zeros)
+    double x0[NS], x1[NS];  // rows 2p and 2p + 1
 #pragma unroll
     for (int m = 0; m < NS; ++m) {
         const int s = s0 + m;
-        xl[m] = (p < nb && s <= p) ? A[(size_t)p + (size_t)s * lda] : 0.0;
-        xh[m] = (ph < nb && s <= ph) ? A[(size_t)ph + (size_t)s * lda] : 0.0;
+        x0[m] = (r0 < nb && s <= r0) ? A[(size_t)r0 + (size_t)s * lda] : 0.0;
+        x1[m] = (r1 < nb && s <= r1) ? A[(size_t)r1 + (size_t)s * lda] : 0.0;
     }
     if (g == 0) {  // hand column 0 over
-        c[0][p] = (p == 0) ? 1.0 : xl[0];
-        c[0][ph] = xh[0];
-        if (p == 0) dg[0] = xl[0];
+        c[0][r0] = (r0 == 0) ? 1.0 : x0[0];
+        c[0][r1] = x1[0];
+        if (r0 == 0) dg[0] = x0[0];
     }
-    xl[0] = (g == 0) ? 0.0 : xl[0];
-    xh[0] = (g == 0) ? 0.0 : xh[0];
+    x0[0] = (g == 0) ? 0.0 : x0[0];
+    x1[0] = (g == 0) ? 0.0 : x1[0];
+    double q0 = 0.0, q1 = 0.0;  // 1 / l_ii of the two rows, once factored
 #pragma nounroll
     for (int j0 = 0; j0 < nb; j0 += 16) {
 #pragma unroll
@@ -124,45 +111,59 @@ __global__ __launch_bounds__(DT) void potrf_diag_k(double* __restrict__ A, size_
                 r = rsqrt_nr(d);
                 ljj = d * r;
             }
-            // u_s for this wave's slots and l_ij of its two rows (c[s] / l_jj); a dead row reads an
-            // entry of no meaning and its result is never used
-            const double cv = c[b][s0 + (p & 15)];  // lane m < 16: this group's column s0 + m
-            const double sv = EXACT ? cv / ljj : cv * r;
-            double u[NS];  // wave-uniform (scalar registers)
+            q0 = (r0 == j) ? r : q0;
+            q1 = (r1 == j) ? r : q1;
+            if (t < nb && t >= j)  // column j of L is final
+                A[(size_t)t + (size_t)j * lda] = (t == j) ? ljj : (EXACT ? c[b][t] / ljj : c[b][t] * r);
+            // the rows' multipliers (0 for the rows done, which keep their slots) and u_s
+            const double c0 = c[b][r0], c1 = c[b][r1];
+            const double cv = c[b][s0 + (p & 15)];  // lane m < 16: column s0 + m of L (1 at j)
+            double m0, m1, u[NS];
+            const double* xj = (jj & 1) ? x1 : x0;  // row j's slots, in lane j / 2
+            const int jl = j >> 1;
+            // EXACT: multipliers l_ij, u_s = l_sj and W[j][s] = (l_jj W[j][s]) / l_jj; otherwise
+            // multipliers l_ij / l_jj and u_s unscaled
+            const double r2 = r * r;
+            m0 = (r0 > j) ? (EXACT ? c0 / ljj : c0 * r2) : 0.0;
+            m1 = (r1 > j) ? (EXACT ? c1 / ljj : c1 * r2) : 0.0;
+            const double sv = EXACT ? cv / ljj : cv;
+            if (s0 + NS <= j) {  // uniform: every slot of this group is a W slot
 #pragma unroll
-            for (int m = 0; m < NS; ++m) u[m] = lane_d(sv, m);
-            const double lh = EXACT ? c[b][ph] / ljj : c[b][ph] * r;
-            const double ll = EXACT ? c[b][p] / ljj : c[b][p] * r;
-            if (t < nb) {  // column j of L and row j of W are final
-                const double v = EXACT ? c[b][t] / ljj : c[b][t] * r;
-                if (t >= j) A[(size_t)t + (size_t)j * lda] = (t == j) ? ljj : v;
-                if (INV) W[(size_t)j + (size_t)t * ldw] = (t <= j) ? v : 0.0;
+                for (int m = 0; m < NS; ++m) u[m] = EXACT ? lane_d(xj[m], jl) * r : lane_d(xj[m], jl);
+            } else if (s0 > j) {  // uniform: every slot is an L slot
+#pragma unroll
+                for (int m = 0; m < NS; ++m) u[m] = lane_d(sv, m);
+            } else {  // the group of column j: W slots below jj
+#pragma unroll
+                for (int m = 0; m < NS; ++m)
+                    u[m] = (m < jj) ? (EXACT ? lane_d(xj[m], jl) * r : lane_d(xj[m], jl)) : lane_d(sv, m);
             }
 #pragma unroll
             for (int m = 0; m < NS; ++m) {
-                xh[m] = EXACT ? xh[m] - lh * u[m] : fma(-lh, u[m], xh[m]);
-                xl[m] = EXACT ? xl[m] - ll * u[m] : fma(-ll, u[m], xl[m]);
+                x0[m] = EXACT ? x0[m] - m0 * u[m] : fma(-m0, u[m], x0[m]);
+                x1[m] = EXACT ? x1[m] - m1 * u[m] : fma(-m1, u[m], x1[m]);
             }
             // hand column j + 1 over: group (j + 1) / 16, slot (jj + 1) % 16, rows > j; keep a zero
             const int jn = j + 1;
             const int mh = (jj + 1) & 15;     // a constant in each unrolled step
             const bool own = g == (jn >> 4);  // uniform
             if (own) {
-                if (ph > j) c[bn][ph] = (ph == jn) ? 1.0 : xh[mh];
-                if (p > j) c[bn][p] = (p == jn) ? 1.0 : xl[mh];
-                if (ph == jn) dg[bn] = xh[mh];
-                if (p == jn) dg[bn] = xl[mh];
+                if (r0 > j) c[bn][r0] = (r0 == jn) ? 1.0 : x0[mh];
+                if (r1 > j) c[bn][r1] = (r1 == jn) ? 1.0 : x1[mh];
+                if (r0 == jn) dg[bn] = x0[mh];
+                if (r1 == jn) dg[bn] = x1[mh];
             }
-            xh[mh] = own ? 0.0 : xh[mh];
-            xl[mh] = own ? 0.0 : xl[mh];
-            if (INV) {  // row j + 1 of W into c[s], s <= j: one lane per wave
-                if (jn >= 64) {  // uniform: the row is a p + 64 row
-                    if (ph == jn) publish_w_row(c[bn], xh, s0, j0, jj);
-                } else if (p == jn) {
-                    publish_w_row(c[bn], xl, s0, j0, jj);
-                }
-            }
+            x0[mh] = own ? 0.0 : x0[mh];
+            x1[mh] = own ? 0.0 : x1[mh];
         }
+    }
+    if (!INV) return;
+#pragma unroll
+    for (int m = 0; m < NS; ++m) {
+        const int s = s0 + m;
+        if (s >= nb) break;  // uniform
+        if (r0 < nb) W[(size_t)r0 + (size_t)s * ldw] = (s < r0) ? x0[m] * q0 : ((s == r0) ? q0 : 0.0);
+        if (r1 < nb) W[(size_t)r1 + (size_t)s * ldw] = (s < r1) ? x1[m] * q1 : ((s == r1) ? q1 : 0.0);
     }
 }
 #pragma clang fp contract(on)

@@ -6,9 +6,9 @@
 //     elements in registers (`potrf_diag_k`); the same sweep forms the block's inverse L11^{-1}.
 //     The leading block -- the whole matrix when n <= 128 -- is factored with the crate's own
 //     operations in the crate's order, so it matches lapack::dpotrf bit for bit;
-//   * the panel below the block, A21 := A21 L11^{-T}, is one FP64 MFMA product with that inverse
-//     (on a copy of A21), and the trailing matrix gets a lower-triangle-only FP64 MFMA rank-128
-//     update.
+//   * the panel below the block, A21 := A21 L11^{-T}, is one FP64 MFMA product with that inverse,
+//     in place (a workgroup owns whole rows of A21), and the trailing matrix gets a
+//     lower-triangle-only FP64 MFMA update.
 // The blocks of the look-ahead are 256 wide (two diagonal blocks, so that the trailing update has
 // depth 256).  Look-ahead on two streams: once block k is factored, the caller's stream applies its
 // panel to the next block's columns first; the side stream then factors that block (diagonal
@@ -177,6 +177,80 @@ __global__ void copy_cols_k(int m, int n, const double* __restrict__ S, size_t l
     D[(size_t)i + (size_t)c * ldd] = S[(size_t)i + (size_t)c * lds];
 }
 
+// The panel below a diagonal block in place: A21 (m x 128) := A21 W^T with W = L11^{-1} (128 x 128,
+// column-major, leading dimension ldw).  A workgroup owns 64 whole rows: it copies them to LDS,
+// and wave w forms output columns [32w, 32w + 32) with v_mfma_f64_16x16x4_f64, W's entries from
+// L2.  The operands go in as (W, A21^T) so that each lane's accumulators hold 16 consecutive
+// rows of one output column.
+constexpr int PR = 64;        // rows per workgroup
+constexpr int PLD = DB + 1;   // LDS row stride (odd: a step's 16 rows x 4 columns fall in distinct banks)
+typedef double pd4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void panel_trmm_k(int m, double* __restrict__ A, size_t lda,
+                                                    const double* __restrict__ W, size_t ldw,
+                                                    const int* __restrict__ info) {
+    if (*info != 0) return;
+    extern __shared__ double As[];  // PR x DB, row-major with stride PLD
+    const int t = threadIdx.x, l = t & 63, w = t >> 6;
+    const int r0 = blockIdx.x * PR;
+    const int rows = min(PR, m - r0);
+    const int lr = l & 15, lk = l >> 4;
+    // every load issued before the first wait: this wave's W entries, then the rows
+    double b[DB / 4][2];
+#pragma unroll
+    for (int s = 0; s < DB / 4; ++s)
+#pragma unroll
+        for (int jt = 0; jt < 2; ++jt) b[s][jt] = W[(size_t)(32 * w + 16 * jt + lr) + (size_t)(4 * s + lk) * ldw];
+    double v[PR * DB / 256];
+    const int i = t & (PR - 1), kq = t / PR;  // row, and the first of the columns kq + 4e
+#pragma unroll
+    for (int e = 0; e < PR * DB / 256; ++e)
+        v[e] = i < rows ? A[(size_t)(r0 + i) + (size_t)(kq + 4 * e) * lda] : 0.0;
+#pragma unroll
+    for (int e = 0; e < PR * DB / 256; ++e) As[i * PLD + kq + 4 * e] = v[e];
+    __syncthreads();
+    pd4 acc[4][2];
+#pragma unroll
+    for (int it = 0; it < 4; ++it)
+#pragma unroll
+        for (int jt = 0; jt < 2; ++jt) acc[it][jt] = pd4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < DB / 4; ++s) {
+        double a[4];
+#pragma unroll
+        for (int it = 0; it < 4; ++it) a[it] = As[(16 * it + lr) * PLD + 4 * s + lk];  // A21[i][k]
+#pragma unroll
+        for (int it = 0; it < 4; ++it)
+#pragma unroll
+            for (int jt = 0; jt < 2; ++jt)
+                acc[it][jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[s][jt], a[it], acc[it][jt], 0, 0, 0);
+    }
+    // accumulator q of lane l: output row 16 it + (l & 15), column 32 w + 16 jt + (l >> 4) + 4 q
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+        const int i = 16 * it + lr;
+        if (i >= rows) continue;
+#pragma unroll
+        for (int jt = 0; jt < 2; ++jt)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                A[(size_t)(r0 + i) + (size_t)(32 * w + 16 * jt + lk + 4 * q) * lda] = acc[it][jt][q];
+    }
+}
+
+int set_panel_lds() {
+    static const int rc = [] {
+        if (hipFuncSetAttribute((const void*)panel_trmm_k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                PR * PLD * (int)sizeof(double)) != hipSuccess) {
+            (void)hipGetLastError();
+            return (int)AL_ERR_DEVICE;
+        }
+        return (int)AL_OK;
+    }();
+    if (rc != AL_OK) set_error("dpotrf: the panel kernel's LDS could not be configured");
+    return rc;
+}
+
 // One diagonal block [k, k + jb) of the whole n x n factorization: the block (and its inverse),
 // then the panel below it as A21 := T L11^{-T} with T a copy of A21.  The leading block is
 // factored with the crate's arithmetic (nothing has updated it), the others with FMA.
@@ -203,6 +277,13 @@ int potrf_block(hipStream_t s, size_t n, double* A, size_t lda, int* info, size_
                            (size_t)DB, info);
     AL_HIP(hipGetLastError());
     double* A21 = Akk + jb;
+    if (jb == (size_t)DB) {
+        AL_CHECK(set_panel_lds());
+        hipLaunchKernelGGL(panel_trmm_k, dim3((unsigned)ceil_div(mr, (size_t)PR)), dim3(256),
+                           PR * PLD * sizeof(double), s, (int)mr, A21, lda, W, (size_t)DB, info);
+        AL_HIP(hipGetLastError());
+        return AL_OK;
+    }
     hipLaunchKernelGGL(copy_cols_k, dim3((unsigned)ceil_div(mr * jb, (size_t)256)), dim3(256), 0, s, (int)mr, (int)jb,
                        A21, lda, T, mr, info);
     AL_HIP(hipGetLastError());

@@ -100,7 +100,7 @@ struct Workspace {
     size_t bbuf_bytes = 0;
     void* zbuf = nullptr;          // complex twins: real-form staging (R(L), R(A), E, W)
     size_t zbuf_bytes = 0;
-    void* cbuf = nullptr;          // FP64 Cholesky: a diagonal block's inverse and a panel copy
+    void* cbuf = nullptr;          // FP64 Cholesky: a diagonal block's inverse (128 x 128)
     size_t cbuf_bytes = 0;
     // a second stream (default priority) for the panel factorization that runs ahead of the
     // trailing update in the FP32 LU, and the two events that order the streams

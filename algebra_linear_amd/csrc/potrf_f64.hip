@@ -168,15 +168,6 @@ __global__ __launch_bounds__(DT) void potrf_diag_k(double* __restrict__ A, size_
 }
 #pragma clang fp contract(on)
 
-__global__ void copy_cols_k(int m, int n, const double* __restrict__ S, size_t lds, double* __restrict__ D,
-                            size_t ldd, const int* __restrict__ info) {
-    if (*info != 0) return;
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (size_t)m * n) return;
-    const int i = (int)(idx % m), c = (int)(idx / m);
-    D[(size_t)i + (size_t)c * ldd] = S[(size_t)i + (size_t)c * lds];
-}
-
 // The panel below a diagonal block in place: A21 (m x 128) := A21 W^T with W = L11^{-1} (128 x 128,
 // column-major, leading dimension ldw).  A workgroup owns 64 whole rows: it copies them to LDS,
 // and wave w forms output columns [32w, 32w + 32) with v_mfma_f64_16x16x4_f64, W's entries from
@@ -251,13 +242,17 @@ int set_panel_lds() {
     return rc;
 }
 
-// One diagonal block [k, k + jb) of the whole n x n factorization: the block (and its inverse),
-// then the panel below it as A21 := T L11^{-T} with T a copy of A21.  The leading block is
-// factored with the crate's arithmetic (nothing has updated it), the others with FMA.
-int potrf_block(hipStream_t s, size_t n, double* A, size_t lda, int* info, size_t k, size_t jb, double* W,
-                double* T) {
+// One diagonal block [k, k + jb) of the whole n x n factorization: the block (and its inverse W),
+// then the panel below it, A21 := A21 W^T.  A block with rows below it is always DB wide (only
+// the last block is narrower).  The leading block is factored with the crate's arithmetic
+// (nothing has updated it), the others with FMA.
+int potrf_block(hipStream_t s, size_t n, double* A, size_t lda, int* info, size_t k, size_t jb, double* W) {
     double* Akk = A + k + k * lda;
     const size_t mr = n - k - jb;
+    if (mr > 0 && jb != (size_t)DB) {
+        set_error("dpotrf: internal error (a narrow block with a panel)");
+        return AL_ERR_DEVICE;
+    }
     const bool exact = k == 0;
     if (mr == 0) {
         if (exact)
@@ -276,31 +271,23 @@ int potrf_block(hipStream_t s, size_t n, double* A, size_t lda, int* info, size_
         hipLaunchKernelGGL((potrf_diag_k<false, true>), dim3(1), dim3(DT), 0, s, Akk, lda, (int)jb, (int)k, W,
                            (size_t)DB, info);
     AL_HIP(hipGetLastError());
-    double* A21 = Akk + jb;
-    if (jb == (size_t)DB) {
-        AL_CHECK(set_panel_lds());
-        hipLaunchKernelGGL(panel_trmm_k, dim3((unsigned)ceil_div(mr, (size_t)PR)), dim3(256),
-                           PR * PLD * sizeof(double), s, (int)mr, A21, lda, W, (size_t)DB, info);
-        AL_HIP(hipGetLastError());
-        return AL_OK;
-    }
-    hipLaunchKernelGGL(copy_cols_k, dim3((unsigned)ceil_div(mr * jb, (size_t)256)), dim3(256), 0, s, (int)mr, (int)jb,
-                       A21, lda, T, mr, info);
+    AL_CHECK(set_panel_lds());
+    hipLaunchKernelGGL(panel_trmm_k, dim3((unsigned)ceil_div(mr, (size_t)PR)), dim3(256), PR * PLD * sizeof(double), s,
+                       (int)mr, Akk + jb, lda, W, (size_t)DB, info);
     AL_HIP(hipGetLastError());
-    return gemm_f64(s, false, true, mr, jb, jb, 1.0, T, mr, W, DB, 0.0, A21, lda);
+    return AL_OK;
 }
 
 // Columns [k, k + ob) (ob <= 2 DB) once every earlier panel has been applied to them: one or two
 // diagonal blocks with their panels, the second after the first's panel has been applied to it.
-int potrf_outer(hipStream_t s, size_t n, double* A, size_t lda, int* info, size_t k, size_t ob, double* W,
-                double* T) {
+int potrf_outer(hipStream_t s, size_t n, double* A, size_t lda, int* info, size_t k, size_t ob, double* W) {
     const size_t jb = std::min(ob, (size_t)DB);
-    AL_CHECK(potrf_block(s, n, A, lda, info, k, jb, W, T));
+    AL_CHECK(potrf_block(s, n, A, lda, info, k, jb, W));
     if (ob == jb) return AL_OK;
     const size_t k2 = k + jb;
     const double* L = A + k2 + k * lda;
     AL_CHECK(gemm_f64(s, false, true, n - k2, ob - jb, jb, -1.0, L, lda, L, lda, 1.0, A + k2 + k2 * lda, lda, 1));
-    return potrf_block(s, n, A, lda, info, k2, ob - jb, W, T);
+    return potrf_block(s, n, A, lda, info, k2, ob - jb, W);
 }
 
 }  // namespace
@@ -308,10 +295,9 @@ int potrf_outer(hipStream_t s, size_t n, double* A, size_t lda, int* info, size_
 int potrf_f64(hipStream_t s, Workspace* ws, size_t n, double* A, size_t lda, int* info) {
     AL_CHECK(fill_i32(s, info, 1, 0));
     if (n == 0) return AL_OK;
-    if (n <= (size_t)DB) return potrf_block(s, n, A, lda, info, 0, n, nullptr, nullptr);
-    // the inverse of a diagonal block (DB x DB) and the copy of a panel (n x DB)
-    const size_t wbytes = (size_t)DB * DB * sizeof(double);
-    const size_t need = wbytes + n * (size_t)DB * sizeof(double);
+    if (n <= (size_t)DB) return potrf_block(s, n, A, lda, info, 0, n, nullptr);
+    // the inverse of a diagonal block (DB x DB)
+    const size_t need = (size_t)DB * DB * sizeof(double);
     if (ws->cbuf_bytes < need) {
         if (ws->cbuf) AL_HIP(hipFree(ws->cbuf));
         ws->cbuf = nullptr;
@@ -325,12 +311,11 @@ int potrf_f64(hipStream_t s, Workspace* ws, size_t n, double* A, size_t lda, int
         ws->cbuf_bytes = need;
     }
     double* W = (double*)ws->cbuf;
-    double* T = (double*)((char*)ws->cbuf + wbytes);
     AL_CHECK(ensure_side_stream(ws));
     hipStream_t side = ws->side;
     AL_HIP(hipEventRecord(ws->ev_main, s));
     AL_HIP(hipStreamWaitEvent(side, ws->ev_main, 0));
-    AL_CHECK(potrf_outer(side, n, A, lda, info, 0, std::min(n, (size_t)OB), W, T));
+    AL_CHECK(potrf_outer(side, n, A, lda, info, 0, std::min(n, (size_t)OB), W));
     AL_HIP(hipEventRecord(ws->ev_side, side));
     for (size_t k = 0; k < n; k += OB) {
         const size_t ke = k + OB;  // every outer block but the last is full
@@ -344,7 +329,7 @@ int potrf_f64(hipStream_t s, Workspace* ws, size_t n, double* A, size_t lda, int
         AL_HIP(hipEventRecord(ws->ev_main, s));
         // side: the next block ...
         AL_HIP(hipStreamWaitEvent(side, ws->ev_main, 0));
-        AL_CHECK(potrf_outer(side, n, A, lda, info, ke, ke2 - ke, W, T));
+        AL_CHECK(potrf_outer(side, n, A, lda, info, ke, ke2 - ke, W));
         AL_HIP(hipEventRecord(ws->ev_side, side));
         // ... while the main stream applies panel k to every column right of it
         if (ke2 < n) {

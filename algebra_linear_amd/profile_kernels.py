@@ -11,6 +11,7 @@ are generated on the host with numpy.  Workloads (--what, comma-separated):
   dgemm  -- al_dev_dgemm nn, m = n = k = 8192, uniform(-1, 1);
   batch  -- al_dev_batch_solve in FP64 on `--systems` systems of the batched workload's size
             distribution (orders log-uniform in [8, 512], half dgesv, half dposv, 1..8 rhs);
+  batch32 -- the same systems in the FP32 + FP64-refinement mode;
   zgemm  -- al_dev_zgemm nn, m = n = k = 4096, entries uniform in the unit square;
   zgetrf -- al_dev_zgetrf, n = 8192, entries uniform in the unit square;
   zpotrf -- al_dev_zpotrf lower, n = 8192, a Hermitian matrix with entries in the unit square
@@ -135,7 +136,7 @@ def main() -> int:
                 raise RuntimeError(_lib.last_error())
         run(gemm, args.reps, dev, f"dgemm nn {n}")
         dev.close()
-    if "batch" in what:
+    if "batch" in what or "batch32" in what:
         count = args.systems
         ns = np.rint(np.exp(rng.uniform(np.log(8), np.log(512), count))).astype(np.int64)
         kinds = np.where(np.arange(count) % 2 == 0, _lib.AL_BATCH_GESV, _lib.AL_BATCH_POSV)
@@ -159,15 +160,20 @@ def main() -> int:
         dB = dev.upload(B)
         dInfo = dev.alloc(4 * count)
 
-        def batch():  # the FP64 mode factors in place: each call starts from the pristine matrices
-            if dev.hip.hipMemcpy(dA, dA0, A.nbytes, 3) != 0:  # hipMemcpyDeviceToDevice
-                raise RuntimeError("hipMemcpy")
-            if lib.al_dev_batch_solve(count, dE, dA, A.size, dB, dInfo, None, _lib.AL_FACTOR_FP64, None) != 0:
-                raise RuntimeError(_lib.last_error())
-        run(batch, args.reps, dev, f"batched fp64, {count} systems")
-        info = dev.download(dInfo, np.zeros(count, dtype=np.int32))
-        if np.any(info != 0):
-            raise RuntimeError("a batched system failed")
+        dB0 = dev.upload(B)
+        for mode, factor in (("batch", _lib.AL_FACTOR_FP64), ("batch32", _lib.AL_FACTOR_FP32)):
+            if mode not in what:
+                continue
+
+            def batch():  # the FP64 mode factors in place: each call starts from the pristine matrices
+                if dev.hip.hipMemcpy(dA, dA0, A.nbytes, 3) != 0 or dev.hip.hipMemcpy(dB, dB0, B.nbytes, 3) != 0:
+                    raise RuntimeError("hipMemcpy")
+                if lib.al_dev_batch_solve(count, dE, dA, A.size, dB, dInfo, None, factor, None) != 0:
+                    raise RuntimeError(_lib.last_error())
+            run(batch, args.reps, dev, f"batched {'fp64' if mode == 'batch' else 'fp32+ir'}, {count} systems")
+            info = dev.download(dInfo, np.zeros(count, dtype=np.int32))
+            if np.any(info != 0):
+                raise RuntimeError("a batched system failed")
         dev.close()
     for name, (m, n) in (("dgetrf", (8192, 8192)), ("dgetrf_rect", (16384, 2048)), ("dgesv64", (16384, 16384))):
         if name not in what:

@@ -3,18 +3,22 @@
 //
 // The crate factors with an unblocked right-looking loop whose elimination walks rows with a
 // stride of n.  This build blocks it for the GPU:
-//   * panel: a cooperative kernel (`panel_getf2`) spreads the m x 64 panel over up to 64
-//     workgroups, each holding its rows in LDS for the whole panel.  Each column costs one
-//     grid-wide exchange: every workgroup publishes its best candidate (first maximum, the
-//     crate's strict '>' scan) plus that row, one grid barrier, then every workgroup picks the
-//     same winner, swaps and eliminates its own rows.  Within the panel the arithmetic is the
-//     crate's own (divide, then a separate multiply and subtract per element, no FMA), so a
-//     matrix that fits in one panel factors bit-for-bit as the crate factors it;
+//   * panel: a cooperative kernel (`panel_getf2`) spreads the m x 64 panel over up to 128
+//     workgroups of 256 rows, each holding its rows in LDS for the whole panel.  Each column
+//     costs one grid-wide exchange: every workgroup publishes its best candidate (first
+//     maximum, the crate's strict '>' scan) and that row with write-through stores, then one
+//     64-bit word carrying the column's generation, the candidate row and its key's top bits;
+//     every workgroup reads all the words, picks the same winner, swaps and eliminates its own
+//     rows.  Within the panel the arithmetic is the crate's own (divide, then a separate
+//     multiply and subtract per element, no FMA), so a matrix that fits in one panel factors
+//     bit-for-bit as the crate factors it.  The workgroups must be resident together: a
+//     process-wide ledger (`coop_ledger.hpp`) keeps concurrent panels within the CU count;
 //   * two-level blocking: 64-wide panels inside 256-wide outer panels, so the trailing update
 //     of the matrix is a k = 256 FP64 MFMA GEMM;
-//   * row swaps of the columns outside a panel, the U12 triangular solve and the trailing
-//     update are separate kernels on the same stream; each checks the status word first so
-//     that a zero pivot stops the factorization like the crate's early return.
+//   * look-ahead on two streams: the workspace's side stream factors an outer panel and
+//     prepares the next one's columns (interchanges, U12, their update) while the caller's
+//     stream applies the outer panel to every other column; each kernel checks the status word
+//     first so that a zero pivot stops the factorization like the crate's early return.
 // The panel kernel is also instantiated for float (the mixed-precision factor) and for the
 // complex element `zd` (lapack::zgetrf, src/lapack/complex.rs:67-121: |Re| + |Im| pivot keys,
 // complex division and multiply-subtract, 32-column panels).

@@ -246,7 +246,10 @@ def test_hilbert_forward_error_scaled_by_rcond():
     assert O.get04(b, np.ones(n), n, 1, rcond) <= 20
 
 
-@pytest.mark.parametrize("n", [65, 200, 300, 700])
+# sizes around the blocking: 128-wide diagonal blocks inside 384-wide look-ahead blocks
+# (129: one column past a diagonal block; 256, 384: whole blocks; 385, 769: one column into the
+# next look-ahead block; 512: a partial look-ahead block of whole diagonal blocks)
+@pytest.mark.parametrize("n", [65, 129, 200, 256, 300, 384, 385, 512, 700, 769])
 def test_dpotrf_blocked_residual_and_upper_untouched(n):
     a0 = spd_gg(n, [n, 1, 1, 1])
     A = mat(a0, n, n)
@@ -261,11 +264,12 @@ def test_dpotrf_blocked_residual_and_upper_untouched(n):
     assert np.allclose(np.tril(got), np.tril(mat(l, n, n)), rtol=1e-10, atol=1e-12)
 
 
-def test_dpotrf_not_spd_in_a_later_block():
-    n = 200
+@pytest.mark.parametrize("n, bad", [(200, 150), (900, 500), (900, 899)])
+def test_dpotrf_not_spd_in_a_later_block(n, bad):
+    # a failing pivot beyond the first diagonal block, in a later look-ahead block, and the last
     a0 = spd_gg(n)
     A = mat(a0, n, n)
-    A[150, 150] = -1e6  # the failing pivot is beyond the first 128-block
+    A[bad, bad] = -1e6
     with pytest.raises(al.LapackError) as e:
         lapack.dpotrf(cm(A), n)
     assert e.value.kind == al.SINGULAR

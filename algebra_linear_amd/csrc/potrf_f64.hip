@@ -104,7 +104,7 @@ __global__ __launch_bounds__(DT) void potrf_diag_k(double* __restrict__ A, size_
                 return;
             }
             double ljj, r;
-            if (EXACT) {
+            if (EXACT || !(d >= 0x1p-1000 && d <= 0x1p+1000)) {  // uniform; rsqrt_nr wants a normal d
                 ljj = sqrt(d);
                 r = 1.0 / ljj;
             } else {

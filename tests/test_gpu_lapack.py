@@ -264,6 +264,18 @@ def test_dpotrf_blocked_residual_and_upper_untouched(n):
     assert np.allclose(np.tril(got), np.tril(mat(l, n, n)), rtol=1e-10, atol=1e-12)
 
 
+def test_dpotrf_extreme_pivot_scales_in_later_blocks():
+    # a diagonal matrix whose entries span 1e-305 .. 1e305 (past the blocks that use the crate's
+    # own arithmetic): L is the entrywise square root, to an ulp
+    n = 300
+    d = 10.0 ** np.linspace(-305, 305, n)
+    a = cm(np.diag(d))
+    lapack.dpotrf(a, n)
+    got = mat(a, n, n)
+    assert np.allclose(np.diag(got), np.sqrt(d), rtol=4 * EPS, atol=0)
+    assert np.all(np.tril(got, -1) == 0.0)
+
+
 @pytest.mark.parametrize("n, bad", [(200, 150), (900, 500), (900, 899)])
 def test_dpotrf_not_spd_in_a_later_block(n, bad):
     # a failing pivot beyond the first diagonal block, in a later look-ahead block, and the last

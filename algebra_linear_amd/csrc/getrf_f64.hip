@@ -25,6 +25,7 @@
 #include "al_internal.hpp"
 #include "coop_ledger.hpp"
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <mutex>
@@ -51,9 +52,16 @@ struct PanelElem<zd> {
     typedef double key_t;
     static constexpr int NB = 32;
 };
-// exchange slot: key, index, candidate row, row j
+// single-workgroup exchange slot (LDS): key, index, candidate row, row j
 template <typename T>
 constexpr int slot2() { return 2 + 2 * PanelElem<T>::NB; }
+// cooperative exchange slot (global memory), in 64-bit words that each carry a 32-bit part of a
+// value in the low half and the column's generation in the high half: the key's parts, the row
+// index, the candidate row's parts, row j's parts
+template <typename T>
+constexpr int slot_row() { return (int)(sizeof(typename PanelElem<T>::key_t) / 4) + 1; }
+template <typename T>
+constexpr int slot_words() { return slot_row<T>() + 2 * PanelElem<T>::NB * (int)(sizeof(T) / 4); }
 
 // Candidate ordering: larger key wins, ties go to the lower row (first maximum).
 template <typename T>
@@ -131,6 +139,29 @@ __device__ __forceinline__ zd ld_agent(const zd* p) {
 __device__ __forceinline__ unsigned long long ld_flag(const unsigned long long* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// A value as generation-tagged words (one per 32-bit part), and back: a 64-bit relaxed atomic is
+// single-copy atomic, so a word whose tag is the current column's holds this column's part.
+template <typename V>
+__device__ __forceinline__ void put_tagged(unsigned long long* p, V v, unsigned tag) {
+    unsigned u[sizeof(V) / 4];
+    __builtin_memcpy(u, &v, sizeof(V));
+#pragma unroll
+    for (int h = 0; h < (int)(sizeof(V) / 4); ++h)
+        __hip_atomic_store(&p[h], ((unsigned long long)tag << 32) | u[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename V>
+__device__ __forceinline__ bool get_tagged(const unsigned long long* p, V& v, unsigned tag) {
+    unsigned u[sizeof(V) / 4];
+    bool ok = true;
+#pragma unroll
+    for (int h = 0; h < (int)(sizeof(V) / 4); ++h) {
+        const unsigned long long x = __hip_atomic_load(&p[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok &= (unsigned)(x >> 32) == tag;
+        u[h] = (unsigned)x;
+    }
+    __builtin_memcpy(&v, u, sizeof(V));
+    return ok;
+}
 
 // The 32-bit head of a pivot key's integer image: orders keys as key_code does except that keys
 // differing only below their top 32 bits compare equal (such ties are settled from the full
@@ -206,10 +237,12 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
                                                   int chunk, int* __restrict__ ipiv, int* __restrict__ info,
                                                   unsigned long long* __restrict__ flags,
                                                   unsigned* __restrict__ abort_flag,
-                                                  T* __restrict__ slots, unsigned gen0, long long timeout) {
+                                                  unsigned long long* __restrict__ slots, unsigned gen0,
+                                                  long long timeout) {
     typedef typename PanelElem<T>::key_t K;
     constexpr int NB = PanelElem<T>::NB;
     constexpr int SLOT2 = slot2<T>();
+    constexpr int SW = slot_words<T>(), SR = slot_row<T>(), H = (int)(sizeof(T) / 4);
     extern __shared__ __align__(16) unsigned char smem[];
     T* S = (T*)smem;  // chunk x NB, column-major with leading dimension chunk | 1
     __shared__ T lslot[2][SLOT2];  // single-workgroup exchange
@@ -236,32 +269,30 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
         if (has) S[c * ls + t] = P[(size_t)g + (size_t)c * lda];
     __syncthreads();
 
-    auto slot_ptr = [&](int par, int q) -> T* {
-        return COOP ? slots + ((size_t)par * nwg + q) * SLOT2 : &lslot[par][0];
-    };
+    auto slot_ptr = [&](int par, int q) -> unsigned long long* { return slots + ((size_t)par * nwg + q) * SW; };
     // wave 0 writes this workgroup's slot for column jj and publishes its word
     auto publish = [&](int jj, K key, int idx) {
         if (wave == 0) {
-            T* my = slot_ptr(jj & 1, w);
             const bool cand = key > (K)-2;
             const int lr = idx - rb;
             const bool ownj = jj >= rb && jj < rb + nr;
             if (COOP) {
+                // tagged words need no acknowledgement before the publication word: a reader that
+                // sees the word before the slot's words have landed sees old tags and reads again
+                unsigned long long* my = slot_ptr(jj & 1, w);
+                const unsigned tag = gen0 + (unsigned)jj + 1u;
                 if (cand)
-                    for (int c = lane; c < ncols; c += 64) st_agent(&my[2 + c], S[c * ls + lr]);
+                    for (int c = lane; c < ncols; c += 64) put_tagged(&my[SR + c * H], S[c * ls + lr], tag);
                 if (ownj)
-                    for (int c = lane; c < ncols; c += 64) st_agent(&my[2 + NB + c], S[c * ls + (jj - rb)]);
+                    for (int c = lane; c < ncols; c += 64) put_tagged(&my[SR + (NB + c) * H], S[c * ls + (jj - rb)], tag);
                 if (lane == 0) {
-                    st_agent(&my[0], T(key));
-                    st_agent(&my[1], T((K)idx));
+                    put_tagged(&my[0], key, tag);
+                    put_tagged(&my[SR - 1], (unsigned)idx, tag);
+                    __hip_atomic_store(&flags[w * 8 + (jj & 1)], flag_word(tag, idx, key_head(key)), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
                 }
-                // the slot's stores are acknowledged (write-through) before the word goes out
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-                __builtin_amdgcn_s_waitcnt(0);
-                if (lane == 0)
-                    __hip_atomic_store(&flags[w * 8 + (jj & 1)], flag_word(gen0 + (unsigned)jj + 1u, idx, key_head(key)),
-                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             } else {
+                T* my = &lslot[jj & 1][0];
                 if (cand)
                     for (int c = lane; c < ncols; c += 64) my[2 + c] = S[c * ls + lr];
                 if (ownj)
@@ -278,15 +309,11 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
         const int par = jj & 1;
         if (COOP) {
             if (wave == 0) {
-                int ok = 1;
-                const unsigned long long want = (unsigned long long)((gen0 + (unsigned)jj + 1u) & 0xFFFFu);
+                const unsigned tag = gen0 + (unsigned)jj + 1u;
+                const unsigned long long want = (unsigned long long)(tag & 0xFFFFu);
                 const long long t0 = wall_clock64();
-                unsigned long long f0 = 0, f1 = 0;
-                for (;;) {
-                    f0 = lane < nwg ? ld_flag(&flags[lane * 8 + par]) : 0ull;
-                    f1 = lane + 64 < nwg ? ld_flag(&flags[(lane + 64) * 8 + par]) : 0ull;
-                    const bool mine = (lane >= nwg || (f0 >> 48) == want) && (lane + 64 >= nwg || (f1 >> 48) == want);
-                    if (__all(mine)) break;
+                // lane 0 checks the abort flag and the time limit; the wave leaves together
+                auto give_up = [&]() -> bool {
                     int stop = 0;
                     if (lane == 0) {
                         if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
@@ -296,15 +323,23 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
                             stop = 1;
                         }
                     }
-                    if (__shfl(stop, 0, 64)) {
+                    return __shfl(stop, 0, 64) != 0;
+                };
+                int ok = 1;
+                unsigned long long f0 = 0, f1 = 0;
+                for (;;) {
+                    f0 = lane < nwg ? ld_flag(&flags[lane * 8 + par]) : 0ull;
+                    f1 = lane + 64 < nwg ? ld_flag(&flags[(lane + 64) * 8 + par]) : 0ull;
+                    const bool mine = (lane >= nwg || (f0 >> 48) == want) && (lane + 64 >= nwg || (f1 >> 48) == want);
+                    if (__all(mine)) break;
+                    if (give_up()) {
                         ok = 0;
                         break;
                     }
                     __builtin_amdgcn_s_sleep(1);
                 }
+                int bq = 0;  // the winning workgroup
                 if (ok) {
-                    // nothing read below may be issued before the words were seen
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                     const unsigned h0 = lane < nwg ? (unsigned)f0 : 0u;
                     const unsigned h1 = lane + 64 < nwg ? (unsigned)f1 : 0u;
                     unsigned hmax = h0 > h1 ? h0 : h1;
@@ -315,43 +350,74 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
                     }
                     const bool t0m = lane < nwg && h0 == hmax, t1m = lane + 64 < nwg && h1 == hmax;
                     const unsigned long long b0 = __ballot(t0m), b1 = __ballot(t1m);
-                    int bq;  // the winning workgroup
                     if (__popcll(b0) + __popcll(b1) == 1) {
                         bq = b0 ? __ffsll((long long)b0) - 1 : 64 + __ffsll((long long)b1) - 1;
                     } else {
                         // several workgroups share the largest key head: compare their full keys
-                        K bk = (K)-3;
-                        int bi = 0x7fffffff;
-                        for (int q = lane; q < nwg; q += 64) {
-                            if (!(q < 64 ? t0m : t1m)) continue;
-                            const T* sq = slot_ptr(par, q);
-                            const K kq = slot_val(ld_agent(&sq[0]));
-                            const int iq = (int)slot_val(ld_agent(&sq[1]));
-                            if (better(kq, iq, bk, bi)) {
-                                bk = kq;
-                                bi = iq;
+                        for (;;) {
+                            K bk = (K)-3;
+                            int bi = 0x7fffffff;
+                            bool good = true;
+                            for (int q = lane; q < nwg; q += 64) {
+                                if (!(q < 64 ? t0m : t1m)) continue;
+                                const unsigned long long* sq = slot_ptr(par, q);
+                                K kq;
+                                unsigned iq;
+                                good &= get_tagged(&sq[0], kq, tag);
+                                good &= get_tagged(&sq[SR - 1], iq, tag);
+                                if (better(kq, (int)iq, bk, bi)) {
+                                    bk = kq;
+                                    bi = (int)iq;
+                                }
                             }
-                        }
 #pragma unroll
-                        for (int off = 32; off >= 1; off >>= 1) {
-                            const K ok2 = __shfl_xor(bk, off, 64);
-                            const int oi = __shfl_xor(bi, off, 64);
-                            if (better(ok2, oi, bk, bi)) {
-                                bk = ok2;
-                                bi = oi;
+                            for (int off = 32; off >= 1; off >>= 1) {
+                                const K ok2 = __shfl_xor(bk, off, 64);
+                                const int oi = __shfl_xor(bi, off, 64);
+                                if (better(ok2, oi, bk, bi)) {
+                                    bk = ok2;
+                                    bi = oi;
+                                }
                             }
+                            if (__all(good)) {
+                                bq = bi / chunk;
+                                break;
+                            }
+                            if (give_up()) {
+                                ok = 0;
+                                break;
+                            }
+                            __builtin_amdgcn_s_sleep(1);
                         }
-                        bq = bi / chunk;
                     }
-                    const T* win = slot_ptr(par, bq);
-                    const T* own = slot_ptr(par, jj / chunk);
-                    for (int c = lane; c < ncols; c += 64) {
-                        prow_c[c] = ld_agent(&win[2 + c]);
-                        rj_c[c] = ld_agent(&own[2 + NB + c]);
-                    }
-                    if (lane == 0) {
-                        win_lds = (int)slot_val(ld_agent(&win[1]));
-                        wkey_lds = slot_val(ld_agent(&win[0]));
+                }
+                if (ok) {
+                    // the winner's row, its key and index, and row jj
+                    const unsigned long long* win = slot_ptr(par, bq);
+                    const unsigned long long* own = slot_ptr(par, jj / chunk);
+                    for (;;) {
+                        bool good = true;
+                        for (int c = lane; c < ncols; c += 64) {
+                            T pv, rv;
+                            good &= get_tagged(&win[SR + c * H], pv, tag);
+                            good &= get_tagged(&own[SR + (NB + c) * H], rv, tag);
+                            prow_c[c] = pv;
+                            rj_c[c] = rv;
+                        }
+                        if (lane == 0) {
+                            K wk0;
+                            unsigned wi0;
+                            good &= get_tagged(&win[0], wk0, tag);
+                            good &= get_tagged(&win[SR - 1], wi0, tag);
+                            win_lds = (int)wi0;
+                            wkey_lds = wk0;
+                        }
+                        if (__all(good)) break;
+                        if (give_up()) {
+                            ok = 0;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
                     }
                 }
                 if (lane == 0) ok_lds = ok;
@@ -478,6 +544,7 @@ int launch_panel(hipStream_t s, Workspace* ws, T* Ppanel, size_t lda, size_t row
     // would wrap, so a word left from an earlier panel never matches a current generation
     if (ws->gen + (unsigned)ncols >= 0xFFFFu) {
         AL_CHECK(fill_u32(s, ws->flags, (size_t)MAX_WG * 16, 0u));
+        AL_CHECK(fill_u32(s, (unsigned*)ws->slots, ws->slots_bytes / sizeof(unsigned), 0u));  // tags 0: never current
         ws->gen = 0;
     }
     const long long timeout = wait_limit_ticks();
@@ -487,11 +554,11 @@ int launch_panel(hipStream_t s, Workspace* ws, T* Ppanel, size_t lda, size_t row
     if (nwg > 1) {
         hipLaunchKernelGGL((panel_getf2<T, true>), dim3((unsigned)nwg), dim3(PT), (chunk | 1) * NB * sizeof(T), s, Ppanel,
                            lda, (int)row0, (int)m, (int)ncols, (int)chunk, ipiv, info, (unsigned long long*)ws->flags, ws->abort_flag,
-                           (T*)ws->slots, gen0, timeout);
+                           (unsigned long long*)ws->slots, gen0, timeout);
     } else {
         hipLaunchKernelGGL((panel_getf2<T, false>), dim3(1), dim3(PT), (chunk | 1) * NB * sizeof(T), s, Ppanel, lda,
                            (int)row0, (int)m, (int)ncols, (int)chunk, ipiv, info, (unsigned long long*)ws->flags, ws->abort_flag,
-                           (T*)ws->slots, gen0, timeout);
+                           (unsigned long long*)ws->slots, gen0, timeout);
     }
     AL_HIP(hipGetLastError());
     if (ticket >= 0) {
@@ -844,9 +911,8 @@ int getrs_f64(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const double*
 }
 
 size_t panel_slots_bytes() {
-    const size_t real = (size_t)2 * MAX_WG * slot2<double>() * sizeof(double);
-    const size_t cplx = (size_t)2 * MAX_WG * slot2<zd>() * sizeof(zd);
-    return real > cplx ? real : cplx;
+    const int words = std::max(slot_words<double>(), std::max(slot_words<float>(), slot_words<zd>()));
+    return (size_t)2 * MAX_WG * (size_t)words * sizeof(unsigned long long);
 }
 size_t panel_flags_count() { return (size_t)MAX_WG * 16; }
 

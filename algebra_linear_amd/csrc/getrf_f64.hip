@@ -75,7 +75,7 @@ __device__ __forceinline__ bool better(T k1, int i1, T k2, int i2) {
 template <typename T>
 __device__ __forceinline__ T pivot_key(T v, int g, int j) {
     if (v != v) return (g == j) ? (T)INFINITY : (T)-1;
-    return v < (T)0 ? -v : v;
+    return v < (T)0 ? -v : (v == (T)0 ? (T)0 : v);  // -0.0 -> +0.0: keys compare as bit patterns too
 }
 // complex entries: |Re| + |Im| (src/lapack/complex.rs:78-90), with the same NaN rule
 __device__ __forceinline__ double pivot_key(zd v, int g, int j) {
@@ -178,21 +178,60 @@ __device__ __forceinline__ unsigned long long flag_word(unsigned gen, int idx, u
     return ((unsigned long long)(gen & 0xFFFFu) << 48) | ((unsigned long long)(idx & 0xFFFF) << 32) | head;
 }
 
-// Workgroup-wide first maximum in every thread: the largest key, ties to the lowest row.  A
-// shuffle tree per wave, then the four waves' winners in LDS
-// behind one barrier; `par` alternates between calls (callers pass a workgroup barrier between
-// two calls, so a call's writes never meet the previous call's reads).
+// Maximum over the wave of unsigned values, in every lane: prefix maxima inside each row of 16
+// lanes (DPP row_shr 1, 2, 4, 8), then row 0 into row 1 and row 2 into row 3 (row_bcast:15),
+// then rows 0-1 into rows 2-3 (row_bcast:31), so that lane 63 holds the maximum.  A lane without
+// a source reads 0, the identity of the unsigned maximum.
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));
+    return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
+}
+__device__ __forceinline__ double lane_val(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ float lane_val(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// The wave's first maximum in every lane: the largest key (compared as key_code images: the
+// high words first, then among the lanes that share the largest high word the low words), ties
+// to the lowest row.  Usually the first DPP maximum already names a single lane.
 template <typename K>
-__device__ __forceinline__ void wg_argmax(K& key, int& idx, K (*wk)[PT / 64], int (*wi)[PT / 64], int& par) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        const K ok = __shfl_xor(key, off, 64);
-        const int oi = __shfl_xor(idx, off, 64);
-        if (better(ok, oi, key, idx)) {
-            key = ok;
-            idx = oi;
+__device__ __forceinline__ void wave_argmax(K& key, int& idx) {
+    const unsigned long long code = key_code(key);
+    const unsigned hi = (unsigned)(code >> 32), lo = (unsigned)code;
+    bool in = hi == wave_max_u32(hi);
+    unsigned long long m = __ballot(in);
+    if (__popcll(m) > 1) {
+        const unsigned lm = wave_max_u32(in ? lo : 0u);
+        in = in && lo == lm;
+        m = __ballot(in);
+        if (__popcll(m) > 1) {  // equal keys: the lowest row (rows are >= 0, so ~row orders them)
+            const unsigned im = wave_max_u32(in ? ~(unsigned)idx : 0u);
+            in = in && ~(unsigned)idx == im;
+            m = __ballot(in);
         }
     }
+    const int wl = __ffsll((long long)m) - 1;
+    key = lane_val(key, wl);
+    idx = __builtin_amdgcn_readlane(idx, wl);
+}
+
+// Workgroup-wide first maximum in every thread: the largest key, ties to the lowest row.  The
+// wave's winner by DPP, then the four waves' winners in LDS behind one barrier; `par`
+// alternates between calls (callers pass a workgroup barrier between two calls, so a call's
+// writes never meet the previous call's reads).
+template <typename K>
+__device__ __forceinline__ void wg_argmax(K& key, int& idx, K (*wk)[PT / 64], int (*wi)[PT / 64], int& par) {
+    wave_argmax(key, idx);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0) {
         wk[par][wave] = key;
@@ -251,7 +290,6 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
     __shared__ K wk[2][PT / 64];
     __shared__ int wi[2][PT / 64];
     __shared__ int win_lds;
-    __shared__ K wkey_lds;
     __shared__ int ok_lds;
 
     const int nwg = COOP ? (int)gridDim.x : 1;
@@ -314,6 +352,9 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
                 const long long t0 = wall_clock64();
                 // lane 0 checks the abort flag and the time limit; the wave leaves together
                 auto give_up = [&]() -> bool {
+                    // a short wait only polls: the abort flag is read once the wait passes 20 us
+                    const long long el = wall_clock64() - t0;
+                    if (el < 2000 && el <= timeout) return false;
                     int stop = 0;
                     if (lane == 0) {
                         if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
@@ -338,20 +379,17 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
                     }
                     __builtin_amdgcn_s_sleep(1);
                 }
-                int bq = 0;  // the winning workgroup
+                int bq = 0, bidx = 0;  // the winning workgroup and its candidate row
                 if (ok) {
                     const unsigned h0 = lane < nwg ? (unsigned)f0 : 0u;
                     const unsigned h1 = lane + 64 < nwg ? (unsigned)f1 : 0u;
-                    unsigned hmax = h0 > h1 ? h0 : h1;
-#pragma unroll
-                    for (int off = 32; off >= 1; off >>= 1) {
-                        const unsigned o = (unsigned)__shfl_xor((int)hmax, off, 64);
-                        hmax = o > hmax ? o : hmax;
-                    }
+                    const unsigned hmax = wave_max_u32(h0 > h1 ? h0 : h1);
                     const bool t0m = lane < nwg && h0 == hmax, t1m = lane + 64 < nwg && h1 == hmax;
                     const unsigned long long b0 = __ballot(t0m), b1 = __ballot(t1m);
                     if (__popcll(b0) + __popcll(b1) == 1) {
                         bq = b0 ? __ffsll((long long)b0) - 1 : 64 + __ffsll((long long)b1) - 1;
+                        const unsigned long long fw = __shfl(bq < 64 ? f0 : f1, bq & 63, 64);
+                        bidx = (int)((fw >> 32) & 0xFFFFu);
                     } else {
                         // several workgroups share the largest key head: compare their full keys
                         for (;;) {
@@ -381,6 +419,7 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
                             }
                             if (__all(good)) {
                                 bq = bi / chunk;
+                                bidx = bi;
                                 break;
                             }
                             if (give_up()) {
@@ -392,9 +431,11 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
                     }
                 }
                 if (ok) {
-                    // the winner's row, its key and index, and row jj
+                    // the winner's row and row jj (one round trip); the winner's index came with its
+                    // word, and its key is the pivot key of the row's entry in column jj
                     const unsigned long long* win = slot_ptr(par, bq);
                     const unsigned long long* own = slot_ptr(par, jj / chunk);
+                    if (lane == 0) win_lds = bidx;
                     for (;;) {
                         bool good = true;
                         for (int c = lane; c < ncols; c += 64) {
@@ -403,14 +444,6 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
                             good &= get_tagged(&own[SR + (NB + c) * H], rv, tag);
                             prow_c[c] = pv;
                             rj_c[c] = rv;
-                        }
-                        if (lane == 0) {
-                            K wk0;
-                            unsigned wi0;
-                            good &= get_tagged(&win[0], wk0, tag);
-                            good &= get_tagged(&win[SR - 1], wi0, tag);
-                            win_lds = (int)wi0;
-                            wkey_lds = wk0;
                         }
                         if (__all(good)) break;
                         if (give_up()) {
@@ -431,10 +464,7 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
             const T* sl = &lslot[par][0];
             prow = sl + 2;
             rj = sl + 2 + NB;
-            if (t == 0) {
-                win_lds = (int)slot_val(sl[1]);
-                wkey_lds = slot_val(sl[0]);
-            }
+            if (t == 0) win_lds = (int)slot_val(sl[1]);
             __syncthreads();
             return true;
         }
@@ -455,7 +485,7 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
             break;
         }
         const int p = win_lds;
-        if (!(wkey_lds > (K)0)) {
+        if (!(pivot_key(prow[j], p, j) > (K)0)) {
             // |pivot| <= 0: singular (src/lapack/mod.rs:136-138); a NaN diagonal carries +inf
             if (w == 0 && t == 0) *info = row0 + j + 1;
             break;

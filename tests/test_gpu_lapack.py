@@ -158,6 +158,27 @@ def test_dgetrf_rect_is_bit_exact_within_one_panel(nrows, ncols):
     assert np.array_equal(a, lu)
 
 
+@pytest.mark.parametrize("zero_rows", [200, 600])
+def test_dgetrf_rect_negative_zeros_in_a_workgroups_rows(zero_rows):
+    # 600 x 64 runs as three cooperating workgroups of 200 rows; column 0 of the first
+    # workgroup's rows (or of every row) holds -0.0, which ties with +0.0 and never beats a
+    # nonzero entry: pivots and factors as the crate computes them (or its zero-pivot report)
+    nrows, ncols = 600, 64
+    a0 = uniform(nrows * ncols, [3, 1, 4, 1])
+    a0[:zero_rows] = -0.0
+    a = a0.copy()
+    ipiv = np.zeros(ncols, dtype=np.uintp)
+    st, lu, piv_o = O.dgetrf_rect(a0, nrows, ncols)
+    if st != 0:
+        with pytest.raises(al.LapackError) as e:
+            lapack.dgetrf_rect(a, nrows, ncols, ipiv)
+        assert e.value.kind == al.SINGULAR
+        return
+    lapack.dgetrf_rect(a, nrows, ncols, ipiv)
+    assert list(ipiv) == list(piv_o)
+    assert np.array_equal(a, lu)
+
+
 @pytest.mark.parametrize("n", [1, 2, 5, 9, 33, 64, 65, 100, 128])
 def test_dpotrf_is_bit_exact_within_one_block(n):
     a0 = spd(n) if n < 10 else spd_gg(n)

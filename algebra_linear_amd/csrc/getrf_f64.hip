@@ -200,6 +200,7 @@ __device__ __forceinline__ double lane_val(double v, int l) {
 __device__ __forceinline__ float lane_val(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
+__device__ __forceinline__ zd lane_val(zd v, int l) { return zd(lane_val(v.re, l), lane_val(v.im, l)); }
 
 // The wave's first maximum in every lane: the largest key (compared as key_code images: the
 // high words first, then among the lanes that share the largest high word the low words), ties
@@ -286,7 +287,6 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
     T* S = (T*)smem;  // chunk x NB, column-major with leading dimension chunk | 1
     __shared__ T lslot[2][SLOT2];  // single-workgroup exchange
     __shared__ T prow_c[NB];
-    __shared__ T rj_c[NB];
     __shared__ K wk[2][PT / 64];
     __shared__ int wi[2][PT / 64];
     __shared__ int win_lds;
@@ -308,37 +308,32 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
     __syncthreads();
 
     auto slot_ptr = [&](int par, int q) -> unsigned long long* { return slots + ((size_t)par * nwg + q) * SW; };
-    // wave 0 writes this workgroup's slot for column jj and publishes its word
-    auto publish = [&](int jj, K key, int idx) {
-        if (wave == 0) {
-            const bool cand = key > (K)-2;
-            const int lr = idx - rb;
-            const bool ownj = jj >= rb && jj < rb + nr;
-            if (COOP) {
-                // tagged words need no acknowledgement before the publication word: a reader that
-                // sees the word before the slot's words have landed sees old tags and reads again
-                unsigned long long* my = slot_ptr(jj & 1, w);
-                const unsigned tag = gen0 + (unsigned)jj + 1u;
-                if (cand)
-                    for (int c = lane; c < ncols; c += 64) put_tagged(&my[SR + c * H], S[c * ls + lr], tag);
-                if (ownj)
-                    for (int c = lane; c < ncols; c += 64) put_tagged(&my[SR + (NB + c) * H], S[c * ls + (jj - rb)], tag);
-                if (lane == 0) {
-                    put_tagged(&my[0], key, tag);
-                    put_tagged(&my[SR - 1], (unsigned)idx, tag);
-                    __hip_atomic_store(&flags[w * 8 + (jj & 1)], flag_word(tag, idx, key_head(key)), __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                }
-            } else {
-                T* my = &lslot[jj & 1][0];
-                if (cand)
-                    for (int c = lane; c < ncols; c += 64) my[2 + c] = S[c * ls + lr];
-                if (ownj)
-                    for (int c = lane; c < ncols; c += 64) my[2 + NB + c] = S[c * ls + (jj - rb)];
-                if (lane == 0) {
-                    my[0] = T(key);
-                    my[1] = T((K)idx);
-                }
+    // wave 0 writes this workgroup's slot for column jj and publishes its word; lane c holds
+    // column c of the candidate row (vc) and of row jj (vn, when this workgroup owns row jj)
+    auto publish = [&](int jj, K key, int idx, T vc, T vn) {
+        const bool cand = key > (K)-2;
+        const bool ownj = jj >= rb && jj < rb + nr;
+        const bool col = lane < ncols;  // ncols <= NB <= 64: one column per lane
+        if (COOP) {
+            // tagged words need no acknowledgement before the publication word: a reader that
+            // sees the word before the slot's words have landed sees old tags and reads again
+            unsigned long long* my = slot_ptr(jj & 1, w);
+            const unsigned tag = gen0 + (unsigned)jj + 1u;
+            if (cand && col) put_tagged(&my[SR + lane * H], vc, tag);
+            if (ownj && col) put_tagged(&my[SR + (NB + lane) * H], vn, tag);
+            if (lane == 0) {
+                put_tagged(&my[0], key, tag);
+                put_tagged(&my[SR - 1], (unsigned)idx, tag);
+                __hip_atomic_store(&flags[w * 8 + (jj & 1)], flag_word(tag, idx, key_head(key)), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+        } else {
+            T* my = &lslot[jj & 1][0];
+            if (cand && col) my[2 + lane] = vc;
+            if (ownj && col) my[2 + NB + lane] = vn;
+            if (lane == 0) {
+                my[0] = T(key);
+                my[1] = T((K)idx);
             }
         }
     };
@@ -435,15 +430,12 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
                     // word, and its key is the pivot key of the row's entry in column jj
                     const unsigned long long* win = slot_ptr(par, bq);
                     const unsigned long long* own = slot_ptr(par, jj / chunk);
-                    if (lane == 0) win_lds = bidx;
+                    T pv = T(0), rv = T(0);
                     for (;;) {
                         bool good = true;
-                        for (int c = lane; c < ncols; c += 64) {
-                            T pv, rv;
-                            good &= get_tagged(&win[SR + c * H], pv, tag);
-                            good &= get_tagged(&own[SR + (NB + c) * H], rv, tag);
-                            prow_c[c] = pv;
-                            rj_c[c] = rv;
+                        if (lane < ncols) {
+                            good &= get_tagged(&win[SR + lane * H], pv, tag);
+                            good &= get_tagged(&own[SR + (NB + lane) * H], rv, tag);
                         }
                         if (__all(good)) break;
                         if (give_up()) {
@@ -452,12 +444,22 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
                         }
                         __builtin_amdgcn_s_sleep(1);
                     }
+                    if (ok) {
+                        // swap rows jj and bidx where this workgroup holds them (not before a zero
+                        // pivot: the crate stops before its swap), and keep the pivot row for all
+                        if (pivot_key(lane_val(pv, jj), bidx, jj) > (K)0 && bidx != jj && lane < ncols) {
+                            if (bidx >= rb && bidx < rb + nr) S[lane * ls + (bidx - rb)] = rv;
+                            if (jj >= rb && jj < rb + nr) S[lane * ls + (jj - rb)] = pv;
+                        }
+                        if (lane < ncols) prow_c[lane] = pv;
+                        if (lane == 0) win_lds = bidx;
+                    }
                 }
                 if (lane == 0) ok_lds = ok;
             }
             __syncthreads();
             prow = prow_c;
-            rj = rj_c;
+            rj = prow_c;  // not read: the exchange swapped the rows already
             return ok_lds != 0;
         } else {
             __syncthreads();
@@ -474,7 +476,14 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
         K key = has ? pivot_key(S[t], g, 0) : (K)-2;
         int idx = has ? g : 0x7fffffff;
         wg_argmax(key, idx, wk, wi, argpar);
-        publish(0, key, idx);
+        if (wave == 0) {
+            T vc = T(0), vn = T(0);
+            if (lane < ncols) {
+                if (key > (K)-2) vc = S[lane * ls + (idx - rb)];
+                if (rb == 0 && nr > 0) vn = S[lane * ls];
+            }
+            publish(0, key, idx, vc, vn);
+        }
     }
 
     for (int j = 0; j < ncols; ++j) {
@@ -490,15 +499,15 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
             if (w == 0 && t == 0) *info = row0 + j + 1;
             break;
         }
-        // swap rows j and p across the panel, record the pivot
-        if (p != j) {
+        // swap rows j and p across the panel (a cooperative exchange did it already), record the pivot
+        if (!COOP && p != j) {
             if (p >= rb && p < rb + nr)
                 for (int c = t; c < ncols; c += PT) S[c * ls + (p - rb)] = rj[c];
             if (j >= rb && j < rb + nr)
                 for (int c = t; c < ncols; c += PT) S[c * ls + (j - rb)] = prow[c];
         }
         if (w == 0 && t == 0) ipiv[row0 + j] = row0 + p;
-        __syncthreads();
+        if (!COOP) __syncthreads();
 
         const T d = prow[j];
         const bool mine = has && g > j;
@@ -515,15 +524,29 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
                 idx = g;
             }
             wg_argmax(key, idx, wk, wi, argpar);
-            // bring the candidate's row and row j+1 up to date, publish, then finish the rest
+            // wave 0 brings the candidate's row and row j+1 up to date (a lane per column) and
+            // publishes them; every other row is finished meanwhile
             const int lc = (key > (K)-2) ? idx - rb : -1;
             const int ln = (j + 1 >= rb && j + 1 < rb + nr) ? j + 1 - rb : -1;
-            for (int c = j + 2 + t; c < ncols; c += PT) {
-                if (lc >= 0) S[c * ls + lc] -= S[j * ls + lc] * prow[c];
-                if (ln >= 0 && ln != lc) S[c * ls + ln] -= S[j * ls + ln] * prow[c];
+            if (wave == 0) {
+                T vc = T(0), vn = T(0);
+                if (lane < ncols) {
+                    if (lc >= 0) vc = S[lane * ls + lc];
+                    if (ln >= 0) vn = S[lane * ls + ln];
+                    if (lane >= j + 2) {
+                        if (lc >= 0) {
+                            vc = vc - S[j * ls + lc] * prow[lane];
+                            S[lane * ls + lc] = vc;
+                        }
+                        if (ln >= 0 && ln != lc) {
+                            vn = vn - S[j * ls + ln] * prow[lane];
+                            S[lane * ls + ln] = vn;
+                        }
+                    }
+                    if (ln >= 0 && ln == lc) vn = vc;
+                }
+                publish(j + 1, key, idx, vc, vn);
             }
-            __syncthreads();
-            publish(j + 1, key, idx);
             if (mine && t != lc && t != ln) {
                 const T f = S[j * ls + t];
                 int c = j + 2;

@@ -35,7 +35,8 @@ namespace al {
 namespace {
 
 constexpr int PNB = 64;        // panel width of the real factorizations
-constexpr int PT = 256;        // threads per panel workgroup
+constexpr int PT = 256;        // rows per panel workgroup, one per thread of waves 1-4
+constexpr int PTW = PT + 64;   // threads per panel workgroup: wave 0 exchanges, waves 1-4 hold rows
 constexpr int CHUNK_MAX = 256; // rows per workgroup, one per thread (LDS: 256 x 64 doubles = 128 KiB)
 constexpr int MAX_WG = 128;    // cooperative workgroups per panel (one per CU at most)
 constexpr int NBO = 256;       // outer block (trailing-update depth)
@@ -52,12 +53,10 @@ struct PanelElem<zd> {
     typedef double key_t;
     static constexpr int NB = 32;
 };
-// single-workgroup exchange slot (LDS): key, index, candidate row, row j
-template <typename T>
-constexpr int slot2() { return 2 + 2 * PanelElem<T>::NB; }
 // cooperative exchange slot (global memory), in 64-bit words that each carry a 32-bit part of a
 // value in the low half and the column's generation in the high half: the key's parts, the row
-// index, the candidate row's parts, row j's parts
+// index, then the candidate row and row j, each as its columns' first parts, then their second
+// parts, and so on (a store or load of one part for 64 columns covers 512 contiguous bytes)
 template <typename T>
 constexpr int slot_row() { return (int)(sizeof(typename PanelElem<T>::key_t) / 4) + 1; }
 template <typename T>
@@ -84,11 +83,6 @@ __device__ __forceinline__ double pivot_key(zd v, int g, int j) {
     return a;
 }
 
-// A slot's key and index travel as elements of the panel's type (the index is exact: rows
-// stay below 2^24); a complex slot carries them in the real part.
-__device__ __forceinline__ double slot_val(double v) { return v; }
-__device__ __forceinline__ float slot_val(float v) { return v; }
-__device__ __forceinline__ double slot_val(zd v) { return v.re; }
 
 // Order-preserving integer image of a pivot key: non-negative keys (|v| or +inf) keep their
 // IEEE order as unsigned bit patterns; the sentinels -1 (a skipped NaN) and -2 (no candidate)
@@ -139,23 +133,25 @@ __device__ __forceinline__ zd ld_agent(const zd* p) {
 __device__ __forceinline__ unsigned long long ld_flag(const unsigned long long* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// A value as generation-tagged words (one per 32-bit part), and back: a 64-bit relaxed atomic is
-// single-copy atomic, so a word whose tag is the current column's holds this column's part.
+// A value as generation-tagged words (one per 32-bit part, `stride` words apart), and back: a
+// 64-bit relaxed atomic is single-copy atomic, so a word whose tag is the current column's holds
+// this column's part.
 template <typename V>
-__device__ __forceinline__ void put_tagged(unsigned long long* p, V v, unsigned tag) {
+__device__ __forceinline__ void put_tagged(unsigned long long* p, V v, unsigned tag, int stride = 1) {
     unsigned u[sizeof(V) / 4];
     __builtin_memcpy(u, &v, sizeof(V));
 #pragma unroll
     for (int h = 0; h < (int)(sizeof(V) / 4); ++h)
-        __hip_atomic_store(&p[h], ((unsigned long long)tag << 32) | u[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&p[h * stride], ((unsigned long long)tag << 32) | u[h], __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
 }
 template <typename V>
-__device__ __forceinline__ bool get_tagged(const unsigned long long* p, V& v, unsigned tag) {
+__device__ __forceinline__ bool get_tagged(const unsigned long long* p, V& v, unsigned tag, int stride = 1) {
     unsigned u[sizeof(V) / 4];
     bool ok = true;
 #pragma unroll
     for (int h = 0; h < (int)(sizeof(V) / 4); ++h) {
-        const unsigned long long x = __hip_atomic_load(&p[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long x = __hip_atomic_load(&p[h * stride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         ok &= (unsigned)(x >> 32) == tag;
         u[h] = (unsigned)x;
     }
@@ -227,11 +223,11 @@ __device__ __forceinline__ void wave_argmax(K& key, int& idx) {
 }
 
 // Workgroup-wide first maximum in every thread: the largest key, ties to the lowest row.  The
-// wave's winner by DPP, then the four waves' winners in LDS behind one barrier; `par`
+// wave's winner by DPP, then the waves' winners in LDS behind one barrier; `par`
 // alternates between calls (callers pass a workgroup barrier between two calls, so a call's
 // writes never meet the previous call's reads).
 template <typename K>
-__device__ __forceinline__ void wg_argmax(K& key, int& idx, K (*wk)[PT / 64], int (*wi)[PT / 64], int& par) {
+__device__ __forceinline__ void wg_argmax(K& key, int& idx, K (*wk)[PTW / 64], int (*wi)[PTW / 64], int& par) {
     wave_argmax(key, idx);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0) {
@@ -242,7 +238,7 @@ __device__ __forceinline__ void wg_argmax(K& key, int& idx, K (*wk)[PT / 64], in
     key = wk[par][0];
     idx = wi[par][0];
 #pragma unroll
-    for (int q = 1; q < PT / 64; ++q)
+    for (int q = 1; q < PTW / 64; ++q)
         if (better(wk[par][q], wi[par][q], key, idx)) {
             key = wk[par][q];
             idx = wi[par][q];
@@ -253,42 +249,44 @@ __device__ __forceinline__ void wg_argmax(K& key, int& idx, K (*wk)[PT / 64], in
 #pragma clang fp contract(off)
 // Factors the m x ncols panel at P (leading dimension lda), whose top-left element is the
 // global diagonal element (row0, row0).  Workgroup w owns panel rows [w*chunk, (w+1)*chunk)
-// (chunk <= 256, one row per thread) and keeps them in LDS for the whole panel.
+// (chunk <= 256), one per thread of waves 1-4, and keeps them in LDS for the whole panel.
+// Wave 0 holds no row: it exchanges the columns' candidates.
 //
 // Column exchange (COOP, several workgroups): wave 0 writes the workgroup's slot -- its best
-// candidate (key, row index) with that row, and row j if it owns it -- with write-through
-// stores, waits until they are acknowledged, and then publishes one 64-bit word for the column
-// (its generation, the candidate's row and the top 32 bits of its key).  Words and slots
-// alternate by column parity: a workgroup overwrites parity p only after every workgroup has
-// published the next column, which each does only after it has finished reading parity p.
-// Wave 0 of every workgroup polls the words of the column's parity (L2-missing loads) until
-// all carry the column's generation; the largest key head names the winner, and only when
-// several workgroups share it are their full keys read from the slots.  Then the winner's row,
-// its full key and row j come from the slots in one round trip.  No L2 writeback or invalidate
-// is involved.  Every spin is bounded by `timeout` ticks of the 100 MHz wall clock; on expiry
-// the abort flag stops every workgroup and info is set to -1.  A single workgroup exchanges
-// through LDS only.
+// candidate (key, row index) with that row, and row j if it owns it -- as generation-tagged
+// words, and publishes one 64-bit word for the column (its generation, the candidate's row and
+// the top 32 bits of its key) without waiting for the slot's stores.  Words and slots alternate
+// by column parity: a workgroup overwrites parity p only after every workgroup has published the
+// next column, which each does only after it has finished reading parity p.  Wave 0 of every
+// workgroup polls the words of the column's parity (L2-missing loads) until all carry the
+// column's generation; the largest key head names the winner, and only when several workgroups
+// share it are their full keys read from the slots.  Then the winner's row and row j come from
+// the slots in one round trip, read again until every word carries the column's tag.  No L2
+// writeback or invalidate is involved.  Every spin is bounded by `timeout` ticks of the 100 MHz
+// wall clock; on expiry the abort flag stops every workgroup and info is set to -1.  A single
+// workgroup takes its own candidate without an exchange.
 //
-// Pipelining: after the swap, column j+1 is eliminated first, the next candidate is chosen,
-// its row (and row j+1) are brought up to date, and the slot is published before the rest of
-// the elimination, which then overlaps the other workgroups' publication latency.
+// Pipelining: once column j's multipliers are formed, column j+1 is updated and the next
+// candidate chosen (one barrier), wave 0 brings the candidate's row and row j+1 up to date,
+// publishes them, waits for column j+1's exchange and swaps the two rows involved, while waves
+// 1-4 finish column j's elimination on every other row.  The rows wave 0 swaps are the two it
+// brought up to date (the global winner is some workgroup's candidate), so the two never touch
+// the same row; the pivot row is double-buffered by column parity.
 template <typename T, bool COOP>
-__global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda, int row0, int m, int ncols,
-                                                  int chunk, int* __restrict__ ipiv, int* __restrict__ info,
-                                                  unsigned long long* __restrict__ flags,
-                                                  unsigned* __restrict__ abort_flag,
-                                                  unsigned long long* __restrict__ slots, unsigned gen0,
-                                                  long long timeout) {
+__global__ __launch_bounds__(PTW) void panel_getf2(T* __restrict__ P, size_t lda, int row0, int m, int ncols,
+                                                   int chunk, int* __restrict__ ipiv, int* __restrict__ info,
+                                                   unsigned long long* __restrict__ flags,
+                                                   unsigned* __restrict__ abort_flag,
+                                                   unsigned long long* __restrict__ slots, unsigned gen0,
+                                                   long long timeout) {
     typedef typename PanelElem<T>::key_t K;
     constexpr int NB = PanelElem<T>::NB;
-    constexpr int SLOT2 = slot2<T>();
     constexpr int SW = slot_words<T>(), SR = slot_row<T>(), H = (int)(sizeof(T) / 4);
     extern __shared__ __align__(16) unsigned char smem[];
     T* S = (T*)smem;  // chunk x NB, column-major with leading dimension chunk | 1
-    __shared__ T lslot[2][SLOT2];  // single-workgroup exchange
-    __shared__ T prow_c[NB];
-    __shared__ K wk[2][PT / 64];
-    __shared__ int wi[2][PT / 64];
+    __shared__ T prow_c[2][NB];  // the pivot row of column j in prow_c[j & 1]
+    __shared__ K wk[2][PTW / 64];
+    __shared__ int wi[2][PTW / 64];
     __shared__ int win_lds;
     __shared__ int ok_lds;
 
@@ -299,181 +297,163 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
     const int rb = w * chunk;
     const int ls = chunk | 1;  // LDS column stride: odd, so a row's entries (stride ls) fall in different banks
     const int nr = (m - rb) < chunk ? ((m - rb) > 0 ? (m - rb) : 0) : chunk;
-    const bool has = t < nr;  // this thread's row: panel row rb + t
-    const int g = rb + t;
+    const int tr = t - 64;  // this thread's row (waves 1-4): panel row rb + tr
+    const bool has = tr >= 0 && tr < nr;
+    const int g = rb + tr;
     int argpar = 0;
 
     for (int c = 0; c < ncols; ++c)
-        if (has) S[c * ls + t] = P[(size_t)g + (size_t)c * lda];
+        if (has) S[c * ls + tr] = P[(size_t)g + (size_t)c * lda];
     __syncthreads();
 
     auto slot_ptr = [&](int par, int q) -> unsigned long long* { return slots + ((size_t)par * nwg + q) * SW; };
-    // wave 0 writes this workgroup's slot for column jj and publishes its word; lane c holds
+    // wave 0, several workgroups: this workgroup's slot for column jj and its word; lane c holds
     // column c of the candidate row (vc) and of row jj (vn, when this workgroup owns row jj)
     auto publish = [&](int jj, K key, int idx, T vc, T vn) {
+        if (!COOP) return;
         const bool cand = key > (K)-2;
         const bool ownj = jj >= rb && jj < rb + nr;
         const bool col = lane < ncols;  // ncols <= NB <= 64: one column per lane
-        if (COOP) {
-            // tagged words need no acknowledgement before the publication word: a reader that
-            // sees the word before the slot's words have landed sees old tags and reads again
-            unsigned long long* my = slot_ptr(jj & 1, w);
-            const unsigned tag = gen0 + (unsigned)jj + 1u;
-            if (cand && col) put_tagged(&my[SR + lane * H], vc, tag);
-            if (ownj && col) put_tagged(&my[SR + (NB + lane) * H], vn, tag);
-            if (lane == 0) {
-                put_tagged(&my[0], key, tag);
-                put_tagged(&my[SR - 1], (unsigned)idx, tag);
-                __hip_atomic_store(&flags[w * 8 + (jj & 1)], flag_word(tag, idx, key_head(key)), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-            }
-        } else {
-            T* my = &lslot[jj & 1][0];
-            if (cand && col) my[2 + lane] = vc;
-            if (ownj && col) my[2 + NB + lane] = vn;
-            if (lane == 0) {
-                my[0] = T(key);
-                my[1] = T((K)idx);
-            }
+        // tagged words need no acknowledgement before the publication word: a reader that sees
+        // the word before the slot's words have landed sees old tags and reads again
+        unsigned long long* my = slot_ptr(jj & 1, w);
+        const unsigned tag = gen0 + (unsigned)jj + 1u;
+        if (cand && col) put_tagged(&my[SR + lane], vc, tag, NB);
+        if (ownj && col) put_tagged(&my[SR + H * NB + lane], vn, tag, NB);
+        if (lane == 0) {
+            put_tagged(&my[0], key, tag);
+            put_tagged(&my[SR - 1], (unsigned)idx, tag);
+            __hip_atomic_store(&flags[w * 8 + (jj & 1)], flag_word(tag, idx, key_head(key)), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
         }
     };
-    // everyone: the winner of column jj with its row in prow and row jj in rj; false on abort
-    auto exchange = [&](int jj, const T*& prow, const T*& rj) -> bool {
-        const int par = jj & 1;
+    // wave 0: column jj's winner (a single workgroup's winner is its candidate idx with row vc),
+    // the two rows swapped where this workgroup holds them, the pivot row in prow_c[jj & 1] and
+    // its index in win_lds; ok_lds = 0 on abort
+    auto exchange = [&](int jj, int idx, T vc, T vn) {
+        int ok = 1, bidx = idx;
+        T pv = vc, rv = vn;
         if (COOP) {
-            if (wave == 0) {
-                const unsigned tag = gen0 + (unsigned)jj + 1u;
-                const unsigned long long want = (unsigned long long)(tag & 0xFFFFu);
-                const long long t0 = wall_clock64();
-                // lane 0 checks the abort flag and the time limit; the wave leaves together
-                auto give_up = [&]() -> bool {
-                    // a short wait only polls: the abort flag is read once the wait passes 20 us
-                    const long long el = wall_clock64() - t0;
-                    if (el < 2000 && el <= timeout) return false;
-                    int stop = 0;
-                    if (lane == 0) {
-                        if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
-                            stop = 1;
-                        } else if (wall_clock64() - t0 > timeout) {
-                            __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                            stop = 1;
-                        }
-                    }
-                    return __shfl(stop, 0, 64) != 0;
-                };
-                int ok = 1;
-                unsigned long long f0 = 0, f1 = 0;
-                for (;;) {
-                    f0 = lane < nwg ? ld_flag(&flags[lane * 8 + par]) : 0ull;
-                    f1 = lane + 64 < nwg ? ld_flag(&flags[(lane + 64) * 8 + par]) : 0ull;
-                    const bool mine = (lane >= nwg || (f0 >> 48) == want) && (lane + 64 >= nwg || (f1 >> 48) == want);
-                    if (__all(mine)) break;
-                    if (give_up()) {
-                        ok = 0;
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                }
-                int bq = 0, bidx = 0;  // the winning workgroup and its candidate row
-                if (ok) {
-                    const unsigned h0 = lane < nwg ? (unsigned)f0 : 0u;
-                    const unsigned h1 = lane + 64 < nwg ? (unsigned)f1 : 0u;
-                    const unsigned hmax = wave_max_u32(h0 > h1 ? h0 : h1);
-                    const bool t0m = lane < nwg && h0 == hmax, t1m = lane + 64 < nwg && h1 == hmax;
-                    const unsigned long long b0 = __ballot(t0m), b1 = __ballot(t1m);
-                    if (__popcll(b0) + __popcll(b1) == 1) {
-                        bq = b0 ? __ffsll((long long)b0) - 1 : 64 + __ffsll((long long)b1) - 1;
-                        const unsigned long long fw = __shfl(bq < 64 ? f0 : f1, bq & 63, 64);
-                        bidx = (int)((fw >> 32) & 0xFFFFu);
-                    } else {
-                        // several workgroups share the largest key head: compare their full keys
-                        for (;;) {
-                            K bk = (K)-3;
-                            int bi = 0x7fffffff;
-                            bool good = true;
-                            for (int q = lane; q < nwg; q += 64) {
-                                if (!(q < 64 ? t0m : t1m)) continue;
-                                const unsigned long long* sq = slot_ptr(par, q);
-                                K kq;
-                                unsigned iq;
-                                good &= get_tagged(&sq[0], kq, tag);
-                                good &= get_tagged(&sq[SR - 1], iq, tag);
-                                if (better(kq, (int)iq, bk, bi)) {
-                                    bk = kq;
-                                    bi = (int)iq;
-                                }
-                            }
-#pragma unroll
-                            for (int off = 32; off >= 1; off >>= 1) {
-                                const K ok2 = __shfl_xor(bk, off, 64);
-                                const int oi = __shfl_xor(bi, off, 64);
-                                if (better(ok2, oi, bk, bi)) {
-                                    bk = ok2;
-                                    bi = oi;
-                                }
-                            }
-                            if (__all(good)) {
-                                bq = bi / chunk;
-                                bidx = bi;
-                                break;
-                            }
-                            if (give_up()) {
-                                ok = 0;
-                                break;
-                            }
-                            __builtin_amdgcn_s_sleep(1);
-                        }
+            const int par = jj & 1;
+            const unsigned tag = gen0 + (unsigned)jj + 1u;
+            const unsigned long long want = (unsigned long long)(tag & 0xFFFFu);
+            const long long t0 = wall_clock64();
+            // lane 0 checks the abort flag and the time limit; the wave leaves together
+            auto give_up = [&]() -> bool {
+                // a short wait only polls: the abort flag is read once the wait passes 20 us
+                const long long el = wall_clock64() - t0;
+                if (el < 2000 && el <= timeout) return false;
+                int stop = 0;
+                if (lane == 0) {
+                    if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+                        stop = 1;
+                    } else if (el > timeout) {
+                        __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        stop = 1;
                     }
                 }
-                if (ok) {
-                    // the winner's row and row jj (one round trip); the winner's index came with its
-                    // word, and its key is the pivot key of the row's entry in column jj
-                    const unsigned long long* win = slot_ptr(par, bq);
-                    const unsigned long long* own = slot_ptr(par, jj / chunk);
-                    T pv = T(0), rv = T(0);
+                return __shfl(stop, 0, 64) != 0;
+            };
+            unsigned long long f0 = 0, f1 = 0;
+            for (;;) {
+                f0 = lane < nwg ? ld_flag(&flags[lane * 8 + par]) : 0ull;
+                f1 = lane + 64 < nwg ? ld_flag(&flags[(lane + 64) * 8 + par]) : 0ull;
+                const bool mine = (lane >= nwg || (f0 >> 48) == want) && (lane + 64 >= nwg || (f1 >> 48) == want);
+                if (__all(mine)) break;
+                if (give_up()) {
+                    ok = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            int bq = 0;  // the winning workgroup
+            if (ok) {
+                const unsigned h0 = lane < nwg ? (unsigned)f0 : 0u;
+                const unsigned h1 = lane + 64 < nwg ? (unsigned)f1 : 0u;
+                const unsigned hmax = wave_max_u32(h0 > h1 ? h0 : h1);
+                const bool t0m = lane < nwg && h0 == hmax, t1m = lane + 64 < nwg && h1 == hmax;
+                const unsigned long long b0 = __ballot(t0m), b1 = __ballot(t1m);
+                if (__popcll(b0) + __popcll(b1) == 1) {
+                    bq = b0 ? __ffsll((long long)b0) - 1 : 64 + __ffsll((long long)b1) - 1;
+                    const unsigned long long fw = __shfl(bq < 64 ? f0 : f1, bq & 63, 64);
+                    bidx = (int)((fw >> 32) & 0xFFFFu);
+                } else {
+                    // several workgroups share the largest key head: compare their full keys
                     for (;;) {
+                        K bk = (K)-3;
+                        int bi = 0x7fffffff;
                         bool good = true;
-                        if (lane < ncols) {
-                            good &= get_tagged(&win[SR + lane * H], pv, tag);
-                            good &= get_tagged(&own[SR + (NB + lane) * H], rv, tag);
+                        for (int q = lane; q < nwg; q += 64) {
+                            if (!(q < 64 ? t0m : t1m)) continue;
+                            const unsigned long long* sq = slot_ptr(par, q);
+                            K kq;
+                            unsigned iq;
+                            good &= get_tagged(&sq[0], kq, tag);
+                            good &= get_tagged(&sq[SR - 1], iq, tag);
+                            if (better(kq, (int)iq, bk, bi)) {
+                                bk = kq;
+                                bi = (int)iq;
+                            }
                         }
-                        if (__all(good)) break;
+#pragma unroll
+                        for (int off = 32; off >= 1; off >>= 1) {
+                            const K ok2 = __shfl_xor(bk, off, 64);
+                            const int oi = __shfl_xor(bi, off, 64);
+                            if (better(ok2, oi, bk, bi)) {
+                                bk = ok2;
+                                bi = oi;
+                            }
+                        }
+                        if (__all(good)) {
+                            bq = bi / chunk;
+                            bidx = bi;
+                            break;
+                        }
                         if (give_up()) {
                             ok = 0;
                             break;
                         }
                         __builtin_amdgcn_s_sleep(1);
                     }
-                    if (ok) {
-                        // swap rows jj and bidx where this workgroup holds them (not before a zero
-                        // pivot: the crate stops before its swap), and keep the pivot row for all
-                        if (pivot_key(lane_val(pv, jj), bidx, jj) > (K)0 && bidx != jj && lane < ncols) {
-                            if (bidx >= rb && bidx < rb + nr) S[lane * ls + (bidx - rb)] = rv;
-                            if (jj >= rb && jj < rb + nr) S[lane * ls + (jj - rb)] = pv;
-                        }
-                        if (lane < ncols) prow_c[lane] = pv;
-                        if (lane == 0) win_lds = bidx;
-                    }
                 }
-                if (lane == 0) ok_lds = ok;
             }
-            __syncthreads();
-            prow = prow_c;
-            rj = prow_c;  // not read: the exchange swapped the rows already
-            return ok_lds != 0;
-        } else {
-            __syncthreads();
-            const T* sl = &lslot[par][0];
-            prow = sl + 2;
-            rj = sl + 2 + NB;
-            if (t == 0) win_lds = (int)slot_val(sl[1]);
-            __syncthreads();
-            return true;
+            if (ok) {
+                // the winner's row and row jj (one round trip, repeated while the slots' words
+                // have not landed); the winner's index came with its word, and its key is the
+                // pivot key of the row's entry in column jj
+                const unsigned long long* win = slot_ptr(par, bq);
+                const unsigned long long* own = slot_ptr(par, jj / chunk);
+                for (;;) {
+                    bool good = true;
+                    if (lane < ncols) {
+                        good &= get_tagged(&win[SR + lane], pv, tag, NB);
+                        good &= get_tagged(&own[SR + H * NB + lane], rv, tag, NB);
+                    }
+                    if (__all(good)) break;
+                    if (give_up()) {
+                        ok = 0;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
         }
+        if (ok) {
+            // swap rows jj and bidx where this workgroup holds them (not before a zero pivot: the
+            // crate stops before its swap), and keep the pivot row for every thread
+            if (pivot_key(lane_val(pv, jj), bidx, jj) > (K)0 && bidx != jj && lane < ncols) {
+                if (bidx >= rb && bidx < rb + nr) S[lane * ls + (bidx - rb)] = rv;
+                if (jj >= rb && jj < rb + nr) S[lane * ls + (jj - rb)] = pv;
+            }
+            if (lane < ncols) prow_c[jj & 1][lane] = pv;
+            if (lane == 0) win_lds = bidx;
+        }
+        if (lane == 0) ok_lds = ok;
     };
 
     {
-        K key = has ? pivot_key(S[t], g, 0) : (K)-2;
+        // column 0: the workgroup's candidate, published and exchanged
+        K key = has ? pivot_key(S[tr], g, 0) : (K)-2;
         int idx = has ? g : 0x7fffffff;
         wg_argmax(key, idx, wk, wi, argpar);
         if (wave == 0) {
@@ -483,31 +463,24 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
                 if (rb == 0 && nr > 0) vn = S[lane * ls];
             }
             publish(0, key, idx, vc, vn);
+            exchange(0, idx, vc, vn);
         }
+        __syncthreads();
     }
 
     for (int j = 0; j < ncols; ++j) {
-        const T* prow;
-        const T* rj;
-        if (!exchange(j, prow, rj)) {
+        if (!ok_lds) {
             if (w == 0 && t == 0) *info = -1;
             break;
         }
+        const T* prow = prow_c[j & 1];
         const int p = win_lds;
         if (!(pivot_key(prow[j], p, j) > (K)0)) {
             // |pivot| <= 0: singular (src/lapack/mod.rs:136-138); a NaN diagonal carries +inf
             if (w == 0 && t == 0) *info = row0 + j + 1;
             break;
         }
-        // swap rows j and p across the panel (a cooperative exchange did it already), record the pivot
-        if (!COOP && p != j) {
-            if (p >= rb && p < rb + nr)
-                for (int c = t; c < ncols; c += PT) S[c * ls + (p - rb)] = rj[c];
-            if (j >= rb && j < rb + nr)
-                for (int c = t; c < ncols; c += PT) S[c * ls + (j - rb)] = prow[c];
-        }
         if (w == 0 && t == 0) ipiv[row0 + j] = row0 + p;
-        if (!COOP) __syncthreads();
 
         const T d = prow[j];
         const bool mine = has && g > j;
@@ -516,19 +489,19 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
             K key = (K)-2;
             int idx = 0x7fffffff;
             if (mine) {
-                const T f = S[j * ls + t] / d;
-                S[j * ls + t] = f;
-                const T v = S[(j + 1) * ls + t] - f * prow[j + 1];
-                S[(j + 1) * ls + t] = v;
+                const T f = S[j * ls + tr] / d;
+                S[j * ls + tr] = f;
+                const T v = S[(j + 1) * ls + tr] - f * prow[j + 1];
+                S[(j + 1) * ls + tr] = v;
                 key = pivot_key(v, g, j + 1);
                 idx = g;
             }
             wg_argmax(key, idx, wk, wi, argpar);
-            // wave 0 brings the candidate's row and row j+1 up to date (a lane per column) and
-            // publishes them; every other row is finished meanwhile
             const int lc = (key > (K)-2) ? idx - rb : -1;
             const int ln = (j + 1 >= rb && j + 1 < rb + nr) ? j + 1 - rb : -1;
             if (wave == 0) {
+                // the candidate's row and row j+1 brought up to date (a lane per column),
+                // published, then column j+1's exchange
                 T vc = T(0), vn = T(0);
                 if (lane < ncols) {
                     if (lc >= 0) vc = S[lane * ls + lc];
@@ -546,27 +519,28 @@ __global__ __launch_bounds__(PT) void panel_getf2(T* __restrict__ P, size_t lda,
                     if (ln >= 0 && ln == lc) vn = vc;
                 }
                 publish(j + 1, key, idx, vc, vn);
-            }
-            if (mine && t != lc && t != ln) {
-                const T f = S[j * ls + t];
+                exchange(j + 1, idx, vc, vn);
+            } else if (mine && tr != lc && tr != ln) {
+                // column j's elimination on every other row
+                const T f = S[j * ls + tr];
                 int c = j + 2;
                 for (; c + 8 <= ncols; c += 8) {
                     T v[8];
 #pragma unroll
-                    for (int q = 0; q < 8; ++q) v[q] = S[(c + q) * ls + t];
+                    for (int q = 0; q < 8; ++q) v[q] = S[(c + q) * ls + tr];
 #pragma unroll
-                    for (int q = 0; q < 8; ++q) S[(c + q) * ls + t] = v[q] - f * prow[c + q];
+                    for (int q = 0; q < 8; ++q) S[(c + q) * ls + tr] = v[q] - f * prow[c + q];
                 }
-                for (; c < ncols; ++c) S[c * ls + t] -= f * prow[c];
+                for (; c < ncols; ++c) S[c * ls + tr] -= f * prow[c];
             }
             __syncthreads();
         } else {
-            if (mine) S[j * ls + t] = S[j * ls + t] / d;
+            if (mine) S[j * ls + tr] = S[j * ls + tr] / d;
             __syncthreads();
         }
     }
     for (int c = 0; c < ncols; ++c)
-        if (has) P[(size_t)g + (size_t)c * lda] = S[c * ls + t];
+        if (has) P[(size_t)g + (size_t)c * lda] = S[c * ls + tr];
 }
 #pragma clang fp contract(on)
 
@@ -605,11 +579,11 @@ int launch_panel(hipStream_t s, Workspace* ws, T* Ppanel, size_t lda, size_t row
     ws->gen += (unsigned)ncols;
     const int ticket = prof_begin(s);
     if (nwg > 1) {
-        hipLaunchKernelGGL((panel_getf2<T, true>), dim3((unsigned)nwg), dim3(PT), (chunk | 1) * NB * sizeof(T), s, Ppanel,
+        hipLaunchKernelGGL((panel_getf2<T, true>), dim3((unsigned)nwg), dim3(PTW), (chunk | 1) * NB * sizeof(T), s, Ppanel,
                            lda, (int)row0, (int)m, (int)ncols, (int)chunk, ipiv, info, (unsigned long long*)ws->flags, ws->abort_flag,
                            (unsigned long long*)ws->slots, gen0, timeout);
     } else {
-        hipLaunchKernelGGL((panel_getf2<T, false>), dim3(1), dim3(PT), (chunk | 1) * NB * sizeof(T), s, Ppanel, lda,
+        hipLaunchKernelGGL((panel_getf2<T, false>), dim3(1), dim3(PTW), (chunk | 1) * NB * sizeof(T), s, Ppanel, lda,
                            (int)row0, (int)m, (int)ncols, (int)chunk, ipiv, info, (unsigned long long*)ws->flags, ws->abort_flag,
                            (unsigned long long*)ws->slots, gen0, timeout);
     }

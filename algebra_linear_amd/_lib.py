@@ -135,6 +135,8 @@ SIGNATURES = {
                                 ctypes.POINTER(IrStats), _v]),
     "al_debug_set_wait_limit": (None, [ctypes.c_longlong]),
     "al_debug_coop_units_in_use": (_i, []),
+    "al_debug_panel_pin_half": (_i, [_v]),
+    "al_debug_set_panel_pinning": (None, [_i]),
     "al_prof_enable": (None, [_i]),
     "al_prof_reset": (None, []),
     "al_prof_query": (_i, [ctypes.c_char_p, ctypes.POINTER(_i), ctypes.POINTER(_d), ctypes.POINTER(_d),

@@ -77,6 +77,9 @@ struct Workspace {
     unsigned gen = 0;              // last generation used on `flags` (host side, monotonic)
     double* slots = nullptr;       // panel exchange area
     size_t slots_bytes = 0;
+    int pin_half = -1;             // cooperative panels on XCDs 4*pin_half..+3 (-1: anywhere)
+    size_t pin_flags = 0;          // pinned: word offset of the publication words in `slots`
+    unsigned* slot_tab = nullptr;  // pinned: word offset of each exchange slot in `slots`
     int* ipiv32 = nullptr;         // int32 pivots on the device
     size_t ipiv_len = 0;
     int* pairs = nullptr;          // composed row permutation of one pivot block
@@ -114,6 +117,11 @@ int workspace_on(hipStream_t stream, Workspace** out);
 
 // Grows the one-launch triangular solve's flag array (entries, zero-filled when new).
 int ws_reserve_flags(Workspace* ws, size_t entries);
+
+// Clears the cooperative panels' exchange area and decides whether this workspace's panels run
+// on one half of the chip (a short calibration launch on `s`, then a wait for it).
+int panel_exchange_init(hipStream_t s, Workspace* ws);
+void set_panel_pinning(bool on);  // false: cooperative panels run anywhere (test hook)
 
 // One-launch blocked triangular solve of op = lower/upper, unit or not, with nrhs <= 16
 // right-hand sides in X (n x nrhs, FP64); the matrix may be FP64 or FP32.

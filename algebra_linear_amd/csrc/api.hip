@@ -38,6 +38,7 @@ struct ThreadCtx {
             Workspace& w = kv.second;
             if (w.info) (void)hipFree(w.info);
             if (w.slots) (void)hipFree(w.slots);
+            if (w.slot_tab) (void)hipFree(w.slot_tab);
             if (w.flags) (void)hipFree(w.flags);
             if (w.ipiv32) (void)hipFree(w.ipiv32);
             if (w.pairs) (void)hipFree(w.pairs);
@@ -190,6 +191,7 @@ int workspace_for(hipStream_t s, Workspace** out, size_t ipiv_len) {
         w.gen = 0;
         w.slots_bytes = panel_slots_bytes();
         AL_HIP(hipMalloc((void**)&w.slots, w.slots_bytes));
+        AL_CHECK(panel_exchange_init(s, &w));
         AL_HIP(hipMalloc((void**)&w.pairs, (1 + 2 * 512) * sizeof(int)));
         AL_HIP(hipMalloc((void**)&w.pairs2, (1 + 2 * 512) * sizeof(int)));
         w.device = g_ctx.device;
@@ -858,6 +860,16 @@ int al_debug_coop_units_in_use(void) {
     if (ensure_device() != AL_OK) return 0;
     return coop_in_use();
 }
+
+int al_debug_panel_pin_half(void* stream) {
+    hipStream_t s;
+    if (dev_stream(stream, &s) != AL_OK) return -1;
+    Workspace* ws;
+    if (workspace_for(s, &ws, 1) != AL_OK) return -1;
+    return ws->pin_half;
+}
+
+void al_debug_set_panel_pinning(int on) { set_panel_pinning(on != 0); }
 
 int al_dev_dlaswp(size_t ncols, double* A, size_t lda, size_t k0, size_t k1, const int32_t* d_ipiv, void* stream) {
     if (k1 < k0) return bad_shape("dlaswp: k1 < k0");

@@ -40,6 +40,14 @@ constexpr int PTW = PT + 64;   // threads per panel workgroup: wave 0 exchanges,
 constexpr int CHUNK_MAX = 256; // rows per workgroup, one per thread (LDS: 256 x 64 doubles = 128 KiB)
 constexpr int MAX_WG = 128;    // cooperative workgroups per panel (one per CU at most)
 constexpr int NBO = 256;       // outer block (trailing-update depth)
+std::atomic<bool> g_pinning{true};  // calibrated pinning of the cooperative panels allowed
+// Pinned exchange layout (see panel_exchange_init): the publication words in one 8 KiB block of
+// the exchange area and three slots of SLOT_PAD words in each of XUSED - 1 others, the blocks
+// measured nearest to the chosen half of the chip among XBLOCKS candidates.
+constexpr int XBLK_WORDS = 1024;                  // 8 KiB
+constexpr int SLOT_PAD = 260;                     // >= every element type's slot_words
+constexpr int XUSED = 1 + (2 * MAX_WG + 2) / 3;   // blocks the pinned layout uses
+constexpr int XBLOCKS = 6 * XUSED;                // candidate blocks measured
 
 // Per element type: the pivot key's type and the panel width that keeps a CHUNK_MAX-row
 // chunk of the panel at 128 KiB of LDS (64 real columns, 32 complex ones).
@@ -278,7 +286,8 @@ __global__ __launch_bounds__(PTW) void panel_getf2(T* __restrict__ P, size_t lda
                                                    unsigned long long* __restrict__ flags,
                                                    unsigned* __restrict__ abort_flag,
                                                    unsigned long long* __restrict__ slots, unsigned gen0,
-                                                   long long timeout) {
+                                                   long long timeout, int nwg_pin, int pin,
+                                                   const unsigned* __restrict__ slot_tab) {
     typedef typename PanelElem<T>::key_t K;
     constexpr int NB = PanelElem<T>::NB;
     constexpr int SW = slot_words<T>(), SR = slot_row<T>(), H = (int)(sizeof(T) / 4);
@@ -289,9 +298,18 @@ __global__ __launch_bounds__(PTW) void panel_getf2(T* __restrict__ P, size_t lda
     __shared__ int wi[2][PTW / 64];
     __shared__ int win_lds;
     __shared__ int ok_lds;
+    __shared__ unsigned tab[2 * MAX_WG];  // pinned: word offset of slot (par, q) at par * MAX_WG + q
 
-    const int nwg = COOP ? (int)gridDim.x : 1;
-    const int w = COOP ? (int)blockIdx.x : 0;
+    // pin >= 0: only the workgroups dispatched to that half of the chip (XCDs 4*pin .. 4*pin+3;
+    // workgroups go to XCD blockIdx % 8) take part, nwg_pin of them; the others leave at once
+    int nwg = COOP ? (int)gridDim.x : 1, w = COOP ? (int)blockIdx.x : 0;
+    if (COOP && pin >= 0) {
+        const int xb = (int)(blockIdx.x & 7);
+        if ((xb >> 2) != pin) return;
+        w = (int)(blockIdx.x >> 3) * 4 + (xb & 3);
+        if (w >= nwg_pin) return;
+        nwg = nwg_pin;
+    }
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     if (*info != 0) return;  // an earlier panel hit a zero pivot or failed
     const int rb = w * chunk;
@@ -304,9 +322,14 @@ __global__ __launch_bounds__(PTW) void panel_getf2(T* __restrict__ P, size_t lda
 
     for (int c = 0; c < ncols; ++c)
         if (has) S[c * ls + tr] = P[(size_t)g + (size_t)c * lda];
+    if (COOP && pin >= 0)
+        for (int i = t; i < 2 * MAX_WG; i += PTW) tab[i] = slot_tab[i];
     __syncthreads();
 
-    auto slot_ptr = [&](int par, int q) -> unsigned long long* { return slots + ((size_t)par * nwg + q) * SW; };
+    auto slot_ptr = [&](int par, int q) -> unsigned long long* {
+        if (pin >= 0) return slots + tab[par * MAX_WG + q];
+        return slots + ((size_t)par * nwg + q) * SW;
+    };
     // wave 0, several workgroups: this workgroup's slot for column jj and its word; lane c holds
     // column c of the candidate row (vc) and of row jj (vn, when this workgroup owns row jj)
     auto publish = [&](int jj, K key, int idx, T vc, T vn) {
@@ -578,14 +601,22 @@ int launch_panel(hipStream_t s, Workspace* ws, T* Ppanel, size_t lda, size_t row
     const unsigned gen0 = ws->gen;
     ws->gen += (unsigned)ncols;
     const int ticket = prof_begin(s);
-    if (nwg > 1) {
+    if (nwg > 1 && ws->pin_half >= 0 && g_pinning.load(std::memory_order_relaxed)) {
+        // every workgroup on one half of the chip, the exchange area's words in memory near it;
+        // eight workgroups per four taking part (the other half's leave at once)
+        const unsigned grid = 8u * (unsigned)((nwg + 3) / 4);
+        unsigned long long* area = (unsigned long long*)ws->slots;
+        hipLaunchKernelGGL((panel_getf2<T, true>), dim3(grid), dim3(PTW), (chunk | 1) * NB * sizeof(T), s, Ppanel, lda,
+                           (int)row0, (int)m, (int)ncols, (int)chunk, ipiv, info, area + ws->pin_flags,
+                           ws->abort_flag, area, gen0, timeout, (int)nwg, ws->pin_half, ws->slot_tab);
+    } else if (nwg > 1) {
         hipLaunchKernelGGL((panel_getf2<T, true>), dim3((unsigned)nwg), dim3(PTW), (chunk | 1) * NB * sizeof(T), s, Ppanel,
                            lda, (int)row0, (int)m, (int)ncols, (int)chunk, ipiv, info, (unsigned long long*)ws->flags, ws->abort_flag,
-                           (unsigned long long*)ws->slots, gen0, timeout);
+                           (unsigned long long*)ws->slots, gen0, timeout, (int)nwg, -1, nullptr);
     } else {
         hipLaunchKernelGGL((panel_getf2<T, false>), dim3(1), dim3(PTW), (chunk | 1) * NB * sizeof(T), s, Ppanel, lda,
                            (int)row0, (int)m, (int)ncols, (int)chunk, ipiv, info, (unsigned long long*)ws->flags, ws->abort_flag,
-                           (unsigned long long*)ws->slots, gen0, timeout);
+                           (unsigned long long*)ws->slots, gen0, timeout, 1, -1, nullptr);
     }
     AL_HIP(hipGetLastError());
     if (ticket >= 0) {
@@ -705,7 +736,8 @@ PanelLedger* ledger_of_current() {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return nullptr;
     PanelLedger* L = &g_ledgers[dev];
-    if (L->ledger.capacity() == 0) L->ledger.set_capacity(cu_count());
+    // half the CUs: a panel pinned to one half of the chip must find all its workgroups' CUs there
+    if (L->ledger.capacity() == 0) L->ledger.set_capacity(std::max(1, cu_count() / 2));
     return L;
 }
 
@@ -939,7 +971,119 @@ int getrs_f64(hipStream_t s, Workspace* ws, size_t n, size_t nrhs, const double*
 
 size_t panel_slots_bytes() {
     const int words = std::max(slot_words<double>(), std::max(slot_words<float>(), slot_words<zd>()));
-    return (size_t)2 * MAX_WG * (size_t)words * sizeof(unsigned long long);
+    static_assert(slot_words<double>() <= SLOT_PAD && slot_words<zd>() <= SLOT_PAD && 3 * SLOT_PAD <= XBLK_WORDS,
+                  "three slots per 8 KiB block");
+    static_assert(MAX_WG * 8 <= XBLK_WORDS, "the publication words fit one block");
+    const size_t plain = (size_t)2 * MAX_WG * (size_t)words * sizeof(unsigned long long);
+    const size_t pinned = (size_t)XBLOCKS * XBLK_WORDS * sizeof(unsigned long long);
+    return std::max(plain, pinned);
+}
+
+namespace {
+
+// Round trips between two workgroups on XCD 0 (workgroups 0 and 8) and, at the same time, two on
+// XCD 4 (workgroups 4 and 12), through relaxed agent-scope words in each 8 KiB block of the
+// exchange area; every workgroup records the XCD it runs on.  Each spin is bounded; a block whose
+// round trips did not complete records -1 and raises a stop word that ends every pair's loop.
+__global__ __launch_bounds__(64) void xcd_calib_k(unsigned long long* __restrict__ area, int nblk, int rts,
+                                                  long long* __restrict__ out, unsigned* __restrict__ xcc) {
+    const int g = (int)blockIdx.x;
+    if (threadIdx.x != 0) return;
+    xcc[g] = (unsigned)__builtin_amdgcn_s_getreg((3 << 11) | 20) & 15u;  // HW_REG_XCC_ID
+    if ((g & 3) != 0 || g >= 16) return;
+    const int pair = (g >> 2) & 1;  // 0: XCD 0, 1: XCD 4
+    const bool lead = g < 8;
+    unsigned long long* stop = area + 512;
+    for (int b = 0; b < nblk; ++b) {
+        if (__hip_atomic_load(stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0ull) break;
+        unsigned long long* fa = area + (size_t)b * XBLK_WORDS + pair * 64;
+        unsigned long long* fb = fa + 16;
+        const long long t0 = wall_clock64();
+        bool bad = false;
+        for (int i = 0; i < rts && !bad; ++i) {
+            const unsigned long long v = ((unsigned long long)(b + 1) << 32) | (unsigned long long)(2 * i + 1);
+            long long spins = 0;
+            if (lead) {
+                __hip_atomic_store(fa, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                while (__hip_atomic_load(fb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != v)
+                    if (++spins > 100000) {
+                        bad = true;
+                        break;
+                    }
+            } else {
+                while (__hip_atomic_load(fa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != v)
+                    if (++spins > 100000) {
+                        bad = true;
+                        break;
+                    }
+                __hip_atomic_store(fb, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        if (lead) out[pair * nblk + b] = bad ? -1 : wall_clock64() - t0;
+        if (bad) {
+            __hip_atomic_store(stop, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+    }
+}
+
+}  // namespace
+
+void set_panel_pinning(bool on) { g_pinning.store(on, std::memory_order_relaxed); }
+
+// Decides whether the cooperative panels of this workspace run on one half of the chip.  The
+// agent-scope words of the exchange travel to the memory that holds them, and an 8 KiB block of
+// the area is nearer to some XCDs than to others (round trips of about 0.6, 0.8 or 1.05 us between
+// two workgroups of one XCD, by block); a column's exchange is a few such trips.  A calibration
+// launch measures every block from XCD 0 and from XCD 4.  If workgroup i runs on XCD i % 8, the
+// half (XCDs 0-3 or 4-7) that more blocks are clearly nearer to is chosen, and its XUSED nearest
+// blocks hold the publication words and the slots (ws->pin_flags, ws->slot_tab); with too few
+// such blocks the panels run anywhere.  Then the area is cleared.
+int panel_exchange_init(hipStream_t s, Workspace* ws) {
+    ws->pin_half = -1;
+    unsigned long long* area = (unsigned long long*)ws->slots;
+    AL_CHECK(fill_u32(s, (unsigned*)area, ws->slots_bytes / sizeof(unsigned), 0u));
+    if (ws->slots_bytes >= (size_t)XBLOCKS * XBLK_WORDS * sizeof(unsigned long long)) {
+        const int nblk = XBLOCKS;
+        long long* d_out = nullptr;
+        AL_HIP(hipMalloc((void**)&d_out, 2 * (size_t)nblk * sizeof(long long) + 64 * sizeof(unsigned)));
+        unsigned* d_xcc = (unsigned*)(d_out + 2 * nblk);
+        std::vector<long long> t(2 * (size_t)nblk, 0);
+        std::vector<unsigned> xcc(16, 99u);
+        hipLaunchKernelGGL(xcd_calib_k, dim3(16), dim3(64), 0, s, area, nblk, 12, d_out, d_xcc);
+        hipError_t r = hipGetLastError();
+        if (r == hipSuccess) r = hipMemcpyAsync(t.data(), d_out, t.size() * sizeof(long long), hipMemcpyDeviceToHost, s);
+        if (r == hipSuccess) r = hipMemcpyAsync(xcc.data(), d_xcc, 16 * sizeof(unsigned), hipMemcpyDeviceToHost, s);
+        if (r == hipSuccess) r = hipStreamSynchronize(s);
+        (void)hipFree(d_out);
+        AL_HIP(r);
+        bool ok = true;
+        for (int g = 0; g < 16; ++g) ok = ok && xcc[g] == (unsigned)(g % 8);
+        for (int b = 0; b < nblk && ok; ++b) ok = t[b] > 0 && t[nblk + b] > 0;
+        if (ok) {
+            // blocks at least 10% nearer to one half, by half, nearest first
+            std::vector<std::pair<long long, int>> near[2];
+            for (int b = 0; b < nblk; ++b) {
+                const long long a = t[b], c = t[nblk + b];
+                if (a * 10 < c * 9) near[0].push_back({a, b});
+                if (c * 10 < a * 9) near[1].push_back({c, b});
+            }
+            const int h = near[1].size() > near[0].size() ? 1 : 0;
+            if ((int)near[h].size() >= XUSED) {
+                std::sort(near[h].begin(), near[h].end());
+                std::vector<unsigned> tab(2 * MAX_WG);
+                for (int k = 0; k < 2 * MAX_WG; ++k)
+                    tab[k] = (unsigned)near[h][1 + k / 3].second * XBLK_WORDS + (unsigned)(k % 3) * SLOT_PAD;
+                if (!ws->slot_tab) AL_HIP(hipMalloc((void**)&ws->slot_tab, tab.size() * sizeof(unsigned)));
+                AL_HIP(hipMemcpy(ws->slot_tab, tab.data(), tab.size() * sizeof(unsigned), hipMemcpyHostToDevice));
+                ws->pin_flags = (size_t)near[h][0].second * XBLK_WORDS;
+                ws->pin_half = h;
+            }
+        }
+        AL_CHECK(fill_u32(s, (unsigned*)area, ws->slots_bytes / sizeof(unsigned), 0u));
+    }
+    AL_HIP(hipStreamSynchronize(s));
+    return AL_OK;
 }
 size_t panel_flags_count() { return (size_t)MAX_WG * 16; }
 

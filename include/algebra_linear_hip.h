@@ -394,11 +394,20 @@ int al_prof_query(const char* kernel_class, int* launches, double* total_ms, dou
  * (<= 0 restores the default), so a test can make them give up from the host and check that
  * every entry point reports AL_ERR_DEVICE.  al_debug_coop_units_in_use returns the
  * cooperative-panel workgroups currently reserved by unfinished FP64 factorizations: their sum
- * is kept at or below the device's CU count, so concurrent factorizations on several streams
- * cannot fill the device with panels that wait on each other.
+ * is kept at or below half the device's CU count (a panel may run on one half of the chip), so
+ * concurrent factorizations on several streams cannot fill the device with panels that wait on
+ * each other.
  * ------------------------------------------------------------------------------------ */
 void al_debug_set_wait_limit(long long ticks);
 int al_debug_coop_units_in_use(void);
+/* The cooperative panels of a workspace (one per calling thread and stream) run on one half of
+ * the chip (XCDs 0-3 or 4-7) when a calibration at the workspace's creation finds its exchange
+ * words nearer to that half; al_debug_panel_pin_half returns that half for `stream` (creating
+ * the workspace if needed), or -1 when its panels run anywhere.  al_debug_set_panel_pinning(0)
+ * makes every later panel run anywhere, (1) restores the calibrated choice; results are the
+ * same either way. */
+int al_debug_panel_pin_half(void* stream);
+void al_debug_set_panel_pinning(int on);
 
 #ifdef __cplusplus
 }

@@ -158,6 +158,30 @@ def test_dgetrf_rect_is_bit_exact_within_one_panel(nrows, ncols):
     assert np.array_equal(a, lu)
 
 
+@pytest.mark.parametrize("nrows,ncols", [(1000, 64), (2600, 300), (3000, 3000), (16384, 64)])
+def test_pinned_and_unpinned_panels_agree(nrows, ncols):
+    # the cooperative panels give the same pivots and factors whether they run on one half of
+    # the chip with their exchange words near it, or anywhere (4 to 64 workgroups, a count that
+    # is not a multiple of four included)
+    from algebra_linear_amd import _lib
+
+    L = _lib.load()
+    half = L.al_debug_panel_pin_half(None)
+    assert half in (-1, 0, 1)
+    print(f"cooperative panels pinned to half {half}", flush=True)
+    a0 = uniform(nrows * ncols, [5, 6, 7, 9])
+    a1, a2 = a0.copy(), a0.copy()
+    p1, p2 = np.zeros(ncols, dtype=np.uintp), np.zeros(ncols, dtype=np.uintp)
+    lapack.dgetrf_rect(a1, nrows, ncols, p1)
+    L.al_debug_set_panel_pinning(0)
+    try:
+        lapack.dgetrf_rect(a2, nrows, ncols, p2)
+    finally:
+        L.al_debug_set_panel_pinning(1)
+    assert list(p1) == list(p2)
+    assert np.array_equal(a1, a2)
+
+
 @pytest.mark.parametrize("zero_rows", [200, 600])
 def test_dgetrf_rect_negative_zeros_in_a_workgroups_rows(zero_rows):
     # 600 x 64 runs as three cooperating workgroups of 200 rows; column 0 of the first

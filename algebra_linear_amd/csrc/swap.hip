@@ -171,9 +171,13 @@ __global__ void perm_iota(double* perm, size_t n) {
 }
 
 // Row origins of a whole pivot sequence: thread i traces row i backwards through all n swaps
-// (the pivots staged through LDS in chunks), ending at the original row that lands at i.
+// (the pivots staged through LDS in chunks), ending at the original row that lands at i.  The
+// pivots are read eight at a time (two 16-byte LDS loads) and the eight steps applied from
+// registers, in the same order, so the chain waits on LDS once per eight swaps.
+__device__ __forceinline__ int trace_swap(int x, int a, int b) { return (x == a) ? b : (x == b) ? a : x; }
+
 __global__ __launch_bounds__(256) void perm_from_pivots(const int* __restrict__ ipiv, int n, double* __restrict__ perm) {
-    __shared__ int piv[1024];
+    __shared__ __align__(16) int piv[1024];
     const int i = blockIdx.x * 256 + threadIdx.x;
     int x = i;
     const int nchunks = (n + 1023) / 1024;
@@ -183,9 +187,18 @@ __global__ __launch_bounds__(256) void perm_from_pivots(const int* __restrict__ 
         __syncthreads();
         for (int q = threadIdx.x; q < len; q += 256) piv[q] = ipiv[base + q];
         __syncthreads();
-        for (int s = len - 1; s >= 0; --s) {
-            const int a = base + s, b = piv[s];
-            x = (x == a) ? b : (x == b) ? a : x;
+        const int full = len & ~7;
+        for (int s = len - 1; s >= full; --s) x = trace_swap(x, base + s, piv[s]);
+        for (int g = full - 8; g >= 0; g -= 8) {
+            const int4 lo = *(const int4*)&piv[g], hi = *(const int4*)&piv[g + 4];
+            x = trace_swap(x, base + g + 7, hi.w);
+            x = trace_swap(x, base + g + 6, hi.z);
+            x = trace_swap(x, base + g + 5, hi.y);
+            x = trace_swap(x, base + g + 4, hi.x);
+            x = trace_swap(x, base + g + 3, lo.w);
+            x = trace_swap(x, base + g + 2, lo.z);
+            x = trace_swap(x, base + g + 1, lo.y);
+            x = trace_swap(x, base + g, lo.x);
         }
     }
     if (i < n) perm[i] = (double)x;

@@ -269,6 +269,25 @@ def test_dgesv_single_rhs_and_dgetrs_agree_with_oracle():
     assert np.allclose(b2, bo, rtol=1e-10, atol=1e-10)
 
 
+@pytest.mark.parametrize("n", [1, 7, 8, 9, 1023, 1024, 1025, 3001])
+def test_dgetrs_applies_any_pivot_sequence_in_order(n):
+    # the crate's dgetrs swaps b[i] and b[ipiv[i]] for i = 0, 1, ... (src/lapack/mod.rs:330-354)
+    # whatever the pivots are, including ipiv[i] < i; with LU = I the solve is exact, so x is b
+    # with the interchanges applied, bit for bit (lengths with a remainder of 1 and 7 pivots, and
+    # across the 1024-pivot chunks in which the device composes them)
+    rng = np.random.default_rng(n)
+    ipiv = rng.integers(0, n, size=n)
+    lu = np.eye(n).reshape(-1, order="F").copy()
+    b0 = rng.standard_normal(n)
+    want = b0.copy()
+    for i, p in enumerate(ipiv):
+        want[i], want[p] = want[p], want[i]
+    b = b0.copy()
+    lapack.dgetrs(lu, ipiv, b)
+    assert np.array_equal(b, want)
+    assert np.array_equal(want, O.dgetrs(lu, ipiv, b0))
+
+
 @pytest.mark.parametrize("n", [1, 2, 3, 5, 9])
 def test_dgesv_on_the_parity_samples(n):
     # src/parity.rs:152-195: sample(n), x_true = 1 + i/4, TOL 1e-10

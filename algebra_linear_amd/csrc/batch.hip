@@ -621,6 +621,32 @@ __device__ __noinline__ void apply_swaps(T* M, int n, int k0, int kb, LDS& L) {
     }
 }
 
+// U12 := L11^{-1} A12 for the columns right of the KB-column panel at k0, in place: one thread
+// per column with the column's kb entries in registers (row r: its entry minus L11[r, i] times
+// the solved entries i < r, in that order), L11 from the panel in LDS (P, leading dimension m).
+// Out of line, so its registers do not add to the factorization's.
+template <typename T, int KB>
+__device__ __noinline__ void panel_u12(T* M, int n, int k0, int kb, const T* P, int m) {
+    for (int c = k0 + kb + (int)threadIdx.x; c < n; c += NT) {
+        T* pc = M + (size_t)c * n + k0;
+        T u[KB];
+#pragma unroll
+        for (int r = 0; r < KB; ++r) u[r] = r < kb ? pc[r] : (T)0;
+#pragma unroll
+        for (int r = 1; r < KB; ++r) {
+            if (r < kb) {
+                T v = u[r];
+#pragma unroll
+                for (int i = 0; i < r; ++i) v -= P[r + i * m] * u[i];
+                u[r] = v;
+            }
+        }
+#pragma unroll
+        for (int r = 1; r < KB; ++r)
+            if (r < kb) pc[r] = u[r];
+    }
+}
+
 // LU with partial pivoting of the n x n matrix M (leading dimension n); pivots (0-based) in
 // L.ipiv; returns the first zero pivot's 1-based column or 0.
 template <typename T, int KB, typename LDS>
@@ -687,16 +713,7 @@ __device__ int lu_factor(T* M, int n, LDS& L) {
         }
         apply_swaps<T>(M, n, k0, kb, L);
         if (k0 + kb < n) {
-            // U12 := L11^{-1} A12, one thread per column, in place (rolled loops: the column's
-            // entries are re-read from cache; unrolled, the L11 loads would spill)
-            for (int c = k0 + kb + t; c < n; c += NT) {
-                T* pc = M + (size_t)c * n + k0;
-                for (int r = 1; r < kb; ++r) {
-                    T v = pc[r];
-                    for (int i = 0; i < r; ++i) v -= P[r + i * m] * pc[i];
-                    pc[r] = v;
-                }
-            }
+            panel_u12<T, KB>(M, n, k0, kb, P, m);
             __syncthreads();
             trailing_update<T, false, LDS, KB>(M, n, k0, kb, P, m, L);
         }
